@@ -1,0 +1,231 @@
+#!/usr/bin/env python3
+"""Benchmark of the iterative cone-beam hot path on one MI355X.
+
+Workload (the reference's plain-FP32-at-scale configuration): 768x768 detector
+-> 768^3 grid (vx = px*so/sd), 720 views over 360 deg, sit = 20 -> A_s = 36 views
+per sub-iteration, so=100, sd=1000, px=0.1 mm, rx=2.5 px, one module rolled
+0.3 deg, I0 = 5e4 Poisson, weight 1.0, weight_factor 0.9, method_1a schedule
+with slab height 64 (12 slab launches per projector), device-resident.
+
+One step = one steady-state sub-iteration: Forward over the 12 slabs, Error,
+Backward + update over the 12 slabs, sums cleared.  value = voxel-views of
+both projectors (2 * V * A_s) per second of device time.
+
+Synthetic data: seeded phantom of 12 random ellipsoids (512^3 phantom units),
+FP64 analytic line integrals on the GPU, Poisson counts, transmission,
+-log preprocessing by the library's preprocess kernel.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "voxel-proj updates/s vs plain FP32 at fixed RMSE; p50/p99 recon latency"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--dtype", choices=["f32", "f16", "bf16"], default="f32",
+                    help="storage of volume, projections and error (math is FP32)")
+    ap.add_argument("--slab", type=int, default=64)
+    ap.add_argument("--det", type=int, default=768)
+    ap.add_argument("--views", type=int, default=720)
+    ap.add_argument("--sit", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="target length of the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(pd_sit, images_host, W, H, A_s, orig, vx, target_s):
+    """The oracle (C, OpenMP) on a bounded slab of the same sub-iteration."""
+    from oracle import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    rows = images_host.shape[1]
+
+    def run(L):
+        vol = np.ones((L, W, W), np.float32)
+        sums = np.zeros((A_s, rows, W), np.uint64)
+        g = O.make_geom(W, A_s, 1, (0, H // 2 - L // 2, 0), (W, H // 2 - L // 2 + L, W), 0, rows,
+                        orig, (vx, vx, vx))
+        t0 = time.perf_counter()
+        O.forward_f32(vol, sums, pd_sit, g, threads)
+        err = O.error_f32(images_host, sums, 1.0)
+        O.backward_f32(vol, err, pd_sit, g, threads)
+        return time.perf_counter() - t0
+
+    L = 2
+    t = run(L)
+    if t < 0.5 * target_s:
+        L = int(max(2, min(H, L * target_s / max(t, 1e-3))))
+        t = run(L)
+    vv = 2.0 * L * W * W * A_s
+    return {"value": vv / t, "unit": "voxel-views/s", "cores": threads, "kind": "port",
+            "sample": f"one sub-iteration (Forward + Error + Backward, {A_s} views) over a "
+                      f"{L}-layer central slab of the {W}x{H}x{W} grid, FP32 canonical oracle, "
+                      f"{t:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the slab "
+              "decomposition across GPUs is described in DESIGN.md, not run.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    import xpulse_recon_amd as X
+    from xpulse_recon_amd import synth
+    from xpulse_recon_amd.geometry import Geometry
+
+    dev = torch.device("cuda:0")
+    tdt = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16}[args.dtype]
+    sbytes = 4 if args.dtype == "f32" else 2
+    D, N, S = args.det, args.views, args.sit
+    geom = Geometry(dwidth=D, dheight=D, projections=N, so=100.0, sd=1000.0, px=0.1, rx=2.5,
+                    roll=0.3)
+    pd_all, grid = geom.projdata(fp64=False)
+    W, H, vx = grid["vwidth"], grid["vheight"], grid["vx"]
+    A_s = N // S
+    V = W * W * H
+    log(f"grid {W}x{H}x{W}, {A_s} views/sub-iteration, slab {args.slab}, storage {args.dtype}")
+
+    # ---- synthetic projections (FP64 chords -> Poisson -> -log) ------------
+    t0 = time.perf_counter()
+    fov = W * vx
+    ell = synth.random_ellipsoids(1002, 12, fov, 512)
+    images = []
+    for s in range(S):
+        views = list(range(s, N, S))[:A_s]
+        p = synth.line_integrals(geom, views, ell, dev)
+        tr = synth.transmission(p, 5e4, 1003 + s)
+        del p
+        img = torch.empty(tr.shape, dtype=tdt, device=dev)
+        X.preprocess(tr, img, normalize=False, mlog=True)
+        images.append(img)
+        del tr
+    pds = [torch.from_numpy(geom.sit_projdata(pd_all, S, s)).to(dev) for s in range(S)]
+    torch.cuda.synchronize()
+    log(f"synthetic data ready in {time.perf_counter() - t0:.1f} s")
+
+    vol = torch.empty((H, W, W), dtype=tdt, device=dev)
+    X.fill(vol, 1.0)
+    sums = torch.zeros((A_s, D, D), dtype=torch.int64, device=dev)
+    err = torch.empty((A_s, D, D), dtype=tdt, device=dev)
+    slabs = [(l0, min(H, l0 + args.slab)) for l0 in range(0, H, args.slab)]
+    jit = np.random.default_rng(1004).uniform(0.0, vx, (args.warmup + args.steps, 3))
+    weight = 1.0
+
+    def args_for(l0, l1, origin):
+        return X.proj_args(D, A_s, 1, (0, l0, 0), (W, l1, W), 0, D, origin, (vx, vx, vx))
+
+    def step(k, ev=None):
+        s = k % S
+        origin = [np.float32(np.float32(grid["orig"][i]) + np.float32(jit[k][i])) for i in range(3)]
+        sl = [args_for(l0, l1, origin) for (l0, l1) in slabs]
+        if ev: ev[0].record()
+        for (l0, l1), a in zip(slabs, sl):
+            X.forward(vol[l0:l1], sums, pds[s], a)
+        if ev: ev[1].record()
+        X.error(images[s], sums, err, weight)
+        if ev: ev[2].record()
+        for (l0, l1), a in zip(slabs, sl):
+            X.backward(vol[l0:l1], err, pds[s], a)
+        if ev: ev[3].record()
+        X.fill_u64(sums, 0)
+        if ev: ev[4].record()
+
+    for k in range(args.warmup):
+        step(k)
+    torch.cuda.synchronize()
+    log("warmup done")
+
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(args.steps)]
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for k in range(args.steps):
+        step(args.warmup + k, evs[k])
+    stop.record()
+    torch.cuda.synchronize()
+    total_ms = start.elapsed_time(stop)
+    fp_ms = sum(e[0].elapsed_time(e[1]) for e in evs) / args.steps
+    er_ms = sum(e[1].elapsed_time(e[2]) for e in evs) / args.steps
+    bp_ms = sum(e[2].elapsed_time(e[3]) for e in evs) / args.steps
+    fl_ms = sum(e[3].elapsed_time(e[4]) for e in evs) / args.steps
+    ms = total_ms / args.steps
+    if not torch.isfinite(vol.float()).all():
+        raise SystemExit("non-finite volume after the timed steps")
+    log(f"per step {ms:.2f} ms: forward {fp_ms:.2f}, error {er_ms:.2f}, backward {bp_ms:.2f}, "
+        f"clear {fl_ms:.2f}")
+
+    vv_step = 2.0 * V * A_s
+    value = vv_step / (ms * 1e-3)
+    nsl = len(slabs)
+    v_slab = W * W * args.slab
+    fp_bytes = v_slab * A_s * 4 + v_slab * sbytes
+    bp_bytes = v_slab * A_s * sbytes + 2 * v_slab * sbytes
+    fp_launch, bp_launch = fp_ms / nsl, bp_ms / nsl
+    if bp_ms >= fp_ms:
+        name, launch_ms, nbytes = "k_backward", bp_launch, bp_bytes
+    else:
+        name, launch_ms, nbytes = "k_forward", fp_launch, fp_bytes
+    ach = nbytes / (launch_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+            "launch_ms": round(launch_ms, 4),
+            "algorithmic_bytes_per_launch": int(nbytes),
+            "other": {"forward_launch_ms": round(fp_launch, 4),
+                      "forward_frac": round(fp_bytes / (fp_launch * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                      "backward_launch_ms": round(bp_launch, 4),
+                      "backward_frac": round(bp_bytes / (bp_launch * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                      "error_ms": round(er_ms, 4)}}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle) ...")
+        img_host = images[0].float().cpu().numpy()
+        orig = [float(np.float32(grid["orig"][i])) for i in range(3)]
+        cpu = cpu_baseline(pds[0].cpu().numpy(), img_host, W, H, A_s, orig, float(np.float32(vx)),
+                           args.cpu_seconds)
+        log(f"cpu baseline {cpu['value']:.3e} voxel-views/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "voxel-views/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic: 12 random ellipsoids (seed 1002), FP64 analytic line integrals, "
+                "Poisson I0=5e4, -log preprocessed on the GPU",
+        "config": {"workload": f"{W}^3 grid from a {D}x{D} detector, {N} views, sit={S} "
+                               f"(A_s={A_s}), method_1a slab {args.slab} device-resident, one "
+                               f"sub-iteration per step, {args.dtype} storage",
+                   "grid": [W, H, W], "views_per_subiteration": A_s, "slab": args.slab,
+                   "voxel_views_per_step": vv_step},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

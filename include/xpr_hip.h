@@ -1,0 +1,196 @@
+/*
+ * xpr_hip.h - C-ABI of the MI355X (gfx950) iterative cone-beam hot path.
+ *
+ * Plain C: pointers, sizes and POD structs only.  Every device pointer is a
+ * HIP device address owned by the caller; the library keeps no pointer past a
+ * call.  Every call is stateless (all kernel arguments are passed per call),
+ * enqueues on the given stream (NULL = the null stream) and returns an int
+ * status: XPR_OK or a negative XPR_E_* code, with a message in
+ * xpr_last_error() (thread-local).  No C++ exception crosses this boundary.
+ *
+ * Each entry point replaces one piece of the reference's OpenCL operator
+ * API; the citation names the reference interface it stands in for
+ * (paths relative to the reference checkout root).
+ */
+#ifndef XPR_HIP_H
+#define XPR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XPR_ABI_VERSION 1
+
+/* status codes (stand in for CHECK -> std::runtime_error, gpu/error.hpp:17-27,
+ * and std::bad_alloc, gpu/context.hpp:329-336) */
+#define XPR_OK            0
+#define XPR_E_INVALID    -1   /* bad argument / shape                        */
+#define XPR_E_HIP        -2   /* HIP runtime error                           */
+#define XPR_E_NOMEM      -3   /* device allocation failed / over budget      */
+#define XPR_E_UNSUPPORTED -4  /* dtype or mode not built                     */
+
+/* storage types of volume, projection and error buffers */
+#define XPR_F32  0
+#define XPR_F16  1
+#define XPR_BF16 2
+
+/* Q.22 accumulator widths of the detector sums buffer.  32 reproduces the
+ * reference's uint32 Q10.22 (wraps mod 2^32, constants.cl:17-22); 64 is the
+ * same sum without wrap-around. */
+#define XPR_ACC_U32 32
+#define XPR_ACC_U64 64
+
+typedef void *xpr_stream;   /* a hipStream_t */
+
+/* Kernel arguments of Forward and Backward, passed by value instead of
+ * positional OpenCL arguments (constants.cl:30-54; KernelForward.cl:10-21;
+ * set by Reconstruction::setImageParameters / setVolumeParameters /
+ * setAngleNumber / setOrigin / setImageOffset, src/reconstruction.hpp:208-281). */
+typedef struct {
+    uint32_t det_width;    /* size.x: detector columns (= dwidth)                    */
+    uint32_t n_views;      /* angleNumber: views in this launch                      */
+    uint32_t modules;      /* modulePerAngle: ProjData records per view              */
+    int32_t  vmin[3];      /* volumeMin (x, layer, depth) of the slab, voxels        */
+    int32_t  vmax[3];      /* volumeMax (exclusive)                                  */
+    int32_t  line_offset;  /* lineOffsetY.x: first detector row held in the buffers  */
+    int32_t  line_rows;    /* lineOffsetY.y: detector rows held per view             */
+    float    origin[3];    /* origin (mm) of voxel (0,0,0), jitter included          */
+    float    voxel[3];     /* voxelSize (mm)                                         */
+} xpr_proj_args;
+
+/* ---- library / device --------------------------------------------------- */
+
+/* Version of this ABI (XPR_ABI_VERSION). */
+int xpr_abi_version(void);
+/* Last error message of the calling thread ("" if none). */
+const char *xpr_last_error(void);
+/* Select the device for the calling thread; replaces gpu::Context's device
+ * pick (gpu/context.hpp:36-71). */
+int xpr_set_device(int device);
+/* Free and total device memory in bytes (gpu/context.hpp:87-110). */
+int xpr_mem_info(size_t *free_bytes, size_t *total_bytes);
+
+/* ---- buffers (gpu::Context::createBuffer / setBuffer / getBuffer,
+ *      gpu/context.hpp:152-300) -------------------------------------------- */
+int xpr_alloc(void **dptr, size_t bytes);
+int xpr_free(void *dptr);
+int xpr_host_alloc(void **hptr, size_t bytes);   /* pinned host memory */
+int xpr_host_free(void *hptr);
+int xpr_upload(xpr_stream s, void *dst, const void *src, size_t bytes);   /* async */
+int xpr_download(xpr_stream s, void *dst, const void *src, size_t bytes); /* async */
+int xpr_stream_create(xpr_stream *s);
+int xpr_stream_destroy(xpr_stream s);
+int xpr_stream_sync(xpr_stream s);
+/* Fill kernels (enqueueFillBuffer, gpu/context.hpp:222-239): n elements. */
+int xpr_fill_u32(xpr_stream s, uint32_t *dst, uint32_t value, uint64_t n);
+int xpr_fill_u64(xpr_stream s, uint64_t *dst, uint64_t value, uint64_t n);
+/* Fill a buffer of dtype XPR_F32/F16/BF16 with value (rounded to nearest even). */
+int xpr_fill(xpr_stream s, void *dst, int dtype, float value, uint64_t n);
+/* Convert n elements between storage types (round to nearest even). */
+int xpr_convert(xpr_stream s, void *dst, int dst_dtype, const void *src, int src_dtype,
+                uint64_t n);
+
+/* ---- the hot path -------------------------------------------------------- */
+
+/* Forward projection (Reconstruction::forward, src/reconstruction.hpp:294-296;
+ * kernel Forward, KernelForward.cl:10-55).  Adds trunc(v*2^22) of every voxel
+ * of the slab into the detector pixel each view rounds it to:
+ *   volume   : slab [layer][depth][x] of (vmax-vmin) voxels, dtype vol_dtype
+ *   sums     : uint32 or uint64 Q.22 [n_views][line_rows][det_width]
+ *   projdata : device float[n_views*modules][20] (ProjData, constants.cl:25-28)
+ * Integer accumulation: the result does not depend on launch order. */
+int xpr_forward(xpr_stream s, const void *volume, int vol_dtype, void *sums, int acc_bits,
+                const float *projdata, const xpr_proj_args *a);
+
+/* Error (Reconstruction::error, src/reconstruction.hpp:302-304; kernel Error,
+ * KernelError.cl:8-24): err = log(max(img / max(sum*2^-22, eps), eps)) * weight,
+ * eps = 2^-22, for n pixels.  err may alias sums when acc_bits == 32 and
+ * err_dtype == XPR_F32 (the reference's in-place form). */
+int xpr_error(xpr_stream s, const void *images, int img_dtype, const void *sums, int acc_bits,
+              void *err, int err_dtype, uint64_t n, float weight);
+
+/* Backward projection and multiplicative update (Reconstruction::backward,
+ * src/reconstruction.hpp:310-312; kernel Backward, KernelBackward.cl:8-41):
+ *   v <- v * exp(sum_j err_j(pixel_j(v)) / n_in_bounds)
+ * Voxels with no in-bounds view keep their value (the reference computes
+ * exp(0/0)); if empty_count (device uint64) is non-NULL their number is added
+ * to it. */
+int xpr_backward(xpr_stream s, void *volume, int vol_dtype, const void *err, int err_dtype,
+                 const float *projdata, const xpr_proj_args *a, uint64_t *empty_count);
+
+/* ---- preprocessing (DataLoader::initializeTempImages,
+ *      dataset/data_loader.hpp:48-101) -------------------------------------- */
+
+/* Running global min/max over isnormal() samples (data_loader.hpp:55-74).
+ * minmax (device, 2 x int32 order keys) must be initialised with
+ * xpr_minmax_init; read it back with xpr_minmax_read. */
+int xpr_minmax_init(xpr_stream s, int32_t *minmax);
+int xpr_minmax_normal(xpr_stream s, const float *data, uint64_t n, int32_t *minmax);
+int xpr_minmax_read(const int32_t *minmax_host, float *mn, float *mx);
+/* Normalise (v-min)/(max-min) if normalize, then -log(clamp(v, eps, 1-eps))
+ * if mlog (data_loader.hpp:80-86), from float input to dtype out_dtype. */
+int xpr_preprocess(xpr_stream s, const float *in, void *out, int out_dtype, uint64_t n,
+                   int normalize, int mlog, float mn, float mx);
+
+/* ---- geometry (Geometry, dataset/geometry.hpp:35-76, 182-328) ----------- */
+
+/* JSON "geometry" and "detector" sections (global_parameters.hpp:23-54).
+ * rx is in pixels (multiplied by px at load, parameters.hpp:39); sx, sy in mm. */
+typedef struct {
+    double so, sd, angle;            /* source-object, source-detector (mm), total angle (deg) */
+    double heli_offset;              /* mm */
+    double px, rx, sx, sy;           /* pixel pitch (mm), axis offset (px), detector shift (mm) */
+    double module_angle_offset;      /* deg, used when has_module_angle */
+    double module_center_offset_z;   /* mm */
+    double roll, pitch, yaw;         /* deg, module rotation about the detector centre
+                                        (extension; the reference parses none of them) */
+    int64_t module_number;
+    int64_t dwidth, dheight, projections;
+    int32_t has_module_angle;
+    int32_t fp64;                    /* 1: FP64 construction; 0: float32 as the reference */
+    const double *angle_list;        /* NULL: angle i = i*angle/projections (deg) */
+} xpr_geometry_params;
+
+/* Reconstruction grid (geometry.hpp:50-68): vx = px*so/sd, W = dwidth,
+ * H = dheight + |heli_offset/vx|, origin of voxel (0,0,0) = -(W,H,W)*vx/2. */
+typedef struct {
+    double vx;
+    int64_t vwidth, vheight;
+    double orig[3];
+} xpr_grid;
+
+/* ProjData of every (angle, module), angle-major: out has
+ * projections*module_number*20 floats (Geometry::computeMVPs,
+ * geometry.hpp:298-328).  Host function, no device work. */
+int xpr_geometry(const xpr_geometry_params *p, float *projdata_out, xpr_grid *grid_out);
+
+/* Detector frames of every (angle, module) in FP64, 14 doubles each: source
+ * position, upper-left detector corner, column step, row step (mm per pixel),
+ * vpl, vpb (geometry.hpp:244-292).  Used by the synthetic-data generator. */
+int xpr_geometry_frames(const xpr_geometry_params *p, double *frames_out);
+
+/* ---- synthetic data (no counterpart in the reference: test and benchmark
+ *      input generation) ------------------------------------------------------ */
+
+/* One ellipsoid of a phantom: centre c, semi-axes a (mm), row-major rotation
+ * rot (body = rot * (x - c)), attenuation mu per mm. */
+typedef struct {
+    double c[3], a[3], rot[9], mu;
+} xpr_ellipsoid;
+
+/* FP64 line integrals of a sum of ellipsoids along the ray from the source to
+ * the centre of every pixel (pixel (col,row) is centred at detector coordinate
+ * (col,row), the reference's convention).  frames: device copy of
+ * xpr_geometry_frames for the n_records (view, module) records written; out:
+ * device double [n_views][det_height][det_width]. */
+int xpr_synth_line_integrals(xpr_stream s, const double *frames, uint32_t n_views,
+                             uint32_t modules, uint32_t det_width, uint32_t det_height,
+                             const xpr_ellipsoid *ellipsoids, int n_ellipsoids, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

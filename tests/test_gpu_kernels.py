@@ -1,0 +1,261 @@
+"""GPU parity of the HIP kernels against the oracle, through the C-ABI.
+
+Bars: Forward bit-exact (integer Q.22 sums); Backward and Error within a few
+float32 ulp (the only difference is the device expf/logf), stated per test as
+rtol; storage conversions bit-exact.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import geom_args, oracle_geom, projdata_for, rel_l2, small_spec
+
+pytestmark = pytest.mark.gpu
+
+Q = 1 << 22
+
+
+def _args(d):
+    import xpulse_recon_amd as X
+    return X.proj_args(d["det_width"], d["n_views"], d["modules"], d["vmin"], d["vmax"],
+                       d["line_offset"], d["line_rows"], d["origin"], d["voxel"])
+
+
+def _check_shapes(vol, det, d):
+    nx, nl, nk = (d["vmax"][i] - d["vmin"][i] for i in range(3))
+    assert vol.size == nx * nl * nk
+    assert det.size == d["n_views"] * d["line_rows"] * d["det_width"]
+
+
+CASES = [
+    # (spec kwargs, views, slab (vmin, vmax) or None, line_offset/rows or None, jitter)
+    (dict(w=32, h=24, n=12), [0, 3, 6, 9], None, None, (0.0, 0.0, 0.0)),
+    (dict(w=100, h=40, n=20, rx=2.5), list(range(1, 20, 2)), None, None, (0.004, 0.001, 0.002)),
+    (dict(w=130, h=8, n=6, sx=0.2), [0, 1, 2, 3, 4, 5], ((0, 2, 0), (130, 7, 130)), None,
+     (0.001, 0.0, 0.003)),
+    (dict(w=48, h=32, n=8, module_number=3), [0, 2, 4, 6], None, None, (0.0, 0.002, 0.0)),
+    (dict(w=40, h=36, n=10), [1, 4, 7], ((0, 10, 0), (40, 22, 40)), (8, 20), (0.002, 0.003, 0.001)),
+    (dict(w=1, h=3, n=4), [0, 1, 2, 3], None, None, (0.0, 0.0, 0.0)),
+]
+
+
+def _case(i):
+    kw, views, slab, crop, jit = CASES[i]
+    kw = dict(kw)
+    w, h, n = kw.pop("w"), kw.pop("h"), kw.pop("n")
+    spec = small_spec(w, h, n, **kw)
+    pd, grid = projdata_for(spec, views)
+    vmin, vmax = slab if slab else (None, None)
+    lo, rows = crop if crop else (0, None)
+    d = geom_args(spec, grid, len(views), vmin=vmin, vmax=vmax, jitter=jit, line_offset=lo,
+                  line_rows=rows)
+    return spec, pd, d
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("acc", [32, 64])
+def test_forward_bit_exact(cuda, case, acc):
+    import torch
+    import xpulse_recon_amd as X
+    spec, pd, d = _case(case)
+    nx, nl, nk = (d["vmax"][i] - d["vmin"][i] for i in range(3))
+    rng = np.random.default_rng(100 + case)
+    vol = rng.uniform(0.0, 3.0, (nl, nk, nx)).astype(np.float32)
+    vol[0, 0, :] = 0.0
+    shape = (d["n_views"], d["line_rows"], d["det_width"])
+    ref = np.zeros(shape, np.uint32 if acc == 32 else np.uint64)
+    _check_shapes(vol, ref, d)
+    O.forward_f32(vol, ref, pd, oracle_geom(d))
+    tdt = torch.int32 if acc == 32 else torch.int64
+    sums = torch.zeros(shape, dtype=tdt, device=cuda)
+    X.forward(torch.from_numpy(vol).to(cuda), sums, torch.from_numpy(pd).to(cuda), _args(d))
+    got = sums.cpu().numpy().view(ref.dtype)
+    np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_backward_parity(cuda, case):
+    import torch
+    import xpulse_recon_amd as X
+    spec, pd, d = _case(case)
+    nx, nl, nk = (d["vmax"][i] - d["vmin"][i] for i in range(3))
+    rng = np.random.default_rng(200 + case)
+    vol = rng.uniform(0.5, 2.0, (nl, nk, nx)).astype(np.float32)
+    err = rng.normal(0.0, 0.4, (d["n_views"], d["line_rows"], d["det_width"])).astype(np.float32)
+    _check_shapes(vol, err, d)
+    ref = vol.copy()
+    empty_ref = O.backward_f32(ref, err, pd, oracle_geom(d))
+    tv = torch.from_numpy(vol).to(cuda)
+    empty = torch.zeros(1, dtype=torch.int64, device=cuda)
+    X.backward(tv, torch.from_numpy(err).to(cuda), torch.from_numpy(pd).to(cuda), _args(d),
+               empty=empty)
+    got = tv.cpu().numpy()
+    assert int(empty.item()) == empty_ref
+    np.testing.assert_allclose(got, ref, rtol=4e-7, atol=0)
+    assert rel_l2(got, ref) < 1e-6
+
+
+def test_error_parity(cuda):
+    import torch
+    import xpulse_recon_amd as X
+    rng = np.random.default_rng(7)
+    n = 100003
+    img = rng.uniform(0.0, 4.0, n).astype(np.float32)
+    img[:4] = [0.0, 1e-30, 1.0, 3.0]
+    for acc in (32, 64):
+        hi = (1 << 31) if acc == 32 else (1 << 40)
+        sums = rng.integers(0, hi, n).astype(np.uint32 if acc == 32 else np.uint64)
+        sums[:4] = [0, 1, 1 << 22, 5 << 22]
+        ref = O.error_f32(img, sums, 0.9)
+        tdt = torch.int32 if acc == 32 else torch.int64
+        ts = torch.from_numpy(sums.view(np.int32 if acc == 32 else np.int64)).to(cuda)
+        out = torch.empty(n, dtype=torch.float32, device=cuda)
+        X.error(torch.from_numpy(img).to(cuda), ts, out, 0.9)
+        np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=4e-7, atol=1e-7)
+        if acc == 32:                       # in place in the sums buffer
+            X.error(torch.from_numpy(img).to(cuda), ts, ts.view(torch.float32), 0.9)
+            np.testing.assert_array_equal(ts.view(torch.float32).cpu().numpy(), out.cpu().numpy())
+        assert ts.dtype == tdt
+
+
+def test_preprocess_and_minmax_parity(cuda):
+    import torch
+    import xpulse_recon_amd as X
+    rng = np.random.default_rng(12)
+    x = rng.uniform(-0.2, 1.3, 70001).astype(np.float32)
+    x[:6] = [0.0, np.inf, np.nan, 1e-40, 1.0, -3.0]
+    mn, mx = O.minmax_normal(x)
+    tx = torch.from_numpy(x).to(cuda)
+    assert X.minmax_normal(tx) == (mn, mx)
+    for normalize in (False, True):
+        ref = O.preprocess(x.copy(), normalize, True, mn, mx)
+        out = torch.empty_like(tx)
+        X.preprocess(tx, out, normalize, True, mn, mx)
+        got = out.cpu().numpy()
+        fin = np.isfinite(ref)
+        np.testing.assert_array_equal(np.isnan(got), np.isnan(ref))
+        np.testing.assert_allclose(got[fin], ref[fin], rtol=4e-7, atol=1e-7)
+
+
+def _bf16_bits(a):
+    b = a.astype(np.float32).view(np.uint32).astype(np.uint64)
+    r = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16).astype(np.uint16)
+    nan = np.isnan(a)
+    r[nan] = ((b[nan] >> 16) | 0x40).astype(np.uint16)
+    return r
+
+
+def test_convert_and_fill(cuda):
+    import torch
+    import xpulse_recon_amd as X
+    rng = np.random.default_rng(13)
+    x = (rng.standard_normal(5000) * 10.0 ** rng.uniform(-8, 4, 5000)).astype(np.float32)
+    tx = torch.from_numpy(x).to(cuda)
+    h = torch.empty(5000, dtype=torch.float16, device=cuda)
+    X.convert(h, tx)
+    np.testing.assert_array_equal(h.cpu().numpy().view(np.uint16), x.astype(np.float16).view(np.uint16))
+    b = torch.empty(5000, dtype=torch.bfloat16, device=cuda)
+    X.convert(b, tx)
+    np.testing.assert_array_equal(b.cpu().view(torch.int16).numpy().view(np.uint16), _bf16_bits(x))
+    back = torch.empty_like(tx)
+    X.convert(back, b)
+    np.testing.assert_array_equal(back.cpu().numpy(),
+                                  (_bf16_bits(x).astype(np.uint32) << 16).view(np.float32))
+    s = torch.empty(777, dtype=torch.int64, device=cuda)
+    X.fill_u64(s, (1 << 40) + 3)
+    assert (s.cpu() == (1 << 40) + 3).all()
+    X.fill(h, 1.0)
+    assert (h.cpu() == 1.0).all()
+
+
+@pytest.mark.parametrize("vdt", ["f16", "bf16"])
+def test_mixed_precision_storage(cuda, vdt):
+    """16-bit volume and error storage, FP32 arithmetic: forward is bit-exact
+    on the widened values; backward equals the oracle's FP32 update rounded to
+    the storage type (one storage ulp where device expf differs)."""
+    import torch
+    import xpulse_recon_amd as X
+    spec, pd, d = _case(1)
+    nx, nl, nk = (d["vmax"][i] - d["vmin"][i] for i in range(3))
+    tdt = {"f16": torch.float16, "bf16": torch.bfloat16}[vdt]
+    rng = np.random.default_rng(14)
+    vol32 = torch.from_numpy(rng.uniform(0.5, 2.0, (nl, nk, nx)).astype(np.float32))
+    vol16 = vol32.to(tdt)
+    err32 = torch.from_numpy(rng.normal(0, 0.3, (d["n_views"], d["line_rows"], d["det_width"]))
+                             .astype(np.float32))
+    err16 = err32.to(tdt)
+    wide_vol = vol16.float().numpy()
+    wide_err = err16.float().numpy()
+    ref_sums = np.zeros(err32.shape, np.uint64)
+    O.forward_f32(wide_vol, ref_sums, pd, oracle_geom(d))
+    sums = torch.zeros(err32.shape, dtype=torch.int64, device=cuda)
+    tpd = torch.from_numpy(pd).to(cuda)
+    X.forward(vol16.to(cuda), sums, tpd, _args(d))
+    np.testing.assert_array_equal(sums.cpu().numpy().view(np.uint64), ref_sums)
+    ref = wide_vol.copy()
+    O.backward_f32(ref, wide_err, pd, oracle_geom(d))
+    ref16 = torch.from_numpy(ref).to(tdt).float().numpy()
+    tv = vol16.to(cuda)
+    X.backward(tv, err16.to(cuda), tpd, _args(d))
+    got = tv.float().cpu().numpy()
+    ulp = 2.0 ** (-10 if vdt == "f16" else -7)
+    np.testing.assert_allclose(got, ref16, rtol=ulp * 1.01, atol=0)
+    assert np.mean(got != ref16) < 0.01
+
+
+def test_slab_split_is_exact_at_scale(cuda):
+    """256^3 grid, 24 views: forward over 5 uneven slabs sums to the whole-volume
+    forward bit for bit; backward over slabs equals whole-volume backward."""
+    import torch
+    import xpulse_recon_amd as X
+    spec = small_spec(256, 256, 240, rx=1.5)
+    views = list(range(3, 240, 10))
+    pd, grid = projdata_for(spec, views)
+    d = geom_args(spec, grid, len(views), jitter=(0.0031, 0.0012, 0.0077))
+    W = H = 256
+    g = torch.Generator(device="cpu").manual_seed(15)
+    vol = (torch.rand((H, W, W), generator=g) * 2.0 + 0.1).to(cuda)
+    tpd = torch.from_numpy(pd).to(cuda)
+    shape = (len(views), 256, 256)
+    whole = torch.zeros(shape, dtype=torch.int64, device=cuda)
+    X.forward(vol, whole, tpd, _args(d))
+    parts = torch.zeros_like(whole)
+    cuts = [0, 37, 38, 100, 201, 256]
+    for l0, l1 in zip(cuts[:-1], cuts[1:]):
+        ds = dict(d, vmin=(0, l0, 0), vmax=(W, l1, W))
+        X.forward(vol[l0:l1].contiguous(), parts, tpd, _args(ds))
+    assert torch.equal(parts, whole)
+    err = torch.randn(shape, generator=g).mul_(0.2).to(cuda)
+    a = vol.clone()
+    X.backward(a, err, tpd, _args(d))
+    b = vol.clone()
+    for l0, l1 in zip(cuts[:-1], cuts[1:]):
+        ds = dict(d, vmin=(0, l0, 0), vmax=(W, l1, W))
+        piece = b[l0:l1].contiguous()
+        X.backward(piece, err, tpd, _args(ds))
+        b[l0:l1] = piece
+    assert torch.equal(a, b)
+    # uniform volume of 1.0: total Q.22 mass = 2^22 * in-bounds voxel-views,
+    # and every in-bounds voxel sees err == 0 -> stays exactly 1
+    ones = torch.ones((H, W, W), device=cuda)
+    s = torch.zeros(shape, dtype=torch.int64, device=cuda)
+    X.forward(ones, s, tpd, _args(d))
+    hits = int(s.sum().item()) // Q
+    assert int(s.sum().item()) % Q == 0
+    assert 0.5 * H * W * W * len(views) < hits <= H * W * W * len(views)
+
+
+def test_synthetic_line_integrals_match_oracle(cuda):
+    from xpulse_recon_amd import synth
+    from xpulse_recon_amd.geometry import Geometry
+    kw = dict(dwidth=48, dheight=40, projections=8, rx=2.0, sx=0.1, module_number=2)
+    geom = Geometry(**kw)
+    _, grid = geom.projdata()
+    ell = synth.random_ellipsoids(5, 6, grid["vwidth"] * grid["vx"], 128)
+    got = synth.line_integrals(geom, [1, 5], ell, cuda).cpu().numpy()
+    oe = [O.make_ellipsoid(e["c"], e["a"], e["rot"], e["mu"]) for e in ell]
+    spec = O.GeometrySpec(**kw)
+    for j, a in enumerate([1, 5]):
+        ref = O.line_integrals(spec, a, oe)
+        np.testing.assert_allclose(got[j], ref, rtol=1e-9, atol=1e-12)
+    assert got.max() > 0.1

@@ -1,0 +1,137 @@
+"""MI355X-native iterative cone-beam CT hot path (XPRecon's Forward / Error /
+Backward operators) behind a C-ABI (include/xpr_hip.h).
+
+This module is the Python face of the C-ABI for tests and benchmarks: device
+memory comes from torch tensors (plumbing only), every computation runs in the
+HIP library ``lib/libxpr_hip.so``.  The operator names follow the reference's
+``Reconstruction::forward/error/backward`` (src/reconstruction.hpp:294-312).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+from . import _lib
+from ._lib import ProjArgs, XprError, proj_args  # noqa: F401
+
+__all__ = ["forward", "error", "backward", "fill", "fill_u32", "fill_u64", "convert",
+           "preprocess", "minmax_normal", "proj_args", "ProjArgs", "XprError",
+           "dtype_code", "acc_bits"]
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if isinstance(stream, int):
+        return C.c_void_p(stream)
+    return C.c_void_p(stream.cuda_stream)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def dtype_code(t) -> int:
+    import torch
+    m = {torch.float32: _lib.XPR_F32, torch.float16: _lib.XPR_F16,
+         torch.bfloat16: _lib.XPR_BF16}
+    if t.dtype not in m:
+        raise XprError(f"unsupported storage dtype {t.dtype}")
+    return m[t.dtype]
+
+
+def acc_bits(t) -> int:
+    import torch
+    m = {torch.int32: 32, torch.uint32: 32, torch.int64: 64, torch.uint64: 64}
+    if t.dtype not in m:
+        raise XprError(f"sums must be a 32- or 64-bit integer tensor, got {t.dtype}")
+    return m[t.dtype]
+
+
+def _need_cuda(*ts):
+    for t in ts:
+        if not t.is_cuda:
+            raise XprError("tensor is not on the GPU")
+
+
+def _check_sizes(volume, det, projdata, a: ProjArgs):
+    """Host-side shape check before any launch: the buffers must hold the slab,
+    the detector views and the ProjData records the arguments describe."""
+    n = [a.vmax[i] - a.vmin[i] for i in range(3)]
+    if min(n) < 0:
+        raise XprError("volumeMax < volumeMin")
+    if volume.numel() < n[0] * n[1] * n[2] or not volume.is_contiguous():
+        raise XprError(f"volume holds {volume.numel()} voxels, slab needs {n[0] * n[1] * n[2]}")
+    need = a.n_views * a.line_rows * a.det_width
+    if det.numel() < need or not det.is_contiguous():
+        raise XprError(f"detector buffer holds {det.numel()} pixels, needs {need}")
+    if projdata.numel() < 20 * a.n_views * a.modules or projdata.dtype.itemsize != 4:
+        raise XprError("projdata must hold 20 float32 per (view, module)")
+
+
+def forward(volume, sums, projdata, args: ProjArgs, stream=None) -> None:
+    """Forward projection: sums[view,row,col] += trunc(v*2^22) (KernelForward.cl:10-55)."""
+    _need_cuda(volume, sums, projdata)
+    _check_sizes(volume, sums, projdata, args)
+    _lib.call("xpr_forward", _stream(stream), _ptr(volume), dtype_code(volume), _ptr(sums),
+              acc_bits(sums), _ptr(projdata), C.byref(args))
+
+
+def error(images, sums, err, weight: float, n: int | None = None, stream=None) -> None:
+    """err = log(max(img / max(sum*2^-22, eps), eps)) * weight (KernelError.cl:8-24)."""
+    _need_cuda(images, sums, err)
+    n = images.numel() if n is None else n
+    if n > min(images.numel(), sums.numel(), err.numel()):
+        raise XprError("error: n exceeds a buffer")
+    _lib.call("xpr_error", _stream(stream), _ptr(images), dtype_code(images), _ptr(sums),
+              acc_bits(sums), _ptr(err), dtype_code(err), n, float(weight))
+
+
+def backward(volume, err, projdata, args: ProjArgs, empty=None, stream=None) -> None:
+    """Backward projection + multiplicative update (KernelBackward.cl:8-41)."""
+    _need_cuda(volume, err, projdata)
+    _check_sizes(volume, err, projdata, args)
+    _lib.call("xpr_backward", _stream(stream), _ptr(volume), dtype_code(volume), _ptr(err),
+              dtype_code(err), _ptr(projdata), C.byref(args),
+              None if empty is None else _ptr(empty))
+
+
+def fill(t, value: float, stream=None) -> None:
+    _need_cuda(t)
+    _lib.call("xpr_fill", _stream(stream), _ptr(t), dtype_code(t), float(value), t.numel())
+
+
+def fill_u32(t, value: int, stream=None) -> None:
+    _lib.call("xpr_fill_u32", _stream(stream), _ptr(t), value & 0xFFFFFFFF, t.numel())
+
+
+def fill_u64(t, value: int, stream=None) -> None:
+    _lib.call("xpr_fill_u64", _stream(stream), _ptr(t), value & 0xFFFFFFFFFFFFFFFF, t.numel())
+
+
+def convert(dst, src, stream=None) -> None:
+    n = min(dst.numel(), src.numel())
+    _lib.call("xpr_convert", _stream(stream), _ptr(dst), dtype_code(dst), _ptr(src),
+              dtype_code(src), n)
+
+
+def minmax_normal(data, stream=None):
+    """Global min/max over isnormal() samples (data_loader.hpp:55-74)."""
+    import torch
+    mm = torch.empty(2, dtype=torch.int32, device=data.device)
+    s = _stream(stream)
+    _lib.call("xpr_minmax_init", s, _ptr(mm))
+    _lib.call("xpr_minmax_normal", s, _ptr(data), data.numel(), _ptr(mm))
+    host = mm.cpu()
+    a, b = C.c_float(), C.c_float()
+    _lib.call("xpr_minmax_read", C.c_void_p(host.data_ptr()), C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def preprocess(data, out, normalize: bool, mlog: bool, mn: float = 0.0, mx: float = 1.0,
+               stream=None) -> None:
+    """Normalise and -log clamp (data_loader.hpp:80-86) into out's dtype."""
+    if out.numel() < data.numel():
+        raise XprError("preprocess: output smaller than input")
+    _lib.call("xpr_preprocess", _stream(stream), _ptr(data), _ptr(out), dtype_code(out),
+              data.numel(), int(normalize), int(mlog), float(mn), float(mx))

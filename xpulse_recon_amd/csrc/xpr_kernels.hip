@@ -1,0 +1,784 @@
+// HIP kernels (gfx950) and the C-ABI of the iterative cone-beam hot path.
+//
+// Canonical arithmetic shared with the CPU oracle (the reference compiles its
+// OpenCL kernels with -cl-fast-relaxed-math -cl-mad-enable, gpu/kernel.hpp:26-31,
+// so its own output is platform dependent; this build fixes one legal lowering):
+//   position  p_a = fmaf((float)idx_a, voxel_a, origin_a)       KernelForward.cl:26
+//   dot(c, p) = fmaf(c.x, p.x, fmaf(c.y, p.y, fmaf(c.z, p.z, c.w)))   utils.cl:26
+//   (u, v)    = (X * r, Y * r - line_offset), r = 1.0f / W (IEEE)     utils.cl:26
+//   pixel     = roundf (half away from zero), half-open viewport     KernelForward.cl:34-35
+// Built with -ffp-contract=off, so only the explicit fmaf calls fuse.
+//
+// Layouts in HBM:
+//   volume slab : [layer][depth][x] (to3D, utils.cl:39-45), x fastest
+//   sums / err  : [view][row][col], row stride det_width, view stride
+//                 det_width*line_rows (IMAGE_OFFSET_2D, utils.cl:47)
+//   ProjData    : 20 floats per (view, module): glm column-major 4x4, then
+//                 viewport (vpl, vpr, vpb, vpt) (constants.cl:25-28)
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/xpr_hip.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+#define XPR_HIP_CHECK(expr)                                                          \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            return fail(XPR_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define XPR_LAUNCH_CHECK()                                                           \
+    do {                                                                             \
+        hipError_t e_ = hipGetLastError();                                           \
+        if (e_ != hipSuccess)                                                        \
+            return fail(XPR_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr float kQ22One = 4194304.0f;        // FIXED_FRAC_ONE (constants.cl:19)
+constexpr float kQ22Eps = 1.0f / 4194304.0f; // EPSILON (constants.cl:23)
+
+// ---------------------------------------------------------------------------
+// storage types
+// ---------------------------------------------------------------------------
+struct F32 {
+    using T = float;
+    static __device__ __forceinline__ float load(const T *p, uint64_t i) { return p[i]; }
+    static __device__ __forceinline__ void store(T *p, uint64_t i, float v) { p[i] = v; }
+};
+struct F16 {
+    using T = uint16_t;
+    static __device__ __forceinline__ float load(const T *p, uint64_t i)
+    {
+        return (float)__builtin_bit_cast(_Float16, p[i]);
+    }
+    static __device__ __forceinline__ void store(T *p, uint64_t i, float v)
+    {
+        p[i] = __builtin_bit_cast(uint16_t, (_Float16)v);
+    }
+};
+struct BF16 {
+    using T = uint16_t;
+    static __device__ __forceinline__ float load(const T *p, uint64_t i)
+    {
+        return __builtin_bit_cast(float, (uint32_t)p[i] << 16);
+    }
+    static __device__ __forceinline__ void store(T *p, uint64_t i, float v)
+    {
+        uint32_t b = __builtin_bit_cast(uint32_t, v);
+        if ((b & 0x7fffffffu) > 0x7f800000u) {
+            p[i] = (uint16_t)((b >> 16) | 0x40u);                 // quiet NaN
+        } else {
+            b += 0x7fffu + ((b >> 16) & 1u);                      // nearest even
+            p[i] = (uint16_t)(b >> 16);
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// projection of one voxel by one ProjData record
+// ---------------------------------------------------------------------------
+struct ViewConsts {
+    float c0x, c1x, c3x;       // x coefficients of glm columns 0, 1, 3
+    float k0, k1, k3;          // dot of the (y, z, w) part with the row's (py, pz, 1)
+    float vpl, vpr, vlo, vhi;  // half-open column and row bounds
+    uint64_t base;             // view offset in the detector buffers
+};
+
+__device__ __forceinline__ ViewConsts view_consts(const float *__restrict__ pd, float py,
+                                                  float pz, const xpr_proj_args &a,
+                                                  uint32_t view)
+{
+    ViewConsts c;
+    c.c0x = pd[0];
+    c.c1x = pd[4];
+    c.c3x = pd[12];
+    c.k0 = __builtin_fmaf(pd[1], py, __builtin_fmaf(pd[2], pz, pd[3]));
+    c.k1 = __builtin_fmaf(pd[5], py, __builtin_fmaf(pd[6], pz, pd[7]));
+    c.k3 = __builtin_fmaf(pd[13], py, __builtin_fmaf(pd[14], pz, pd[15]));
+    c.vpl = fmaxf(pd[16], 0.0f);                      // never outside the buffers
+    c.vpr = fminf(pd[17], (float)a.det_width);
+    c.vlo = fmaxf(pd[18], 0.0f);
+    c.vhi = fminf(pd[19], (float)a.line_rows);
+    c.base = (uint64_t)view * ((uint64_t)a.det_width * (uint64_t)a.line_rows);
+    return c;
+}
+
+// Returns true and the detector index when voxel x-position px lands in bounds.
+__device__ __forceinline__ bool project(const ViewConsts &c, float px, float line_off,
+                                        uint32_t det_width, uint64_t &idx)
+{
+    const float X = __builtin_fmaf(c.c0x, px, c.k0);
+    const float Y = __builtin_fmaf(c.c1x, px, c.k1);
+    const float W = __builtin_fmaf(c.c3x, px, c.k3);
+    const float r = 1.0f / W;
+    const float u = X * r;
+    const float v = Y * r - line_off;
+    const float cu = __builtin_roundf(u);
+    const float cv = __builtin_roundf(v);
+    const bool inb = (cu >= c.vpl) & (cu < c.vpr) & (cv >= c.vlo) & (cv < c.vhi);
+    idx = c.base + (uint64_t)((uint32_t)(int32_t)cv * det_width + (uint32_t)(int32_t)cu);
+    return inb;
+}
+
+// ---------------------------------------------------------------------------
+// projector kernels: one wave per (layer, depth) row, lane owns J voxels at
+// x = seg*64*J + lane + 64*j.  The (y, z) part of the three dot products is
+// hoisted per row and view; per voxel-view the work is 3 FMA, 1 reciprocal,
+// 2 MUL, 2 round, 4 compares and one gather (backward) or atomic (forward).
+// ---------------------------------------------------------------------------
+constexpr int kWaves = 4;
+constexpr int kThreads = 64 * kWaves;
+
+struct SlabShape {
+    int32_t nx, nl, nk;
+    int32_t nseg;          // x segments per row
+    int64_t row_base;      // first row of this launch
+    int64_t nrows;         // rows in the slab
+};
+
+template <typename VS, typename ES, int J>
+__global__ __launch_bounds__(kThreads) void k_backward(
+    typename VS::T *__restrict__ vol, const typename ES::T *__restrict__ err,
+    const float *__restrict__ pd, xpr_proj_args a, SlabShape s,
+    unsigned long long *__restrict__ empty_count)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int seg = blockIdx.x % s.nseg;
+    const int64_t row = s.row_base + (int64_t)(blockIdx.x / s.nseg) * kWaves + wave;
+    if (row >= s.nrows) return;
+    const int32_t l = (int32_t)(row / s.nk), k = (int32_t)(row % s.nk);
+    const float py = __builtin_fmaf((float)(l + a.vmin[1]), a.voxel[1], a.origin[1]);
+    const float pz = __builtin_fmaf((float)(k + a.vmin[2]), a.voxel[2], a.origin[2]);
+    const int32_t xb = seg * 64 * J + lane;
+    float px[J], sum[J];
+    uint32_t cnt[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        px[j] = __builtin_fmaf((float)(xb + 64 * j + a.vmin[0]), a.voxel[0], a.origin[0]);
+        sum[j] = 0.0f;
+        cnt[j] = 0;
+    }
+    const float line_off = (float)a.line_offset;
+    for (uint32_t view = 0, m = 0; view < a.n_views; ++view)
+    for (uint32_t mm = 0; mm < a.modules; ++mm, ++m) {
+        const ViewConsts c = view_consts(pd + 20u * m, py, pz, a, view);
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            uint64_t idx;
+            const bool inb = project(c, px[j], line_off, a.det_width, idx);
+            // unconditional load from a valid address keeps J loads in flight
+            const float e = ES::load(err, inb ? idx : c.base);
+            sum[j] += inb ? e : 0.0f;
+            cnt[j] += inb ? 1u : 0u;
+        }
+    }
+    uint32_t empty = 0;
+    const uint64_t rowoff = (uint64_t)row * (uint64_t)s.nx;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int32_t x = xb + 64 * j;
+        if (x < s.nx) {
+            if (cnt[j] == 0) {
+                ++empty;
+            } else {
+                const float v = VS::load(vol, rowoff + x);
+                VS::store(vol, rowoff + x, v * expf(sum[j] / (float)cnt[j]));
+            }
+        }
+    }
+    if (empty_count != nullptr) {
+        // wave-level sum, one atomic per wave
+        uint32_t e = empty;
+        for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off);
+        if (lane == 0 && e) atomicAdd(empty_count, (unsigned long long)e);
+    }
+}
+
+// FLOAT_TO_FIXED (constants.cl:22), saturating like the GPU conversion.
+__device__ __forceinline__ uint32_t q22_u32(float v)
+{
+    const float s = v * kQ22One;
+    if (!(s > 0.0f)) return 0u;
+    if (s >= 4294967296.0f) return 0xffffffffu;
+    return (uint32_t)s;
+}
+__device__ __forceinline__ uint64_t q22_u64(float v)
+{
+    const float s = v * kQ22One;
+    if (!(s > 0.0f)) return 0ull;
+    if (s >= 18446744073709551616.0f) return ~0ull;
+    return (uint64_t)s;
+}
+
+template <typename AccT>
+__device__ __forceinline__ AccT q22(float v);
+template <>
+__device__ __forceinline__ uint32_t q22<uint32_t>(float v) { return q22_u32(v); }
+template <>
+__device__ __forceinline__ uint64_t q22<uint64_t>(float v) { return q22_u64(v); }
+
+template <typename VS, typename AccT, int J>
+__global__ __launch_bounds__(kThreads) void k_forward_global(
+    const typename VS::T *__restrict__ vol, AccT *__restrict__ sums,
+    const float *__restrict__ pd, xpr_proj_args a, SlabShape s)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int seg = blockIdx.x % s.nseg;
+    const int64_t row = s.row_base + (int64_t)(blockIdx.x / s.nseg) * kWaves + wave;
+    if (row >= s.nrows) return;
+    const int32_t l = (int32_t)(row / s.nk), k = (int32_t)(row % s.nk);
+    const float py = __builtin_fmaf((float)(l + a.vmin[1]), a.voxel[1], a.origin[1]);
+    const float pz = __builtin_fmaf((float)(k + a.vmin[2]), a.voxel[2], a.origin[2]);
+    const int32_t xb = seg * 64 * J + lane;
+    const uint64_t rowoff = (uint64_t)row * (uint64_t)s.nx;
+    float px[J];
+    AccT q[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int32_t x = xb + 64 * j;
+        px[j] = __builtin_fmaf((float)(x + a.vmin[0]), a.voxel[0], a.origin[0]);
+        q[j] = (x < s.nx) ? q22<AccT>(VS::load(vol, rowoff + x)) : (AccT)0;
+    }
+    const float line_off = (float)a.line_offset;
+    for (uint32_t view = 0, m = 0; view < a.n_views; ++view)
+    for (uint32_t mm = 0; mm < a.modules; ++mm, ++m) {
+        const ViewConsts c = view_consts(pd + 20u * m, py, pz, a, view);
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            uint64_t idx;
+            if (project(c, px[j], line_off, a.det_width, idx) && q[j] != 0)
+                __hip_atomic_fetch_add(sums + idx, q[j], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// elementwise kernels
+// ---------------------------------------------------------------------------
+template <typename IS, typename AccT, typename ES>
+__global__ __launch_bounds__(256) void k_error(const typename IS::T *__restrict__ img,
+                                               const AccT *sums, typename ES::T *err,
+                                               uint64_t n, float weight)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float ref = IS::load(img, i);
+        const float got = (float)sums[i] / kQ22One;                 // FIXED_TO_FLOAT
+        const float e = logf(fmaxf(ref / fmaxf(got, kQ22Eps), kQ22Eps)) * weight;
+        ES::store(err, i, e);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill_raw(T *__restrict__ dst, T value, uint64_t n)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        dst[i] = value;
+}
+
+template <typename DS, typename SS>
+__global__ __launch_bounds__(256) void k_convert(typename DS::T *__restrict__ dst,
+                                                 const typename SS::T *__restrict__ src,
+                                                 uint64_t n)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        DS::store(dst, i, SS::load(src, i));
+}
+
+// Order key of a float: signed-int comparison of keys matches float order.
+__device__ __forceinline__ int32_t fkey(float f)
+{
+    int32_t b = __builtin_bit_cast(int32_t, f);
+    return b < 0 ? (b ^ 0x7fffffff) : b;
+}
+
+__global__ __launch_bounds__(256) void k_minmax(const float *__restrict__ data, uint64_t n,
+                                                int32_t *minmax)
+{
+    int32_t lo = 0x7fffffff, hi = (int32_t)0x80000000;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float v = data[i];
+        if (__builtin_isnormal(v)) {
+            const int32_t kk = fkey(v);
+            lo = min(lo, kk);
+            hi = max(hi, kk);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = min(lo, __shfl_xor(lo, off));
+        hi = max(hi, __shfl_xor(hi, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(minmax + 0, lo);
+        atomicMax(minmax + 1, hi);
+    }
+}
+
+template <typename OS>
+__global__ __launch_bounds__(256) void k_preprocess(const float *__restrict__ in,
+                                                    typename OS::T *__restrict__ out,
+                                                    uint64_t n, int normalize, int mlog,
+                                                    float mn, float mx)
+{
+    const float hi_clamp = 1.0f - kQ22Eps;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        float v = in[i];
+        if (normalize)
+            v = __builtin_isnormal(v) ? (v - mn) / (mx - mn) : (mx + mn) / 2.0f;
+        if (mlog) {
+            float t = (hi_clamp < v) ? hi_clamp : v;   // std::min
+            t = (t < kQ22Eps) ? kQ22Eps : t;           // std::max
+            v = -logf(t);
+        }
+        OS::store(out, i, v);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+inline unsigned grid_for(uint64_t n, unsigned block = 256)
+{
+    uint64_t g = (n + block - 1) / block;
+    const uint64_t cap = 256ull * 256ull;     // grid-stride beyond this
+    if (g > cap) g = cap;
+    if (g == 0) g = 1;
+    return (unsigned)g;
+}
+
+int check_dtype(int dt)
+{
+    if (dt != XPR_F32 && dt != XPR_F16 && dt != XPR_BF16)
+        return fail(XPR_E_UNSUPPORTED, "unknown dtype " + std::to_string(dt));
+    return XPR_OK;
+}
+
+int check_args(const xpr_proj_args *a, SlabShape &s)
+{
+    if (a == nullptr) return fail(XPR_E_INVALID, "null xpr_proj_args");
+    const int64_t nx = (int64_t)a->vmax[0] - a->vmin[0];
+    const int64_t nl = (int64_t)a->vmax[1] - a->vmin[1];
+    const int64_t nk = (int64_t)a->vmax[2] - a->vmin[2];
+    if (nx < 0 || nl < 0 || nk < 0) return fail(XPR_E_INVALID, "volumeMax < volumeMin");
+    if (a->modules == 0) return fail(XPR_E_INVALID, "modules must be >= 1");
+    if (a->line_rows < 0) return fail(XPR_E_INVALID, "line_rows < 0");
+    if (nx > (1 << 24) || nk > (1 << 24) || nl > (1 << 24))
+        return fail(XPR_E_INVALID, "slab edge above 2^24 voxels");
+    if ((uint64_t)a->det_width * (uint64_t)a->line_rows > 0xffffffffull)
+        return fail(XPR_E_INVALID, "detector view above 2^32 pixels");
+    s.nx = (int32_t)nx;
+    s.nl = (int32_t)nl;
+    s.nk = (int32_t)nk;
+    s.nrows = nl * nk;
+    s.row_base = 0;
+    return XPR_OK;
+}
+
+constexpr int kJ = 4;
+
+template <typename Launch>
+int launch_rows(SlabShape s, Launch &&launch)
+{
+    if (s.nx == 0 || s.nrows == 0) return XPR_OK;
+    s.nseg = (s.nx + 64 * kJ - 1) / (64 * kJ);
+    // keep grid * block below 2^31 per launch
+    const int64_t max_blocks = (int64_t(1) << 31) / kThreads;
+    const int64_t rows_per_launch = (max_blocks / s.nseg) * kWaves;
+    for (int64_t r0 = 0; r0 < s.nrows; r0 += rows_per_launch) {
+        SlabShape t = s;
+        t.row_base = r0;
+        const int64_t rows = std::min<int64_t>(rows_per_launch, s.nrows - r0);
+        const int64_t blocks = ((rows + kWaves - 1) / kWaves) * s.nseg;
+        launch((unsigned)blocks, t);
+        XPR_LAUNCH_CHECK();
+    }
+    return XPR_OK;
+}
+
+template <typename VS, typename ES>
+int backward_impl(hipStream_t st, void *vol, const void *err, const float *pd,
+                  const xpr_proj_args &a, SlabShape s, uint64_t *empty)
+{
+    return launch_rows(s, [&](unsigned blocks, const SlabShape &t) {
+        hipLaunchKernelGGL((k_backward<VS, ES, kJ>), dim3(blocks), dim3(kThreads), 0, st,
+                           (typename VS::T *)vol, (const typename ES::T *)err, pd, a, t,
+                           (unsigned long long *)empty);
+    });
+}
+
+template <typename VS, typename AccT>
+int forward_impl(hipStream_t st, const void *vol, void *sums, const float *pd,
+                 const xpr_proj_args &a, SlabShape s)
+{
+    return launch_rows(s, [&](unsigned blocks, const SlabShape &t) {
+        hipLaunchKernelGGL((k_forward_global<VS, AccT, kJ>), dim3(blocks), dim3(kThreads), 0,
+                           st, (const typename VS::T *)vol, (AccT *)sums, pd, a, t);
+    });
+}
+
+template <typename IS, typename AccT>
+int error_impl2(hipStream_t st, const void *img, const void *sums, void *err, int err_dtype,
+                uint64_t n, float w)
+{
+    const unsigned g = grid_for(n);
+    switch (err_dtype) {
+    case XPR_F32:
+        hipLaunchKernelGGL((k_error<IS, AccT, F32>), dim3(g), dim3(256), 0, st,
+                           (const typename IS::T *)img, (const AccT *)sums, (float *)err, n, w);
+        break;
+    case XPR_F16:
+        hipLaunchKernelGGL((k_error<IS, AccT, F16>), dim3(g), dim3(256), 0, st,
+                           (const typename IS::T *)img, (const AccT *)sums, (uint16_t *)err, n, w);
+        break;
+    default:
+        hipLaunchKernelGGL((k_error<IS, AccT, BF16>), dim3(g), dim3(256), 0, st,
+                           (const typename IS::T *)img, (const AccT *)sums, (uint16_t *)err, n, w);
+        break;
+    }
+    XPR_LAUNCH_CHECK();
+    return XPR_OK;
+}
+
+template <typename IS>
+int error_impl(hipStream_t st, const void *img, const void *sums, int acc_bits, void *err,
+               int err_dtype, uint64_t n, float w)
+{
+    if (acc_bits == XPR_ACC_U32)
+        return error_impl2<IS, uint32_t>(st, img, sums, err, err_dtype, n, w);
+    return error_impl2<IS, uint64_t>(st, img, sums, err, err_dtype, n, w);
+}
+
+template <typename DS>
+int convert_from(hipStream_t st, void *dst, const void *src, int src_dtype, uint64_t n)
+{
+    const unsigned g = grid_for(n);
+    switch (src_dtype) {
+    case XPR_F32:
+        hipLaunchKernelGGL((k_convert<DS, F32>), dim3(g), dim3(256), 0, st,
+                           (typename DS::T *)dst, (const float *)src, n);
+        break;
+    case XPR_F16:
+        hipLaunchKernelGGL((k_convert<DS, F16>), dim3(g), dim3(256), 0, st,
+                           (typename DS::T *)dst, (const uint16_t *)src, n);
+        break;
+    default:
+        hipLaunchKernelGGL((k_convert<DS, BF16>), dim3(g), dim3(256), 0, st,
+                           (typename DS::T *)dst, (const uint16_t *)src, n);
+        break;
+    }
+    XPR_LAUNCH_CHECK();
+    return XPR_OK;
+}
+
+} // namespace
+
+int xprint_fail(int code, const char *msg) { return fail(code, msg); }
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+int xpr_abi_version(void) { return XPR_ABI_VERSION; }
+
+const char *xpr_last_error(void) { return g_last_error.c_str(); }
+
+int xpr_set_device(int device)
+{
+    XPR_HIP_CHECK(hipSetDevice(device));
+    return XPR_OK;
+}
+
+int xpr_mem_info(size_t *free_bytes, size_t *total_bytes)
+{
+    if (!free_bytes || !total_bytes) return fail(XPR_E_INVALID, "null output");
+    XPR_HIP_CHECK(hipMemGetInfo(free_bytes, total_bytes));
+    return XPR_OK;
+}
+
+int xpr_alloc(void **dptr, size_t bytes)
+{
+    if (!dptr) return fail(XPR_E_INVALID, "null output");
+    *dptr = nullptr;
+    if (bytes == 0) bytes = 1;
+    hipError_t e = hipMalloc(dptr, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(XPR_E_NOMEM, "hipMalloc(" + std::to_string(bytes) + "): " +
+                                     hipGetErrorString(e));
+    }
+    return XPR_OK;
+}
+
+int xpr_free(void *dptr)
+{
+    XPR_HIP_CHECK(hipFree(dptr));
+    return XPR_OK;
+}
+
+int xpr_host_alloc(void **hptr, size_t bytes)
+{
+    if (!hptr) return fail(XPR_E_INVALID, "null output");
+    hipError_t e = hipHostMalloc(hptr, bytes ? bytes : 1, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(XPR_E_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    }
+    return XPR_OK;
+}
+
+int xpr_host_free(void *hptr)
+{
+    XPR_HIP_CHECK(hipHostFree(hptr));
+    return XPR_OK;
+}
+
+int xpr_upload(xpr_stream s, void *dst, const void *src, size_t bytes)
+{
+    XPR_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)s));
+    return XPR_OK;
+}
+
+int xpr_download(xpr_stream s, void *dst, const void *src, size_t bytes)
+{
+    XPR_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)s));
+    return XPR_OK;
+}
+
+int xpr_stream_create(xpr_stream *s)
+{
+    if (!s) return fail(XPR_E_INVALID, "null output");
+    hipStream_t h;
+    XPR_HIP_CHECK(hipStreamCreateWithFlags(&h, hipStreamNonBlocking));
+    *s = (xpr_stream)h;
+    return XPR_OK;
+}
+
+int xpr_stream_destroy(xpr_stream s)
+{
+    XPR_HIP_CHECK(hipStreamDestroy((hipStream_t)s));
+    return XPR_OK;
+}
+
+int xpr_stream_sync(xpr_stream s)
+{
+    XPR_HIP_CHECK(hipStreamSynchronize((hipStream_t)s));
+    return XPR_OK;
+}
+
+int xpr_fill_u32(xpr_stream s, uint32_t *dst, uint32_t value, uint64_t n)
+{
+    if (n == 0) return XPR_OK;
+    if (!dst) return fail(XPR_E_INVALID, "null buffer");
+    hipLaunchKernelGGL(k_fill_raw<uint32_t>, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)s,
+                       dst, value, n);
+    XPR_LAUNCH_CHECK();
+    return XPR_OK;
+}
+
+int xpr_fill_u64(xpr_stream s, uint64_t *dst, uint64_t value, uint64_t n)
+{
+    if (n == 0) return XPR_OK;
+    if (!dst) return fail(XPR_E_INVALID, "null buffer");
+    hipLaunchKernelGGL(k_fill_raw<uint64_t>, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)s,
+                       dst, value, n);
+    XPR_LAUNCH_CHECK();
+    return XPR_OK;
+}
+
+int xpr_fill(xpr_stream s, void *dst, int dtype, float value, uint64_t n)
+{
+    if (int rc = check_dtype(dtype)) return rc;
+    if (n == 0) return XPR_OK;
+    if (!dst) return fail(XPR_E_INVALID, "null buffer");
+    if (dtype == XPR_F32) {
+        hipLaunchKernelGGL(k_fill_raw<float>, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)s,
+                           (float *)dst, value, n);
+    } else {
+        uint16_t bits;
+        if (dtype == XPR_F16) {
+            bits = __builtin_bit_cast(uint16_t, (_Float16)value);
+        } else {
+            uint32_t b;
+            std::memcpy(&b, &value, 4);
+            if ((b & 0x7fffffffu) > 0x7f800000u)
+                bits = (uint16_t)((b >> 16) | 0x40u);
+            else
+                bits = (uint16_t)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
+        }
+        hipLaunchKernelGGL(k_fill_raw<uint16_t>, dim3(grid_for(n)), dim3(256), 0,
+                           (hipStream_t)s, (uint16_t *)dst, bits, n);
+    }
+    XPR_LAUNCH_CHECK();
+    return XPR_OK;
+}
+
+int xpr_convert(xpr_stream s, void *dst, int dst_dtype, const void *src, int src_dtype,
+                uint64_t n)
+{
+    if (int rc = check_dtype(dst_dtype)) return rc;
+    if (int rc = check_dtype(src_dtype)) return rc;
+    if (n == 0) return XPR_OK;
+    if (!dst || !src) return fail(XPR_E_INVALID, "null buffer");
+    hipStream_t st = (hipStream_t)s;
+    switch (dst_dtype) {
+    case XPR_F32: return convert_from<F32>(st, dst, src, src_dtype, n);
+    case XPR_F16: return convert_from<F16>(st, dst, src, src_dtype, n);
+    default: return convert_from<BF16>(st, dst, src, src_dtype, n);
+    }
+}
+
+int xpr_forward(xpr_stream s, const void *volume, int vol_dtype, void *sums, int acc_bits,
+                const float *projdata, const xpr_proj_args *a)
+{
+    SlabShape sh;
+    if (int rc = check_args(a, sh)) return rc;
+    if (int rc = check_dtype(vol_dtype)) return rc;
+    if (acc_bits != XPR_ACC_U32 && acc_bits != XPR_ACC_U64)
+        return fail(XPR_E_INVALID, "acc_bits must be 32 or 64");
+    if (a->n_views == 0 || sh.nx == 0 || sh.nrows == 0) return XPR_OK;
+    if (!volume || !sums || !projdata) return fail(XPR_E_INVALID, "null buffer");
+    hipStream_t st = (hipStream_t)s;
+    if (acc_bits == XPR_ACC_U32) {
+        switch (vol_dtype) {
+        case XPR_F32: return forward_impl<F32, uint32_t>(st, volume, sums, projdata, *a, sh);
+        case XPR_F16: return forward_impl<F16, uint32_t>(st, volume, sums, projdata, *a, sh);
+        default: return forward_impl<BF16, uint32_t>(st, volume, sums, projdata, *a, sh);
+        }
+    }
+    switch (vol_dtype) {
+    case XPR_F32: return forward_impl<F32, uint64_t>(st, volume, sums, projdata, *a, sh);
+    case XPR_F16: return forward_impl<F16, uint64_t>(st, volume, sums, projdata, *a, sh);
+    default: return forward_impl<BF16, uint64_t>(st, volume, sums, projdata, *a, sh);
+    }
+}
+
+int xpr_error(xpr_stream s, const void *images, int img_dtype, const void *sums, int acc_bits,
+              void *err, int err_dtype, uint64_t n, float weight)
+{
+    if (int rc = check_dtype(img_dtype)) return rc;
+    if (int rc = check_dtype(err_dtype)) return rc;
+    if (acc_bits != XPR_ACC_U32 && acc_bits != XPR_ACC_U64)
+        return fail(XPR_E_INVALID, "acc_bits must be 32 or 64");
+    if (n == 0) return XPR_OK;
+    if (!images || !sums || !err) return fail(XPR_E_INVALID, "null buffer");
+    if (err == sums && !(acc_bits == XPR_ACC_U32 && err_dtype == XPR_F32))
+        return fail(XPR_E_INVALID, "in-place error needs 32-bit sums and f32 error");
+    hipStream_t st = (hipStream_t)s;
+    switch (img_dtype) {
+    case XPR_F32: return error_impl<F32>(st, images, sums, acc_bits, err, err_dtype, n, weight);
+    case XPR_F16: return error_impl<F16>(st, images, sums, acc_bits, err, err_dtype, n, weight);
+    default: return error_impl<BF16>(st, images, sums, acc_bits, err, err_dtype, n, weight);
+    }
+}
+
+int xpr_backward(xpr_stream s, void *volume, int vol_dtype, const void *err, int err_dtype,
+                 const float *projdata, const xpr_proj_args *a, uint64_t *empty_count)
+{
+    SlabShape sh;
+    if (int rc = check_args(a, sh)) return rc;
+    if (int rc = check_dtype(vol_dtype)) return rc;
+    if (int rc = check_dtype(err_dtype)) return rc;
+    if (sh.nx == 0 || sh.nrows == 0) return XPR_OK;
+    if (!volume || !err || !projdata) return fail(XPR_E_INVALID, "null buffer");
+    hipStream_t st = (hipStream_t)s;
+#define XPR_BW(VS, ES) backward_impl<VS, ES>(st, volume, err, projdata, *a, sh, empty_count)
+    switch (vol_dtype * 3 + err_dtype) {
+    case XPR_F32 * 3 + XPR_F32: return XPR_BW(F32, F32);
+    case XPR_F32 * 3 + XPR_F16: return XPR_BW(F32, F16);
+    case XPR_F32 * 3 + XPR_BF16: return XPR_BW(F32, BF16);
+    case XPR_F16 * 3 + XPR_F32: return XPR_BW(F16, F32);
+    case XPR_F16 * 3 + XPR_F16: return XPR_BW(F16, F16);
+    case XPR_F16 * 3 + XPR_BF16: return XPR_BW(F16, BF16);
+    case XPR_BF16 * 3 + XPR_F32: return XPR_BW(BF16, F32);
+    case XPR_BF16 * 3 + XPR_F16: return XPR_BW(BF16, F16);
+    default: return XPR_BW(BF16, BF16);
+    }
+#undef XPR_BW
+}
+
+int xpr_minmax_init(xpr_stream s, int32_t *minmax)
+{
+    if (!minmax) return fail(XPR_E_INVALID, "null buffer");
+    // running values start at FLT_MAX / FLT_MIN (data_loader.hpp:287-288)
+    const uint32_t init[2] = {0x7f7fffffu, 0x00800000u};
+    hipLaunchKernelGGL(k_fill_raw<uint32_t>, dim3(1), dim3(64), 0, (hipStream_t)s,
+                       (uint32_t *)minmax, init[0], (uint64_t)1);
+    hipLaunchKernelGGL(k_fill_raw<uint32_t>, dim3(1), dim3(64), 0, (hipStream_t)s,
+                       (uint32_t *)minmax + 1, init[1], (uint64_t)1);
+    XPR_LAUNCH_CHECK();
+    return XPR_OK;
+}
+
+int xpr_minmax_normal(xpr_stream s, const float *data, uint64_t n, int32_t *minmax)
+{
+    if (n == 0) return XPR_OK;
+    if (!data || !minmax) return fail(XPR_E_INVALID, "null buffer");
+    hipLaunchKernelGGL(k_minmax, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)s, data, n,
+                       minmax);
+    XPR_LAUNCH_CHECK();
+    return XPR_OK;
+}
+
+int xpr_minmax_read(const int32_t *minmax_host, float *mn, float *mx)
+{
+    if (!minmax_host || !mn || !mx) return fail(XPR_E_INVALID, "null pointer");
+    for (int i = 0; i < 2; ++i) {
+        int32_t k = minmax_host[i];
+        int32_t b = k < 0 ? (k ^ 0x7fffffff) : k;
+        float f;
+        std::memcpy(&f, &b, 4);
+        (i == 0 ? *mn : *mx) = f;
+    }
+    return XPR_OK;
+}
+
+int xpr_preprocess(xpr_stream s, const float *in, void *out, int out_dtype, uint64_t n,
+                   int normalize, int mlog, float mn, float mx)
+{
+    if (int rc = check_dtype(out_dtype)) return rc;
+    if (n == 0) return XPR_OK;
+    if (!in || !out) return fail(XPR_E_INVALID, "null buffer");
+    hipStream_t st = (hipStream_t)s;
+    const unsigned g = grid_for(n);
+    switch (out_dtype) {
+    case XPR_F32:
+        hipLaunchKernelGGL(k_preprocess<F32>, dim3(g), dim3(256), 0, st, in, (float *)out, n,
+                           normalize, mlog, mn, mx);
+        break;
+    case XPR_F16:
+        hipLaunchKernelGGL(k_preprocess<F16>, dim3(g), dim3(256), 0, st, in, (uint16_t *)out,
+                           n, normalize, mlog, mn, mx);
+        break;
+    default:
+        hipLaunchKernelGGL(k_preprocess<BF16>, dim3(g), dim3(256), 0, st, in, (uint16_t *)out,
+                           n, normalize, mlog, mn, mx);
+        break;
+    }
+    XPR_LAUNCH_CHECK();
+    return XPR_OK;
+}
+
+} // extern "C"
