@@ -59,6 +59,11 @@ typedef struct {
     int32_t  line_rows;    /* lineOffsetY.y: detector rows held per view             */
     float    origin[3];    /* origin (mm) of voxel (0,0,0), jitter included          */
     float    voxel[3];     /* voxelSize (mm)                                         */
+    /* Optional device array of n_views detector view indices: view j of this
+     * call reads ProjData record j and detector view view_index[j] (the
+     * per-layer view lists of ForwardH/BackwardH, KernelForwardH.cl:8-37,
+     * geometry.hpp:141-171).  NULL: view j is detector view j. */
+    const uint32_t *view_index;
 } xpr_proj_args;
 
 /* ---- library / device --------------------------------------------------- */
@@ -84,6 +89,15 @@ int xpr_download(xpr_stream s, void *dst, const void *src, size_t bytes); /* asy
 int xpr_stream_create(xpr_stream *s);
 int xpr_stream_destroy(xpr_stream s);
 int xpr_stream_sync(xpr_stream s);
+/* Events: ordering between a copy stream and the compute stream, and device
+ * timing (the reference enables queue profiling, gpu/context.hpp:80-84). */
+typedef void *xpr_event;    /* a hipEvent_t */
+int xpr_event_create(xpr_event *e);
+int xpr_event_destroy(xpr_event e);
+int xpr_event_record(xpr_event e, xpr_stream s);
+int xpr_stream_wait_event(xpr_stream s, xpr_event e);
+int xpr_event_sync(xpr_event e);
+int xpr_event_elapsed_ms(xpr_event start, xpr_event stop, float *ms);
 /* Fill kernels (enqueueFillBuffer, gpu/context.hpp:222-239): n elements. */
 int xpr_fill_u32(xpr_stream s, uint32_t *dst, uint32_t value, uint64_t n);
 int xpr_fill_u64(xpr_stream s, uint64_t *dst, uint64_t value, uint64_t n);
@@ -120,6 +134,11 @@ int xpr_error(xpr_stream s, const void *images, int img_dtype, const void *sums,
  * to it. */
 int xpr_backward(xpr_stream s, void *volume, int vol_dtype, const void *err, int err_dtype,
                  const float *projdata, const xpr_proj_args *a, uint64_t *empty_count);
+
+/* Q.22 sums to float (FIXED_TO_FLOAT, constants.cl:21) for n pixels; used to
+ * write reprojected views (Reconstruction::forward_exec,
+ * src/reconstruction.hpp:139-153). */
+int xpr_q22_to_float(xpr_stream s, const void *sums, int acc_bits, float *out, uint64_t n);
 
 /* ---- preprocessing (DataLoader::initializeTempImages,
  *      dataset/data_loader.hpp:48-101) -------------------------------------- */
@@ -189,6 +208,17 @@ typedef struct {
 int xpr_synth_line_integrals(xpr_stream s, const double *frames, uint32_t n_views,
                              uint32_t modules, uint32_t det_width, uint32_t det_height,
                              const xpr_ellipsoid *ellipsoids, int n_ellipsoids, double *out);
+
+/* ---- the XPRecon drop-in (src/main.cpp:62-137) ----------------------------- */
+
+/* Runs one reconstruction in-process from a parameter file path or inline JSON
+ * (the XPRecon CLI body).  Returns the CLI's exit codes: 0 OK, -1 arguments,
+ * -2 parameters, -3 reconstruction.  If record is non-NULL it receives the
+ * JSON run record (truncated to record_cap). */
+int xpr_reconstruct(const char *params, char *record, size_t record_cap);
+/* The default parameters as JSON (XPRecon -template).  0 on success, -1 if
+ * cap is too small. */
+int xpr_template_json(char *out, size_t cap);
 
 #ifdef __cplusplus
 }

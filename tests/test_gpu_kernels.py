@@ -259,3 +259,26 @@ def test_synthetic_line_integrals_match_oracle(cuda):
         ref = O.line_integrals(spec, a, oe)
         np.testing.assert_allclose(got[j], ref, rtol=1e-9, atol=1e-12)
     assert got.max() > 0.1
+
+
+@pytest.mark.parametrize("cap", ["0", "40", "2048"])
+@pytest.mark.parametrize("acc", [32, 64])
+def test_forward_paths_agree_with_large_values(cuda, monkeypatch, cap, acc):
+    """Direct atomics (cap 0), a tile too small for most footprints (cap 40) and
+    the full tile give the oracle's sums bit for bit, including voxels whose
+    Q.22 value does not fit 32 bits (u64 exact, u32 saturated then wrapped)."""
+    import torch
+    import xpulse_recon_amd as X
+    monkeypatch.setenv("XPR_FORWARD_TILE_CAP", cap)
+    spec, pd, d = _case(1)
+    nx, nl, nk = (d["vmax"][i] - d["vmin"][i] for i in range(3))
+    rng = np.random.default_rng(300)
+    vol = rng.uniform(0.0, 3.0, (nl, nk, nx)).astype(np.float32)
+    vol[5, 7, 3:9] = 2000.0
+    vol[20, 1, 50] = 1023.9999
+    shape = (d["n_views"], d["line_rows"], d["det_width"])
+    ref = np.zeros(shape, np.uint32 if acc == 32 else np.uint64)
+    O.forward_f32(vol, ref, pd, oracle_geom(d))
+    sums = torch.zeros(shape, dtype=torch.int32 if acc == 32 else torch.int64, device=cuda)
+    X.forward(torch.from_numpy(vol).to(cuda), sums, torch.from_numpy(pd).to(cuda), _args(d))
+    np.testing.assert_array_equal(sums.cpu().numpy().view(ref.dtype), ref)

@@ -63,6 +63,8 @@ def _check_sizes(volume, det, projdata, a: ProjArgs):
     if volume.numel() < n[0] * n[1] * n[2] or not volume.is_contiguous():
         raise XprError(f"volume holds {volume.numel()} voxels, slab needs {n[0] * n[1] * n[2]}")
     need = a.n_views * a.line_rows * a.det_width
+    if a.view_index:
+        need = 0        # the caller's view list addresses the detector buffer
     if det.numel() < need or not det.is_contiguous():
         raise XprError(f"detector buffer holds {det.numel()} pixels, needs {need}")
     if projdata.numel() < 20 * a.n_views * a.modules or projdata.dtype.itemsize != 4:

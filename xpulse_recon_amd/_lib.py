@@ -36,16 +36,19 @@ class ProjArgs(C.Structure):
         ("line_rows", C.c_int32),
         ("origin", C.c_float * 3),
         ("voxel", C.c_float * 3),
+        ("view_index", C.c_void_p),
     ]
 
 
-def proj_args(det_width, n_views, modules, vmin, vmax, line_offset, line_rows, origin, voxel):
+def proj_args(det_width, n_views, modules, vmin, vmax, line_offset, line_rows, origin, voxel,
+              view_index=None):
     a = ProjArgs()
     a.det_width, a.n_views, a.modules = int(det_width), int(n_views), int(modules)
     for i in range(3):
         a.vmin[i], a.vmax[i] = int(vmin[i]), int(vmax[i])
         a.origin[i], a.voxel[i] = float(origin[i]), float(voxel[i])
     a.line_offset, a.line_rows = int(line_offset), int(line_rows)
+    a.view_index = None if view_index is None else int(view_index)
     return a
 
 
@@ -91,6 +94,12 @@ _SIGS = {
     "xpr_stream_create": (C.c_int, [C.POINTER(C.c_void_p)]),
     "xpr_stream_destroy": (C.c_int, [_P]),
     "xpr_stream_sync": (C.c_int, [_P]),
+    "xpr_event_create": (C.c_int, [C.POINTER(C.c_void_p)]),
+    "xpr_event_destroy": (C.c_int, [_P]),
+    "xpr_event_record": (C.c_int, [_P, _P]),
+    "xpr_stream_wait_event": (C.c_int, [_P, _P]),
+    "xpr_event_sync": (C.c_int, [_P]),
+    "xpr_event_elapsed_ms": (C.c_int, [_P, _P, C.POINTER(C.c_float)]),
     "xpr_fill_u32": (C.c_int, [_P, _P, C.c_uint32, C.c_uint64]),
     "xpr_fill_u64": (C.c_int, [_P, _P, C.c_uint64, C.c_uint64]),
     "xpr_fill": (C.c_int, [_P, _P, C.c_int, C.c_float, C.c_uint64]),
@@ -98,6 +107,7 @@ _SIGS = {
     "xpr_forward": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, _P, C.POINTER(ProjArgs)]),
     "xpr_error": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, _P, C.c_int, C.c_uint64, C.c_float]),
     "xpr_backward": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, _P, C.POINTER(ProjArgs), _P]),
+    "xpr_q22_to_float": (C.c_int, [_P, _P, C.c_int, _P, C.c_uint64]),
     "xpr_minmax_init": (C.c_int, [_P, _P]),
     "xpr_minmax_normal": (C.c_int, [_P, _P, C.c_uint64, _P]),
     "xpr_minmax_read": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
@@ -105,6 +115,8 @@ _SIGS = {
                                  C.c_float, C.c_float]),
     "xpr_geometry": (C.c_int, [C.POINTER(GeometryParams), _P, C.POINTER(Grid)]),
     "xpr_geometry_frames": (C.c_int, [C.POINTER(GeometryParams), _P]),
+    "xpr_reconstruct": (C.c_int, [C.c_char_p, C.c_char_p, C.c_size_t]),
+    "xpr_template_json": (C.c_int, [C.c_char_p, C.c_size_t]),
     "xpr_synth_line_integrals": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, C.c_uint32,
                                            C.c_uint32, _P, C.c_int, _P]),
 }

@@ -31,17 +31,35 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
+BIN_DIR = os.path.join(HERE, "bin")
+CLI = os.path.join(BIN_DIR, "XPRecon")
+HOST_SRCS = ["json.cpp", "params.cpp", "io.cpp", "recon.cpp", "capi.cpp"]
+
+
 def build_hip_lib() -> str:
     os.makedirs(LIB_DIR, exist_ok=True)
     srcs = [os.path.join(HERE, "csrc", f) for f in ("xpr_kernels.hip", "geometry.cpp", "synth.hip")]
+    srcs += [os.path.join(HERE, "csrc", "host", f) for f in HOST_SRCS]
     tmp = HIP_LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *srcs])
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *srcs, "-ldl", "-lpthread"])
     os.replace(tmp, HIP_LIB)
     return HIP_LIB
 
 
+def build_cli() -> str:
+    """The XPRecon command line, linked to the library with an $ORIGIN rpath."""
+    os.makedirs(BIN_DIR, exist_ok=True)
+    tmp = CLI + ".tmp"
+    _run([HIPCC, "-O2", "-std=c++17", "-ffp-contract=off", "-o", tmp,
+          os.path.join(HERE, "csrc", "host", "main.cpp"), "-L" + LIB_DIR, "-lxpr_hip",
+          "-Wl,-rpath,$ORIGIN/../lib", "-lpthread"])
+    os.replace(tmp, CLI)
+    return CLI
+
+
 def build_all() -> None:
     build_hip_lib()
+    build_cli()
 
 
 if __name__ == "__main__":

@@ -1,0 +1,1034 @@
+#include "recon.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <filesystem>
+#include <functional>
+#include <iostream>
+#include <mutex>
+#include <thread>
+
+#include "io.hpp"
+
+namespace fs = std::filesystem;
+
+namespace xpr {
+
+void check(int rc, const char *what)
+{
+    if (rc != XPR_OK) throw XprFailure(std::string(what) + ": " + xpr_last_error());
+}
+
+// ---------------------------------------------------------------------------
+// buffers
+// ---------------------------------------------------------------------------
+DevBuf::~DevBuf() { reset(); }
+DevBuf &DevBuf::operator=(DevBuf &&o) noexcept
+{
+    if (this != &o) {
+        reset();
+        p_ = o.p_;
+        n_ = o.n_;
+        o.p_ = nullptr;
+        o.n_ = 0;
+    }
+    return *this;
+}
+void DevBuf::alloc(size_t bytes)
+{
+    reset();
+    check(xpr_alloc(&p_, bytes), "device allocation");
+    n_ = bytes;
+}
+void DevBuf::reset()
+{
+    if (p_) xpr_free(p_);
+    p_ = nullptr;
+    n_ = 0;
+}
+HostBuf::~HostBuf()
+{
+    if (p_) xpr_host_free(p_);
+}
+void HostBuf::alloc(size_t bytes)
+{
+    if (p_) xpr_host_free(p_);
+    p_ = nullptr;
+    check(xpr_host_alloc(&p_, bytes), "pinned host allocation");
+    n_ = bytes;
+}
+
+namespace {
+
+int io_threads(const Parameters &p)
+{
+    int t = p.xpr.threads > 0 ? p.xpr.threads : int(std::thread::hardware_concurrency());
+    return std::max(1, std::min(t, 16));
+}
+
+// parallel for over [0, n) on up to `threads` host threads
+void parallel_for(int64_t n, int threads, const std::function<void(int64_t)> &f)
+{
+    if (n <= 0) return;
+    threads = int(std::min<int64_t>(threads, n));
+    if (threads <= 1) {
+        for (int64_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::atomic<int64_t> next{0};
+    std::vector<std::thread> pool;
+    std::exception_ptr err;
+    std::mutex m;
+    for (int t = 0; t < threads; ++t)
+        pool.emplace_back([&] {
+            for (int64_t i; (i = next.fetch_add(1)) < n;) {
+                try {
+                    f(i);
+                } catch (...) {
+                    std::lock_guard<std::mutex> g(m);
+                    if (!err) err = std::current_exception();
+                }
+            }
+        });
+    for (auto &th : pool) th.join();
+    if (err) std::rethrow_exception(err);
+}
+
+double now_ms()
+{
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Event {
+    xpr_event e = nullptr;
+    Event() { check(xpr_event_create(&e), "event"); }
+    ~Event() { if (e) xpr_event_destroy(e); }
+};
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Dataset
+// ---------------------------------------------------------------------------
+Dataset::Dataset(Parameters &p) : p_(p)
+{
+    // collectFromDirectory + checkTiffFileInfos (dataset.hpp:214-269)
+    if (!fs::is_directory(p.recon.input))
+        throw std::runtime_error("input directory not found: " + p.recon.input);
+    for (const std::string &f : list_tiffs(p.recon.input)) {
+        TiffInfo i = tiff_info(f);
+        if (i.sample_format != 3 || i.bits != 32)
+            throw std::runtime_error("Trying to load a tiff with an incompatble format (should "
+                                     "be floating-point 32bits)");
+        if (i.width == 0 || i.height == 0)
+            throw std::runtime_error("Trying to load a tiff with an incorrect size.");
+        if (dwidth_ == 0 && dheight_ == 0) {
+            dwidth_ = i.width;
+            dheight_ = i.height;
+        }
+        if (i.width == dwidth_ && i.height == dheight_) files_.push_back(f);
+    }
+    if (files_.empty()) throw std::runtime_error("No valid TIFF file found in the directory.");
+    nproj_ = int64_t(files_.size());
+    if (nproj_ / p.recon.sit < 1)
+        throw std::runtime_error("fewer projections than sub-iterations");
+    if (!p.geometry.angle_list.empty() && int64_t(p.geometry.angle_list.size()) != nproj_)
+        throw std::runtime_error("The size of the angle list is different from the number of "
+                                 "projections");
+    if (dwidth_ % p.detector.module_number != 0)
+        std::cerr << "Warning: detector width is not a multiple of module_number\n";
+    // Geometry (geometry.hpp:35-76)
+    xpr_geometry_params g{};
+    g.so = p.geometry.so;
+    g.sd = p.geometry.sd;
+    g.angle = p.geometry.angle;
+    g.heli_offset = p.geometry.heli_offset;
+    g.px = p.detector.px;
+    g.rx = p.detector.rx;
+    g.sx = p.detector.sx;
+    g.sy = p.detector.sy;
+    g.module_angle_offset = p.detector.module_angle_offset.value_or(0.0f);
+    g.has_module_angle = p.detector.module_angle_offset.has_value();
+    g.module_center_offset_z = p.detector.module_center_offset_z;
+    g.roll = p.detector.roll;
+    g.pitch = p.detector.pitch;
+    g.yaw = p.detector.yaw;
+    g.module_number = p.detector.module_number;
+    g.dwidth = dwidth_;
+    g.dheight = dheight_;
+    g.projections = nproj_;
+    g.fp64 = p.xpr.geometry_fp64 ? 1 : 0;
+    std::vector<double> angles(p.geometry.angle_list.begin(), p.geometry.angle_list.end());
+    g.angle_list = angles.empty() ? nullptr : angles.data();
+    pd_.resize(size_t(nproj_ * p.detector.module_number * 20));
+    xpr_grid grid;
+    check(xpr_geometry(&g, pd_.data(), &grid), "geometry");
+    vx_ = float(grid.vx);
+    vwidth_ = grid.vwidth;
+    vheight_ = grid.vheight;
+    for (int i = 0; i < 3; ++i) orig_[i] = float(grid.orig[i]);
+    fs::create_directories(p.recon.output);
+    if (!p.recon.proj_output.empty()) fs::create_directories(p.recon.proj_output);
+    proj_dir_ = (fs::path(p.recon.output) / "projs").string();
+    std::cout << "Volume dimension : " << vwidth_ << "x" << vheight_ << "x" << vwidth_ << "\n"
+              << "Pixel size : " << p.detector.px << ", Voxel size : " << vx_ << std::endl;
+}
+
+Dataset::~Dataset()
+{
+    std::error_code ec;
+    fs::remove_all(proj_dir_, ec);     // cleanTempImages (data_loader.hpp:107-111)
+}
+
+std::vector<int64_t> Dataset::sit_views(int64_t sit) const
+{
+    std::vector<int64_t> v;
+    for (int64_t i = sit; i < nproj_ && int64_t(v.size()) < concurrent(); i += p_.recon.sit)
+        v.push_back(i);
+    return v;
+}
+
+std::vector<float> Dataset::sit_projdata(int64_t sit) const
+{
+    const int64_t m = p_.detector.module_number;
+    std::vector<float> out;
+    for (int64_t v : sit_views(sit))
+        out.insert(out.end(), pd_.begin() + v * m * 20, pd_.begin() + (v + 1) * m * 20);
+    return out;
+}
+
+void Dataset::initialize(int)
+{
+    const int64_t P = dwidth_ * dheight_;
+    images_.assign(size_t(nproj_), {});
+    xpr_stream s;
+    check(xpr_stream_create(&s), "stream");
+    DevBuf d_in(size_t(P) * 4), d_out(size_t(P) * 4), d_mm(8);
+    HostBuf h(size_t(P) * 4);
+    const int threads = io_threads(p_);
+    // read all TIFFs (host threads), then preprocess on the device
+    std::vector<std::vector<float>> raw(static_cast<size_t>(nproj_));
+    parallel_for(nproj_, threads, [&](int64_t i) { raw[size_t(i)] = tiff_read_f32(files_[size_t(i)]); });
+    if (p_.recon.normalize) {
+        check(xpr_minmax_init(s, d_mm.as<int32_t>()), "minmax init");
+        for (int64_t i = 0; i < nproj_; ++i) {
+            std::memcpy(h.get(), raw[size_t(i)].data(), size_t(P) * 4);
+            check(xpr_upload(s, d_in.get(), h.get(), size_t(P) * 4), "upload");
+            check(xpr_minmax_normal(s, d_in.as<float>(), uint64_t(P), d_mm.as<int32_t>()), "minmax");
+            check(xpr_stream_sync(s), "sync");
+        }
+        int32_t keys[2];
+        check(xpr_download(s, keys, d_mm.get(), 8), "download");
+        check(xpr_stream_sync(s), "sync");
+        check(xpr_minmax_read(keys, &mn_, &mx_), "minmax read");
+    }
+    for (int64_t i = 0; i < nproj_; ++i) {
+        std::memcpy(h.get(), raw[size_t(i)].data(), size_t(P) * 4);
+        check(xpr_upload(s, d_in.get(), h.get(), size_t(P) * 4), "upload");
+        check(xpr_preprocess(s, d_in.as<float>(), d_out.get(), XPR_F32, uint64_t(P),
+                             p_.recon.normalize, p_.recon.mlog, mn_, mx_), "preprocess");
+        check(xpr_download(s, h.get(), d_out.get(), size_t(P) * 4), "download");
+        check(xpr_stream_sync(s), "sync");
+        images_[size_t(i)].assign(h.as<float>(), h.as<float>() + P);
+        raw[size_t(i)].clear();
+        raw[size_t(i)].shrink_to_fit();
+    }
+    xpr_stream_destroy(s);
+}
+
+std::vector<float> Dataset::sit_images(int64_t sit, int64_t row0, int64_t rows) const
+{
+    const auto views = sit_views(sit);
+    std::vector<float> out(size_t(views.size()) * size_t(rows * dwidth_));
+    for (size_t j = 0; j < views.size(); ++j)
+        std::memcpy(out.data() + j * size_t(rows * dwidth_),
+                    images_[size_t(views[j])].data() + row0 * dwidth_, size_t(rows * dwidth_) * 4);
+    return out;
+}
+
+void Dataset::save_layer(const float *data, int64_t layer, bool finalize)
+{
+    const size_t n = size_t(vwidth_ * vwidth_);
+    if (finalize) {
+        save_final_layers(data, layer, layer + 1);
+        if (ram_layers()) {
+            if (size_t(layer) < ram_store_.size()) std::vector<float>().swap(ram_store_[size_t(layer)]);
+        } else {
+            std::error_code ec;
+            fs::remove(numbered_path(p_.recon.output, "layer", layer, 5, "zstd"), ec);
+        }
+        return;
+    }
+    if (ram_layers()) {
+        if (ram_store_.size() < size_t(vheight_)) ram_store_.resize(size_t(vheight_));
+        ram_store_[size_t(layer)].assign(data, data + n);
+    } else {
+        spill_write(numbered_path(p_.recon.output, "layer", layer, 5, "zstd"), data, n);
+    }
+}
+
+void Dataset::load_layer(int64_t layer, float *out) const
+{
+    const size_t n = size_t(vwidth_ * vwidth_);
+    if (ram_layers() && size_t(layer) < ram_store_.size() && !ram_store_[size_t(layer)].empty()) {
+        std::memcpy(out, ram_store_[size_t(layer)].data(), n * 4);
+        return;
+    }
+    const std::string z = numbered_path(p_.recon.output, "layer", layer, 5, "zstd");
+    if (fs::exists(z)) {
+        spill_read(z, out, n);
+        return;
+    }
+    int64_t w = 0, h = 0;
+    auto v = tiff_read_f32(numbered_path(p_.recon.output, "layer", layer, 5, "tif"), &w, &h);
+    if (size_t(w * h) != n) throw std::runtime_error("layer file has the wrong size");
+    std::memcpy(out, v.data(), n * 4);
+}
+
+void Dataset::save_final_layers(const float *vol, int64_t l0, int64_t l1)
+{
+    const size_t n = size_t(vwidth_ * vwidth_);
+    parallel_for(l1 - l0, io_threads(p_), [&](int64_t k) {
+        const float *src = vol + size_t(k) * n;
+        std::vector<float> tmp;
+        if (p_.recon.normalize) {               // saveLayer(finalize), data_loader.hpp:135-139
+            tmp.assign(src, src + n);
+            for (float &v : tmp) v = v * (mx_ - mn_);
+            src = tmp.data();
+        }
+        tiff_write_f32(numbered_path(p_.recon.output, "layer", l0 + k, 5, "tif"), src, vwidth_,
+                       vwidth_);
+    });
+}
+
+void Dataset::save_projection(const float *img, int64_t w, int64_t h, int64_t id) const
+{
+    // saveImage (data_loader.hpp:166-173)
+    if (p_.recon.proj_output.empty())
+        tiff_write_f32(numbered_path(p_.recon.output, "projection", id, 5, "tif"), img, w, h);
+    else
+        tiff_write_f32(numbered_path(p_.recon.proj_output, "p", id, 10, "tif"), img, w, h);
+}
+
+// ---------------------------------------------------------------------------
+// Operators
+// ---------------------------------------------------------------------------
+void Operators::setImageParameters(int64_t w, int64_t h, int64_t n)
+{
+    a_.det_width = uint32_t(w);
+    elements_ = uint64_t(w * h * n);                 // setImageParameters, :208-215
+}
+
+void Operators::setVolumeParameters(std::array<int, 3> mn, std::array<int, 3> mx, float voxel)
+{
+    for (int i = 0; i < 3; ++i) {
+        a_.vmin[i] = mn[i];
+        a_.vmax[i] = mx[i];
+        a_.voxel[i] = voxel;
+    }
+}
+
+void Operators::forward()
+{
+    check(xpr_forward(stream_, vol_, vol_dt_, sums_, acc_, pd_, &a_), "forward");
+}
+
+void Operators::error()
+{
+    check(xpr_error(stream_, img_, img_dt_, sums_, acc_, err_, err_dt_, elements_, w_), "error");
+}
+
+void Operators::backward()
+{
+    check(xpr_backward(stream_, vol_, vol_dt_, err_, err_dt_, pd_, &a_, nullptr), "backward");
+}
+
+// ---------------------------------------------------------------------------
+// Reconstruction base
+// ---------------------------------------------------------------------------
+Reconstruction::Reconstruction(Parameters &p) : p_(p)
+{
+    check(xpr_set_device(p.xpr.device), "set device");
+    ds_ = std::make_unique<Dataset>(p);
+    check(xpr_stream_create(&stream_), "stream");
+    check(xpr_stream_create(&copy_), "stream");
+    size_t fr = 0, tot = 0;
+    check(xpr_mem_info(&fr, &tot), "memory info");
+    const uint64_t reserve = 512ull << 20;                 // gpu/context.hpp:104-108
+    budget_ = p.xpr.device_budget_gb > 0.0 ? uint64_t(p.xpr.device_budget_gb * 1e9)
+                                           : (fr > reserve ? fr - reserve : 0);
+    if (p.xpr.jitter_seed >= 0) {
+        rng_.seed(uint32_t(p.xpr.jitter_seed));
+    } else {
+        std::random_device rd;                              // src/reconstruction.hpp:31,36
+        rng_.seed(rd());
+    }
+    note("method", Json::string(method_name(p.recon.method)));
+    note("device_budget_bytes", Json::number(double(budget_)));
+    Json g = Json::array();
+    for (int64_t v : {ds_->vwidth(), ds_->vheight(), ds_->vwidth()}) g.arr.push_back(Json::number(double(v)));
+    note("grid", g);
+    note("voxel_size", Json::number(ds_->vx()));
+    note("views_per_subiteration", Json::number(double(ds_->concurrent())));
+}
+
+Reconstruction::~Reconstruction()
+{
+    if (stream_) xpr_stream_destroy(stream_);
+    if (copy_) xpr_stream_destroy(copy_);
+}
+
+std::array<float, 3> Reconstruction::next_origin()
+{
+    // setOrigin(orig, {vx, vx, vx}) (src/reconstruction.hpp:268-272)
+    std::array<float, 3> o = ds_->orig();
+    Json j = Json::array();
+    for (int i = 0; i < 3; ++i) {
+        float d = 0.0f;
+        if (p_.xpr.jitter) d = std::uniform_real_distribution<float>(0.0f, ds_->vx())(rng_);
+        o[size_t(i)] += d;
+        j.arr.push_back(Json::number(double(d)));
+    }
+    jitter_log_.arr.push_back(j);
+    record_.obj.erase(std::remove_if(record_.obj.begin(), record_.obj.end(),
+                                     [](const auto &kv) { return kv.first == "jitter"; }),
+                      record_.obj.end());
+    note("jitter", jitter_log_);
+    return o;
+}
+
+void Reconstruction::check_memory(uint64_t need, const char *what)
+{
+    note("required_device_bytes", Json::number(double(need)));
+    std::cout << "Checking GPU memory requierments : " << need / (1024 * 1024) << "Mo" << std::endl;
+    if (need > budget_)
+        throw std::runtime_error(std::string("Not enough GPU memory (") + what + ": " +
+                                 std::to_string(need >> 20) + " MiB needed, budget " +
+                                 std::to_string(budget_ >> 20) + " MiB)");
+}
+
+void Reconstruction::log_sit(int64_t mit, int64_t sit, double ms)
+{
+    (void)mit;
+    (void)sit;
+    sit_ms_.arr.push_back(Json::number(ms));
+    record_.obj.erase(std::remove_if(record_.obj.begin(), record_.obj.end(),
+                                     [](const auto &kv) { return kv.first == "subiteration_ms"; }),
+                      record_.obj.end());
+    note("subiteration_ms", sit_ms_);
+}
+
+// ---------------------------------------------------------------------------
+// helpers shared by the schedules
+// ---------------------------------------------------------------------------
+namespace {
+
+// upload float32 host data into a device buffer of dtype dt (through a float
+// staging buffer when dt is 16-bit)
+void upload_as(xpr_stream s, DevBuf &dst, int dt, const float *src, size_t n, DevBuf &stage)
+{
+    if (dt == XPR_F32) {
+        check(xpr_upload(s, dst.get(), src, n * 4), "upload");
+    } else {
+        if (stage.bytes() < n * 4) stage.alloc(n * 4);
+        check(xpr_upload(s, stage.get(), src, n * 4), "upload");
+        check(xpr_convert(s, dst.get(), dt, stage.get(), XPR_F32, n), "convert");
+    }
+    check(xpr_stream_sync(s), "sync");
+}
+
+void download_f32(xpr_stream s, float *dst, const DevBuf &src, int dt, size_t n, DevBuf &stage)
+{
+    if (dt == XPR_F32) {
+        check(xpr_download(s, dst, src.get(), n * 4), "download");
+    } else {
+        if (stage.bytes() < n * 4) stage.alloc(n * 4);
+        check(xpr_convert(s, stage.get(), XPR_F32, src.get(), dt, n), "convert");
+        check(xpr_download(s, dst, stage.get(), n * 4), "download");
+    }
+    check(xpr_stream_sync(s), "sync");
+}
+
+void fill_sums(xpr_stream s, DevBuf &sums, int acc, uint64_t value, uint64_t n)
+{
+    if (acc == 64) check(xpr_fill_u64(s, sums.as<uint64_t>(), value, n), "fill");
+    else check(xpr_fill_u32(s, sums.as<uint32_t>(), uint32_t(value), n), "fill");
+}
+
+} // namespace
+
+float Reconstruction::next_weight(int64_t mit) const
+{
+    float w = p_.recon.weight;
+    for (int64_t k = 0; k < mit; ++k) w *= p_.recon.weight_factor;
+    return w;
+}
+
+// ---------------------------------------------------------------------------
+// naive: everything resident (reconstruction_n.hpp)
+// ---------------------------------------------------------------------------
+class ReconstructionN : public Reconstruction {
+public:
+    explicit ReconstructionN(Parameters &p) : Reconstruction(p)
+    {
+        check_memory(requiredGPUMemory(), "naive");
+        ds_->initialize();
+    }
+    uint64_t requiredGPUMemory() override
+    {
+        const uint64_t sv = dtype_bytes(storage()), V = uint64_t(ds_->vwidth() * ds_->vwidth() * ds_->vheight());
+        const uint64_t P = uint64_t(ds_->width() * ds_->height());
+        const uint64_t As = uint64_t(ds_->concurrent()), S = uint64_t(p_.recon.sit);
+        const uint64_t acc = uint64_t(p_.xpr.accumulator / 8);
+        uint64_t need = V * sv + S * As * P * sv + As * P * (acc + sv) + ds_->projdata().size() * 4;
+        if (storage() != XPR_F32 && p_.xpr.final_fp32) need += V * 4 + As * P * 8;
+        return need;
+    }
+    void exec() override
+    {
+        auto &ds = *ds_;
+        const int st = storage();
+        const int64_t W = ds.vwidth(), H = ds.vheight(), P = ds.width() * ds.height();
+        const int64_t As = ds.concurrent(), S = p_.recon.sit;
+        const uint64_t V = uint64_t(W * W * H);
+        DevBuf stage;
+        std::vector<DevBuf> images(static_cast<size_t>(S)), pds(static_cast<size_t>(S));
+        for (int64_t s = 0; s < S; ++s) {
+            auto img = ds.sit_images(s, 0, ds.height());
+            images[size_t(s)].alloc(img.size() * dtype_bytes(st));
+            upload_as(stream_, images[size_t(s)], st, img.data(), img.size(), stage);
+            auto pd = ds.sit_projdata(s);
+            pds[size_t(s)].alloc(pd.size() * 4);
+            check(xpr_upload(stream_, pds[size_t(s)].get(), pd.data(), pd.size() * 4), "upload");
+        }
+        DevBuf vol(V * dtype_bytes(st)), sums(uint64_t(As * P) * uint64_t(p_.xpr.accumulator / 8)),
+            err(uint64_t(As * P) * dtype_bytes(st));
+        Operators ops(stream_, p_.xpr.accumulator);
+        ops.setSumImagesBuffer(sums.get());
+        ops.setErrorBuffer(err.get(), st);
+        ops.setVolumeBuffer(vol.get(), st);
+        ops.setImageParameters(ds.width(), ds.height(), As);
+        ops.setVolumeParameters({0, 0, 0}, {int(W), int(H), int(W)}, ds.vx());
+        ops.setImageOffset(0, int(ds.height()));
+        ops.setAngleNumber(As, p_.detector.module_number);
+        check(xpr_fill(stream_, vol.get(), st, 1.0f, V), "fill");
+        fill_sums(stream_, sums, p_.xpr.accumulator, 1ull << 22, uint64_t(As * P));
+        DevBuf vol32, img32, err32;
+        int vdt = st;
+        Event e0, e1;
+        for (int64_t mit = 0; mit < p_.recon.it; ++mit) {
+            ops.setWeight(next_weight(mit));
+            for (int64_t s = 0; s < S; ++s) {
+                const bool last = mit == p_.recon.it - 1 && s == S - 1;
+                check(xpr_event_record(e0.e, stream_), "event");
+                ops.setOrigin(next_origin());
+                ops.setImagesBuffer(images[size_t(s)].get(), st);
+                ops.setProjDataBuffer(pds[size_t(s)].as<float>());
+                if (last && st != XPR_F32 && p_.xpr.final_fp32) {
+                    // final sub-iteration with float32 storage
+                    vol32.alloc(V * 4);
+                    check(xpr_convert(stream_, vol32.get(), XPR_F32, vol.get(), st, V), "convert");
+                    auto img = ds.sit_images(s, 0, ds.height());
+                    img32.alloc(img.size() * 4);
+                    upload_as(stream_, img32, XPR_F32, img.data(), img.size(), stage);
+                    err32.alloc(uint64_t(As * P) * 4);
+                    ops.setVolumeBuffer(vol32.get(), XPR_F32);
+                    ops.setImagesBuffer(img32.get(), XPR_F32);
+                    ops.setErrorBuffer(err32.get(), XPR_F32);
+                    vdt = XPR_F32;
+                }
+                if (s > 0 || mit > 0) ops.forward();
+                ops.error();
+                ops.backward();
+                fill_sums(stream_, sums, p_.xpr.accumulator, 0, uint64_t(As * P));
+                check(xpr_event_record(e1.e, stream_), "event");
+                check(xpr_event_sync(e1.e), "sync");
+                float ms = 0;
+                check(xpr_event_elapsed_ms(e0.e, e1.e, &ms), "elapsed");
+                log_sit(mit, s, ms);
+            }
+        }
+        std::vector<float> out(V);
+        download_f32(stream_, out.data(), vdt == XPR_F32 && vol32.get() ? vol32 : vol, vdt, V, stage);
+        ds.save_final_layers(out.data(), 0, H);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// slab streaming: method_1a (one IO thread), method_1b (IO thread pool),
+// method_1c (plus per-slab view culling).  The volume lives in the layer
+// store on the host; each pass streams slabs of slab_height layers through
+// two device slab buffers, uploads on the copy stream overlapping compute.
+// (reconstruction_lpln.hpp:27-96, reconstruction_lpla.hpp:29-157,
+//  reconstruction_lplh.hpp:40-219)
+// ---------------------------------------------------------------------------
+class ReconstructionSlab : public Reconstruction {
+public:
+    ReconstructionSlab(Parameters &p, int threads, bool cull) : Reconstruction(p), threads_(threads), cull_(cull)
+    {
+        check_memory(requiredGPUMemory(), method_name(p.recon.method));
+        ds_->initialize();
+    }
+    uint64_t requiredGPUMemory() override
+    {
+        const uint64_t sv = dtype_bytes(storage());
+        const uint64_t P = uint64_t(ds_->width() * ds_->height()), As = uint64_t(ds_->concurrent());
+        const uint64_t acc = uint64_t(p_.xpr.accumulator / 8);
+        const uint64_t slab = uint64_t(std::min<int64_t>(p_.xpr.slab_height, ds_->vheight())) *
+                              uint64_t(ds_->vwidth() * ds_->vwidth());
+        return 2 * slab * (sv + 4) + As * P * (sv + acc + sv + 4) + ds_->projdata().size() * 4;
+    }
+
+    void exec() override
+    {
+        auto &ds = *ds_;
+        const int st = storage();
+        const int64_t W = ds.vwidth(), H = ds.vheight(), P = ds.width() * ds.height();
+        const int64_t As = ds.concurrent(), S = p_.recon.sit;
+        const int64_t Hs = std::min<int64_t>(p_.xpr.slab_height, H);
+        const size_t layer = size_t(W * W), slab_n = size_t(Hs) * layer;
+        const int64_t nslab = (H + Hs - 1) / Hs;
+        DevBuf img(uint64_t(As * P) * dtype_bytes(st)), sums(uint64_t(As * P) * uint64_t(p_.xpr.accumulator / 8)),
+            err(uint64_t(As * P) * dtype_bytes(st)), pdb, stage;
+        DevBuf dslab[2] = {DevBuf(slab_n * dtype_bytes(st)), DevBuf(slab_n * dtype_bytes(st))};
+        DevBuf dstage[2] = {DevBuf(slab_n * 4), DevBuf(slab_n * 4)};
+        HostBuf hslab[2] = {HostBuf(slab_n * 4), HostBuf(slab_n * 4)};
+        Event up[2], done[2], e0, e1;
+        Operators ops(stream_, p_.xpr.accumulator);
+        ops.setSumImagesBuffer(sums.get());
+        ops.setErrorBuffer(err.get(), st);
+        ops.setImagesBuffer(img.get(), st);
+        ops.setImageParameters(ds.width(), ds.height(), As);
+        ops.setImageOffset(0, int(ds.height()));
+        fill_sums(stream_, sums, p_.xpr.accumulator, 1ull << 22, uint64_t(As * P));
+        for (int64_t mit = 0; mit < p_.recon.it; ++mit) {
+            ops.setWeight(next_weight(mit));
+            for (int64_t s = 0; s < S; ++s) {
+                const bool first = s == 0 && mit == 0;
+                const bool last = mit == p_.recon.it - 1 && s == S - 1;
+                check(xpr_event_record(e0.e, stream_), "event");
+                ops.setOrigin(next_origin());
+                auto im = ds.sit_images(s, 0, ds.height());
+                upload_as(stream_, img, st, im.data(), im.size(), stage);
+                auto pd = ds.sit_projdata(s);
+                pdb.alloc(pd.size() * 4);
+                check(xpr_upload(stream_, pdb.get(), pd.data(), pd.size() * 4), "upload");
+                check(xpr_stream_sync(stream_), "sync");
+                // views of each slab that can reach the detector rows held
+                std::vector<SlabViews> slab_pd;
+                if (cull_) slab_pd = culled(pd, nslab, Hs);
+                if (!first) pass(false, nslab, Hs, slab_pd, pd, pdb, dslab, dstage, hslab, up, done, ops, last);
+                ops.setAngleNumber(As, p_.detector.module_number);
+                ops.setViewIndex(nullptr);
+                ops.error();
+                pass(true, nslab, Hs, slab_pd, pd, pdb, dslab, dstage, hslab, up, done, ops, last,
+                     first);
+                fill_sums(stream_, sums, p_.xpr.accumulator, 0, uint64_t(As * P));
+                check(xpr_event_record(e1.e, stream_), "event");
+                check(xpr_event_sync(e1.e), "sync");
+                float ms = 0;
+                check(xpr_event_elapsed_ms(e0.e, e1.e, &ms), "elapsed");
+                log_sit(mit, s, ms);
+            }
+        }
+    }
+
+private:
+    int threads_;
+    bool cull_;
+    DevBuf cpd_[2], cidx_[2];
+
+    // Per slab: the views whose projection of the slab can meet detector rows
+    // [vpb, vpt) (method_1c's per-layer view lists, geometry.hpp:141-171).  A
+    // dropped view holds no in-bounds voxel of the slab, so the sums and the
+    // updates are unchanged; only the work shrinks.
+    struct SlabViews {
+        std::vector<float> pd;
+        std::vector<uint32_t> index;
+    };
+    std::vector<SlabViews> culled(const std::vector<float> &pd, int64_t nslab, int64_t Hs)
+    {
+        auto &ds = *ds_;
+        const int64_t W = ds.vwidth(), H = ds.vheight(), mods = p_.detector.module_number;
+        const auto o = ds.orig();
+        const double vx = ds.vx();
+        const int64_t nv = int64_t(pd.size() / size_t(20 * mods));
+        std::vector<SlabViews> out(static_cast<size_t>(nslab));
+        for (int64_t b = 0; b < nslab; ++b) {
+            const int64_t l0 = b * Hs, l1 = std::min(H, l0 + Hs);
+            for (int64_t v = 0; v < nv; ++v) {
+                bool keep = false;
+                for (int64_t m = 0; m < mods && !keep; ++m) {
+                    const float *r = pd.data() + 20 * (v * mods + m);
+                    double vmin = 1e300, vmax = -1e300;
+                    bool front = true;
+                    // corners of the slab's voxel positions widened by one voxel
+                    // (the origin jitter is below one voxel)
+                    for (int c = 0; c < 8; ++c) {
+                        const double x = o[0] + ((c & 1) ? double(W) : -1.0) * vx;
+                        const double y = o[1] + ((c & 2) ? double(l1) : double(l0) - 1.0) * vx;
+                        const double z = o[2] + ((c & 4) ? double(W) : -1.0) * vx;
+                        const double Y = double(r[4]) * x + double(r[5]) * y + double(r[6]) * z + double(r[7]);
+                        const double Wh = double(r[12]) * x + double(r[13]) * y + double(r[14]) * z + double(r[15]);
+                        if (!(Wh > 0)) front = false;
+                        vmin = std::min(vmin, Y / Wh);
+                        vmax = std::max(vmax, Y / Wh);
+                    }
+                    keep = !front || !(vmax < double(r[18]) - 1.0 || vmin > double(r[19]) + 1.0);
+                }
+                if (keep) {
+                    out[size_t(b)].pd.insert(out[size_t(b)].pd.end(), pd.begin() + 20 * v * mods,
+                                             pd.begin() + 20 * (v + 1) * mods);
+                    out[size_t(b)].index.push_back(uint32_t(v));
+                }
+            }
+        }
+        return out;
+    }
+
+    void pass(bool backward, int64_t nslab, int64_t Hs, const std::vector<SlabViews> &slab_pd,
+              const std::vector<float> &pd, DevBuf &pdb, DevBuf (&dslab)[2], DevBuf (&dstage)[2],
+              HostBuf (&hslab)[2], Event (&up)[2], Event (&done)[2], Operators &ops, bool last,
+              bool first = false)
+    {
+        auto &ds = *ds_;
+        const int st = storage();
+        const int64_t W = ds.vwidth(), H = ds.vheight();
+        const size_t layer = size_t(W * W);
+        (void)pd;
+        // host-side fill of slab b into pinned buffer i
+        auto load = [&](int64_t b, int i) {
+            const int64_t l0 = b * Hs, l1 = std::min(H, l0 + Hs);
+            float *dst = hslab[i].as<float>();
+            parallel_for(l1 - l0, threads_, [&](int64_t k) { ds.load_layer(l0 + k, dst + size_t(k) * layer); });
+        };
+        for (int64_t b = 0; b < nslab; ++b) {
+            const int i = int(b & 1);
+            const int64_t l0 = b * Hs, l1 = std::min(H, l0 + Hs);
+            const size_t n = size_t(l1 - l0) * layer;
+            // the previous user of buffer i (slab b-2) has finished with it
+            if (b >= 2) check(xpr_event_sync(done[i].e), "sync");
+            if (backward && first) {
+                check(xpr_fill(copy_, dslab[i].get(), st, 1.0f, n), "fill");
+            } else {
+                load(b, i);
+                if (st == XPR_F32) {
+                    check(xpr_upload(copy_, dslab[i].get(), hslab[i].get(), n * 4), "upload");
+                } else {
+                    check(xpr_upload(copy_, dstage[i].get(), hslab[i].get(), n * 4), "upload");
+                    check(xpr_convert(copy_, dslab[i].get(), st, dstage[i].get(), XPR_F32, n), "convert");
+                }
+            }
+            check(xpr_event_record(up[i].e, copy_), "event");
+            check(xpr_stream_wait_event(stream_, up[i].e), "wait");
+            ops.setVolumeBuffer(dslab[i].get(), st);
+            ops.setVolumeParameters({0, int(l0), 0}, {int(W), int(l1), int(W)}, ds.vx());
+            if (cull_) {
+                const SlabViews &sv = slab_pd[size_t(b)];
+                if (!sv.index.empty()) {
+                    cpd_[i].alloc(sv.pd.size() * 4);
+                    cidx_[i].alloc(sv.index.size() * 4);
+                    check(xpr_upload(stream_, cpd_[i].get(), sv.pd.data(), sv.pd.size() * 4), "upload");
+                    check(xpr_upload(stream_, cidx_[i].get(), sv.index.data(), sv.index.size() * 4), "upload");
+                }
+                ops.setProjDataBuffer(cpd_[i].as<float>());
+                ops.setViewIndex(cidx_[i].as<uint32_t>());
+                ops.setAngleNumber(int64_t(sv.index.size()), p_.detector.module_number);
+            } else {
+                ops.setProjDataBuffer(pdb.as<float>());
+                ops.setViewIndex(nullptr);
+                ops.setAngleNumber(ds.concurrent(), p_.detector.module_number);
+            }
+            if (backward) {
+                ops.backward();
+                if (st == XPR_F32) {
+                    check(xpr_download(stream_, hslab[i].get(), dslab[i].get(), n * 4), "download");
+                } else {
+                    check(xpr_convert(stream_, dstage[i].get(), XPR_F32, dslab[i].get(), st, n), "convert");
+                    check(xpr_download(stream_, hslab[i].get(), dstage[i].get(), n * 4), "download");
+                }
+                check(xpr_event_record(done[i].e, stream_), "event");
+                check(xpr_event_sync(done[i].e), "sync");
+                float *src = hslab[i].as<float>();
+                if (last) {
+                    ds.save_final_layers(src, l0, l1);
+                    for (int64_t l = l0; l < l1; ++l) {
+                        if (!ds.ram_layers()) {
+                            std::error_code ec;
+                            fs::remove(numbered_path(p_.recon.output, "layer", l, 5, "zstd"), ec);
+                        }
+                    }
+                } else {
+                    parallel_for(l1 - l0, threads_, [&](int64_t k) {
+                        ds.save_layer(src + size_t(k) * layer, l0 + k, false);
+                    });
+                }
+            } else {
+                ops.forward();
+                check(xpr_event_record(done[i].e, stream_), "event");
+            }
+        }
+        check(xpr_stream_sync(stream_), "sync");
+    }
+};
+
+// ---------------------------------------------------------------------------
+// method_2: independent chunks with padding and cropped detector rows
+// (reconstruction_chunk.hpp:36-131, dataset/geometry_chunk.hpp:92-197)
+// ---------------------------------------------------------------------------
+uint64_t chunk_bytes(const Chunk &c, int64_t vwidth, int64_t dwidth, int64_t concurrent, int storage)
+{
+    const uint64_t sv = dtype_bytes(storage);
+    const uint64_t vol = uint64_t(c.vSize + c.vPadTop + c.vPadBot) * uint64_t(vwidth * vwidth) * sv;
+    const uint64_t img = uint64_t(c.iSize * dwidth * concurrent);
+    return vol + img * (sv + 8 + sv);
+}
+
+std::vector<Chunk> plan_chunks(const Parameters &p, const Dataset &ds, uint64_t budget,
+                               std::vector<float> *lmin_out, std::vector<float> *lmax_out)
+{
+    const int64_t H = ds.vheight(), W = ds.vwidth(), dh = ds.height();
+    const auto o = ds.orig();
+    const float vx = ds.vx();
+    const auto &pd = ds.projdata();
+    const int64_t nrec = int64_t(pd.size() / 20);
+    // computeLayerYBounds (geometry_chunk.hpp:120-145): the four vertical
+    // edges of the volume at the top and bottom of each layer, all views
+    std::vector<float> lmin(static_cast<size_t>(H)), lmax(static_cast<size_t>(H));
+    const float cx[4] = {o[0], -o[0], o[0], -o[0]}, cz[4] = {-o[2], -o[2], o[2], o[2]};
+    for (int64_t l = 0; l < H; ++l) {
+        float mn = 1e30f, mx = -1e30f;
+        for (int64_t r = 0; r < nrec; ++r) {
+            const float *m = pd.data() + 20 * r;
+            for (int c = 0; c < 4; ++c)
+                for (int e = 0; e < 2; ++e) {
+                    const float y = o[1] + float(l + e) * vx;
+                    const float Y = m[4] * cx[c] + m[5] * y + m[6] * cz[c] + m[7];
+                    const float Wh = m[12] * cx[c] + m[13] * y + m[14] * cz[c] + m[15];
+                    const float v = Y / Wh;
+                    mn = std::min(mn, v);
+                    mx = std::max(mx, v);
+                }
+        }
+        lmin[size_t(l)] = mn;
+        lmax[size_t(l)] = mx;
+    }
+    auto yproj = [&](int64_t off, int64_t h) {
+        return std::pair<float, float>{std::min(lmin[size_t(off)], lmin[size_t(off + h - 1)]),
+                                       std::max(lmax[size_t(off)], lmax[size_t(off + h - 1)])};
+    };
+    // updateImageViewport (:164-168): rows [floor(min)-1, ceil(max)+2) within the detector
+    auto viewport = [&](Chunk &c) {
+        auto mm = yproj(c.vOffset, c.vSize);
+        c.iOffset = int64_t(std::max(std::floor(mm.first) - 1.0f, 0.0f));
+        c.iSize = int64_t(std::max(std::min(std::ceil(mm.second - float(c.iOffset)) + 2.0f,
+                                            float(dh - c.iOffset)), 0.0f));
+    };
+    // updateVolumePadding (:175-188): neighbouring layers that project into
+    // the chunk's rows, scaled by overlap_fct
+    auto padding = [&](Chunk &c) {
+        c.vPadTop = 0;
+        c.vPadBot = 0;
+        while (c.vOffset - c.vPadTop > 0 && yproj(c.vOffset - c.vPadTop - 1, 1).second > float(c.iOffset - 1))
+            ++c.vPadTop;
+        while (c.vOffset + c.vSize + c.vPadBot < H &&
+               yproj(c.vOffset + c.vSize + c.vPadBot, 1).first < float(c.iOffset + c.iSize + 1))
+            ++c.vPadBot;
+        c.vPadTop = int64_t(std::ceil(float(c.vPadTop) * p.method2.overlap_fct));
+        c.vPadBot = int64_t(std::ceil(float(c.vPadBot) * p.method2.overlap_fct));
+    };
+    auto fits = [&](const Chunk &c) {
+        return chunk_bytes(c, W, ds.width(), ds.concurrent(), p.xpr.storage) + pd.size() * 4 < budget;
+    };
+    // computeChunks (:92-114): grow each chunk layer by layer while it fits
+    std::vector<Chunk> chunks{{0, 0, 0, 0, 0, 0}};
+    for (int64_t l = 0; l < H; ++l) {
+        Chunk c = chunks.back();
+        ++c.vSize;
+        viewport(c);
+        padding(c);
+        if (fits(c)) {
+            chunks.back() = c;
+        } else {
+            if (c.vSize <= 1)
+                throw std::runtime_error("Not enough GPU memory for the chunk division (" +
+                                         std::to_string(c.vPadTop) + "-" + std::to_string(c.vSize) + "-" +
+                                         std::to_string(c.vPadBot) + ", " +
+                                         std::to_string(chunk_bytes(c, W, ds.width(), ds.concurrent(),
+                                                                    p.xpr.storage) >> 20) + "Mo)");
+            Chunk n{chunks.back().vOffset + chunks.back().vSize, 1, 0, 0, 0, 0};
+            viewport(n);
+            padding(n);
+            if (!fits(n)) throw std::runtime_error("Not enough GPU memory for the chunk division");
+            chunks.push_back(n);
+        }
+    }
+    if (lmin_out) *lmin_out = lmin;
+    if (lmax_out) *lmax_out = lmax;
+    return chunks;
+}
+
+class ReconstructionChunk : public Reconstruction {
+public:
+    explicit ReconstructionChunk(Parameters &p) : Reconstruction(p)
+    {
+        chunks_ = plan_chunks(p, *ds_, budget());
+        std::cout << "Chunks : ";
+        Json cj = Json::array();
+        for (const auto &c : chunks_) {
+            std::cout << " [" << c.vOffset << "-" << c.vOffset + c.vSize << "],";
+            Json e = Json::array();
+            for (int64_t v : {c.vOffset, c.vSize, c.iOffset, c.iSize, c.vPadTop, c.vPadBot})
+                e.arr.push_back(Json::number(double(v)));
+            cj.arr.push_back(e);
+        }
+        std::cout << std::endl;
+        note("chunks", cj);
+        check_memory(requiredGPUMemory(), "method_2");
+        ds_->initialize();
+    }
+    uint64_t requiredGPUMemory() override
+    {
+        uint64_t m = 0;
+        for (const auto &c : chunks_)
+            m = std::max(m, chunk_bytes(c, ds_->vwidth(), ds_->width(), ds_->concurrent(), storage()));
+        return m + ds_->projdata().size() * 4;
+    }
+    void exec() override
+    {
+        auto &ds = *ds_;
+        const int st = storage();
+        const int64_t W = ds.vwidth(), As = ds.concurrent(), S = p_.recon.sit, dw = ds.width();
+        const size_t layer = size_t(W * W);
+        DevBuf stage;
+        std::vector<DevBuf> pds(static_cast<size_t>(S));
+        for (int64_t s = 0; s < S; ++s) {
+            auto pd = ds.sit_projdata(s);
+            pds[size_t(s)].alloc(pd.size() * 4);
+            check(xpr_upload(stream_, pds[size_t(s)].get(), pd.data(), pd.size() * 4), "upload");
+        }
+        Event e0, e1;
+        for (const Chunk &c : chunks_) {
+            std::cout << "[" << c.vOffset << "-" << c.vOffset + c.vSize << "]... " << std::flush;
+            const int64_t l0 = c.vOffset - c.vPadTop, l1 = c.vOffset + c.vSize + c.vPadBot;
+            const uint64_t nv = uint64_t(l1 - l0) * layer, npx = uint64_t(c.iSize * dw * As);
+            DevBuf vol(nv * dtype_bytes(st)), img(npx * dtype_bytes(st)),
+                sums(npx * uint64_t(p_.xpr.accumulator / 8)), err(npx * dtype_bytes(st));
+            check(xpr_fill(stream_, vol.get(), st, 1.0f, nv), "fill");
+            fill_sums(stream_, sums, p_.xpr.accumulator, 1ull << 22, npx);
+            Operators ops(stream_, p_.xpr.accumulator);
+            ops.setSumImagesBuffer(sums.get());
+            ops.setErrorBuffer(err.get(), st);
+            ops.setImagesBuffer(img.get(), st);
+            ops.setVolumeBuffer(vol.get(), st);
+            ops.setImageParameters(dw, c.iSize, As);            // elements = W*iSize*A_s (Q10)
+            ops.setImageOffset(int(c.iOffset), int(c.iSize));
+            ops.setAngleNumber(As, p_.detector.module_number);
+            ops.setVolumeParameters({0, int(l0), 0}, {int(W), int(l1), int(W)}, ds.vx());
+            for (int64_t mit = 0; mit < p_.recon.it; ++mit) {
+                ops.setWeight(next_weight(mit));
+                for (int64_t s = 0; s < S; ++s) {
+                    check(xpr_event_record(e0.e, stream_), "event");
+                    auto im = ds.sit_images(s, c.iOffset, c.iSize);
+                    upload_as(stream_, img, st, im.data(), im.size(), stage);
+                    ops.setOrigin(next_origin());
+                    ops.setProjDataBuffer(pds[size_t(s)].as<float>());
+                    if (s > 0 || mit > 0) ops.forward();
+                    ops.error();
+                    ops.backward();
+                    fill_sums(stream_, sums, p_.xpr.accumulator, 0, npx);
+                    check(xpr_event_record(e1.e, stream_), "event");
+                    check(xpr_event_sync(e1.e), "sync");
+                    float ms = 0;
+                    check(xpr_event_elapsed_ms(e0.e, e1.e, &ms), "elapsed");
+                    log_sit(mit, s, ms);
+                }
+            }
+            // keep only the chunk's own layers (reconstruction_chunk.hpp:121-127)
+            std::vector<float> out(nv);
+            download_f32(stream_, out.data(), vol, st, nv, stage);
+            ds.save_final_layers(out.data() + size_t(c.vPadTop) * layer, c.vOffset, c.vOffset + c.vSize);
+            std::cout << "Done." << std::endl;
+        }
+    }
+
+private:
+    std::vector<Chunk> chunks_;
+};
+
+// ---------------------------------------------------------------------------
+// forward_exec: reprojection of the saved volume (src/reconstruction.hpp:66-158)
+// ---------------------------------------------------------------------------
+void Reconstruction::forward_exec()
+{
+    auto &ds = *ds_;
+    const int64_t W = ds.vwidth(), H = ds.vheight(), As = ds.concurrent(), S = p_.recon.sit;
+    const int64_t dw = ds.width(), dh = ds.height();
+    const size_t layer = size_t(W * W);
+    const int samples = 32;
+    std::vector<float> vol(layer * size_t(H));
+    parallel_for(H, io_threads(p_), [&](int64_t l) {
+        float *dst = vol.data() + size_t(l) * layer;
+        ds.load_layer(l, dst);
+        for (size_t k = 0; k < layer; ++k) dst[k] /= float(samples);
+        for (int64_t j = 0; j < W; ++j)
+            for (int64_t i = 0; i < W; ++i)
+                if (std::sqrt(std::pow(float(i) - float(W) * 0.5f, 2.0f) +
+                              std::pow(float(j) - float(W) * 0.5f, 2.0f)) >= float(W) * 0.5f - 1.0f)
+                    dst[size_t(j * W + i)] = 0.0f;
+    });
+    DevBuf dvol(vol.size() * 4), sums(uint64_t(As * dw * dh) * 8), pdb, fsum(uint64_t(As * dw * dh) * 4);
+    check(xpr_upload(stream_, dvol.get(), vol.data(), vol.size() * 4), "upload");
+    Operators ops(stream_, 64);
+    ops.setSumImagesBuffer(sums.get());
+    ops.setVolumeBuffer(dvol.get(), XPR_F32);
+    ops.setImageParameters(dw, dh, As);
+    ops.setImageOffset(0, int(dh));
+    ops.setAngleNumber(As, p_.detector.module_number);
+    ops.setVolumeParameters({0, 0, 0}, {int(W), int(H), int(W)}, ds.vx());
+    std::vector<float> out(size_t(As * dw * dh));
+    for (int64_t s = 0; s < S; ++s) {
+        auto pd = ds.sit_projdata(s);
+        pdb.alloc(pd.size() * 4);
+        check(xpr_upload(stream_, pdb.get(), pd.data(), pd.size() * 4), "upload");
+        ops.setProjDataBuffer(pdb.as<float>());
+        check(xpr_fill_u64(stream_, sums.as<uint64_t>(), 0, uint64_t(As * dw * dh)), "fill");
+        for (int k = 0; k < samples; ++k) {
+            ops.setOrigin(next_origin());
+            ops.forward();
+        }
+        check(xpr_q22_to_float(stream_, sums.as<uint64_t>(), 64, fsum.as<float>(), uint64_t(As * dw * dh)),
+              "convert sums");
+        check(xpr_download(stream_, out.data(), fsum.get(), out.size() * 4), "download");
+        check(xpr_stream_sync(stream_), "sync");
+        const auto views = ds.sit_views(s);
+        for (size_t j = 0; j < views.size(); ++j)
+            ds.save_projection(out.data() + j * size_t(dw * dh), dw, dh, views[j]);
+    }
+}
+
+std::unique_ptr<Reconstruction> make_reconstruction(Parameters &p)
+{
+    switch (p.recon.method) {
+    case Method::naive: return std::make_unique<ReconstructionN>(p);
+    case Method::method_1a: return std::make_unique<ReconstructionSlab>(p, 1, false);
+    case Method::method_1b: {
+        int t = p.xpr.threads > 0 ? p.xpr.threads : 12;                // ASYNC_THREAD
+        return std::make_unique<ReconstructionSlab>(p, std::min(t, 16), false);
+    }
+    case Method::method_1c: return std::make_unique<ReconstructionSlab>(p, io_threads(p), true);
+    case Method::method_2: return std::make_unique<ReconstructionChunk>(p);
+    case Method::hartmann:
+    case Method::ipr:
+        throw std::runtime_error(std::string("method '") + method_name(p.recon.method) +
+                                 "' (a separate phase-imaging modality) is not part of this build");
+    }
+    throw std::runtime_error("unknown method");
+}
+
+} // namespace xpr

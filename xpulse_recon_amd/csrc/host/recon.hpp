@@ -1,0 +1,204 @@
+// Host engine of the reconstruction: the reference's Dataset / DataLoader
+// (dataset.hpp, dataset/data_loader.hpp), its operator API (Reconstruction,
+// src/reconstruction.hpp:30-316) and the five memory schedules
+// (reconstruction_{n,lpln,lpla,lplh,chunk}.hpp), driving the HIP kernels
+// through the C-ABI of include/xpr_hip.h.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <memory>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../../include/xpr_hip.h"
+#include "json.hpp"
+#include "params.hpp"
+
+namespace xpr {
+
+struct XprFailure : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+void check(int rc, const char *what);
+
+// Device allocation owned by the host engine.
+class DevBuf {
+public:
+    DevBuf() = default;
+    explicit DevBuf(size_t bytes) { alloc(bytes); }
+    ~DevBuf();
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept;
+    void alloc(size_t bytes);
+    void reset();
+    void *get() const { return p_; }
+    template <typename T> T *as() const { return (T *)p_; }
+    size_t bytes() const { return n_; }
+
+private:
+    void *p_ = nullptr;
+    size_t n_ = 0;
+};
+
+// Pinned host allocation.
+class HostBuf {
+public:
+    HostBuf() = default;
+    explicit HostBuf(size_t bytes) { alloc(bytes); }
+    ~HostBuf();
+    HostBuf(const HostBuf &) = delete;
+    HostBuf &operator=(const HostBuf &) = delete;
+    void alloc(size_t bytes);
+    void *get() const { return p_; }
+    template <typename T> T *as() const { return (T *)p_; }
+    size_t bytes() const { return n_; }
+
+private:
+    void *p_ = nullptr;
+    size_t n_ = 0;
+};
+
+inline size_t dtype_bytes(int dt) { return dt == XPR_F32 ? 4 : 2; }
+
+// ---------------------------------------------------------------------------
+// Dataset: projections on disk, geometry, preprocessed images and the layer
+// store (dataset.hpp:18-270, data_loader.hpp:13-309)
+// ---------------------------------------------------------------------------
+class Dataset {
+public:
+    explicit Dataset(Parameters &p);
+    ~Dataset();
+
+    int64_t width() const { return dwidth_; }
+    int64_t height() const { return dheight_; }
+    int64_t projections() const { return nproj_; }
+    int64_t concurrent() const { return nproj_ / p_.recon.sit; }     // dataset.hpp:225
+    int64_t vwidth() const { return vwidth_; }
+    int64_t vheight() const { return vheight_; }
+    float vx() const { return vx_; }
+    std::array<float, 3> orig() const { return orig_; }
+
+    // ProjData of all (angle, module) records and of one sub-iteration.
+    const std::vector<float> &projdata() const { return pd_; }
+    std::vector<float> sit_projdata(int64_t sit) const;
+    std::vector<int64_t> sit_views(int64_t sit) const;
+
+    // Preprocess every projection (normalise, -log) on the device and keep
+    // the float32 result in host memory (initializeTempImages,
+    // data_loader.hpp:48-101).  Rows [row0, row0+rows) only when cropped.
+    void initialize(int device_stream_dtype_unused = 0);
+    // Preprocessed images of one sub-iteration, rows [row0, row0+rows) of each.
+    std::vector<float> sit_images(int64_t sit, int64_t row0, int64_t rows) const;
+    float norm_min() const { return mn_; }
+    float norm_max() const { return mx_; }
+
+    // Layer store between passes (getLayer/saveLayer, data_loader.hpp:118-146).
+    void save_layer(const float *data, int64_t layer, bool finalize);
+    void load_layer(int64_t layer, float *out) const;
+    bool ram_layers() const { return p_.xpr.layer_store == "ram"; }
+    void save_final_layers(const float *vol, int64_t l0, int64_t l1);   // TIFF, de-normalised
+    void save_projection(const float *img, int64_t w, int64_t h, int64_t id) const;
+
+private:
+    Parameters &p_;
+    std::vector<std::string> files_;
+    int64_t dwidth_ = 0, dheight_ = 0, nproj_ = 0;
+    int64_t vwidth_ = 0, vheight_ = 0;
+    float vx_ = 0.0f;
+    std::array<float, 3> orig_{};
+    std::vector<float> pd_;
+    std::vector<std::vector<float>> images_;    // preprocessed, one per projection
+    std::vector<std::vector<float>> ram_store_; // layer store when "ram"
+    float mn_ = 0.0f, mx_ = 1.0f;
+    std::string proj_dir_;
+};
+
+// ---------------------------------------------------------------------------
+// Operator API: the reference's positional kernel state, kept on the host
+// and handed by value to every stateless C-ABI call.
+// ---------------------------------------------------------------------------
+class Operators {
+public:
+    Operators(xpr_stream s, int acc_bits) : stream_(s), acc_(acc_bits) {}
+    void setImagesBuffer(const void *img, int dtype) { img_ = img; img_dt_ = dtype; }
+    void setSumImagesBuffer(void *sums) { sums_ = sums; }
+    void setErrorBuffer(void *err, int dtype) { err_ = err; err_dt_ = dtype; }
+    void setVolumeBuffer(void *vol, int dtype) { vol_ = vol; vol_dt_ = dtype; }
+    void setProjDataBuffer(const float *pd) { pd_ = pd; }
+    void setImageParameters(int64_t w, int64_t h, int64_t n);
+    void setVolumeParameters(std::array<int, 3> mn, std::array<int, 3> mx, float voxel);
+    void setAngleNumber(int64_t n, int64_t mods) { a_.n_views = uint32_t(n); a_.modules = uint32_t(mods); }
+    void setWeight(float w) { w_ = w; }
+    void setOrigin(std::array<float, 3> o) { for (int i = 0; i < 3; ++i) a_.origin[i] = o[i]; }
+    void setImageOffset(int offset, int rows) { a_.line_offset = offset; a_.line_rows = rows; }
+    void setImageElements(int64_t n) { elements_ = uint64_t(n); }
+    void setViewIndex(const uint32_t *idx) { a_.view_index = idx; }
+    void forward();
+    void error();
+    void backward();
+    xpr_stream stream() const { return stream_; }
+    int acc_bits() const { return acc_; }
+    const xpr_proj_args &args() const { return a_; }
+
+private:
+    xpr_stream stream_;
+    int acc_;
+    const void *img_ = nullptr;
+    void *sums_ = nullptr, *err_ = nullptr, *vol_ = nullptr;
+    int img_dt_ = XPR_F32, err_dt_ = XPR_F32, vol_dt_ = XPR_F32;
+    const float *pd_ = nullptr;
+    xpr_proj_args a_{};
+    float w_ = 1.0f;
+    uint64_t elements_ = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Schedules
+// ---------------------------------------------------------------------------
+struct Chunk {                      // specialized_parameters.hpp:28-32
+    int64_t vOffset, vSize, iOffset, iSize, vPadTop, vPadBot;
+};
+
+class Reconstruction {
+public:
+    explicit Reconstruction(Parameters &p);
+    virtual ~Reconstruction();
+    virtual uint64_t requiredGPUMemory() = 0;      // requieredGPUMemory
+    virtual void exec() = 0;
+    void forward_exec();                            // src/reconstruction.hpp:66-158
+    Json run_record() const { return record_; }
+    Dataset &dataset() { return *ds_; }
+
+protected:
+    std::array<float, 3> next_origin();             // setOrigin(orig, vx) draws
+    uint64_t budget() const { return budget_; }
+    void check_memory(uint64_t need, const char *what);
+    int storage() const { return p_.xpr.storage; }
+    float next_weight(int64_t mit) const;
+    void note(const std::string &k, Json v) { record_.set(k, std::move(v)); }
+    void log_sit(int64_t mit, int64_t sit, double ms);
+
+    Parameters &p_;
+    std::unique_ptr<Dataset> ds_;
+    xpr_stream stream_ = nullptr, copy_ = nullptr;
+    uint64_t budget_ = 0;
+    std::mt19937 rng_;
+    Json record_ = Json::object();
+    Json jitter_log_ = Json::array();
+    Json sit_ms_ = Json::array();
+};
+
+std::unique_ptr<Reconstruction> make_reconstruction(Parameters &p);
+
+// Chunk planner (GeometryChunk, dataset/geometry_chunk.hpp:92-197).
+std::vector<Chunk> plan_chunks(const Parameters &p, const Dataset &ds, uint64_t budget_bytes,
+                               std::vector<float> *layer_min = nullptr,
+                               std::vector<float> *layer_max = nullptr);
+uint64_t chunk_bytes(const Chunk &c, int64_t vwidth, int64_t dwidth, int64_t concurrent,
+                     int storage);
+
+} // namespace xpr

@@ -20,6 +20,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -114,7 +115,8 @@ __device__ __forceinline__ ViewConsts view_consts(const float *__restrict__ pd, 
     c.vpr = fminf(pd[17], (float)a.det_width);
     c.vlo = fmaxf(pd[18], 0.0f);
     c.vhi = fminf(pd[19], (float)a.line_rows);
-    c.base = (uint64_t)view * ((uint64_t)a.det_width * (uint64_t)a.line_rows);
+    const uint32_t dview = a.view_index ? a.view_index[view] : view;
+    c.base = (uint64_t)dview * ((uint64_t)a.det_width * (uint64_t)a.line_rows);
     return c;
 }
 
@@ -233,39 +235,169 @@ __device__ __forceinline__ uint32_t q22<uint32_t>(float v) { return q22_u32(v); 
 template <>
 __device__ __forceinline__ uint64_t q22<uint64_t>(float v) { return q22_u64(v); }
 
-template <typename VS, typename AccT, int J>
-__global__ __launch_bounds__(kThreads) void k_forward_global(
-    const typename VS::T *__restrict__ vol, AccT *__restrict__ sums,
-    const float *__restrict__ pd, xpr_proj_args a, SlabShape s)
+// ---------------------------------------------------------------------------
+// Forward with LDS binning.  A workgroup owns a brick of 64 (x) x 4*LW
+// (layers) x BK (depths) voxels; wave w holds layers w, w+4, ... and lane l the
+// column x0+l.  Per view, the brick's 8 corners bound its detector footprint
+// (one pixel of margin covers float rounding); voxel-views accumulate there
+// with LDS u64 atomics and the non-zero footprint pixels are added to the
+// detector sums with one global atomic each.  Footprints larger than the tile,
+// corners behind the source and bricks holding Q.22 values above 2^32 take the
+// direct path (one global atomic per voxel-view).  All paths add the same
+// integers, so the sums are bit-identical whatever path or launch order.
+// ---------------------------------------------------------------------------
+constexpr int kTileCap = 2048;
+
+struct BrickShape {
+    int32_t nx, nl, nk;      // slab extent (x, layers, depths)
+    int32_t nbx, nbk;        // bricks along x and depth
+    int32_t cap;             // tile entries in use (<= kTileCap); 0 forces the direct path
+    int64_t brick_base;      // first brick of this launch
+    int64_t nbricks;         // bricks in the slab
+};
+
+template <typename AccT>
+__device__ __forceinline__ void global_add(AccT *p, AccT v)
 {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename VS, typename AccT, int LW, int BK>
+__global__ __launch_bounds__(kThreads) void k_forward_lds(
+    const typename VS::T *__restrict__ vol, AccT *__restrict__ sums,
+    const float *__restrict__ pd, xpr_proj_args a, BrickShape s)
+{
+    constexpr int BL = kWaves * LW;
+    __shared__ unsigned long long tile[kTileCap];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int seg = blockIdx.x % s.nseg;
-    const int64_t row = s.row_base + (int64_t)(blockIdx.x / s.nseg) * kWaves + wave;
-    if (row >= s.nrows) return;
-    const int32_t l = (int32_t)(row / s.nk), k = (int32_t)(row % s.nk);
-    const float py = __builtin_fmaf((float)(l + a.vmin[1]), a.voxel[1], a.origin[1]);
-    const float pz = __builtin_fmaf((float)(k + a.vmin[2]), a.voxel[2], a.origin[2]);
-    const int32_t xb = seg * 64 * J + lane;
-    const uint64_t rowoff = (uint64_t)row * (uint64_t)s.nx;
-    float px[J];
-    AccT q[J];
+    int64_t b = s.brick_base + blockIdx.x;
+    if (b >= s.nbricks) return;                       // uniform over the block
+    const int32_t bx = (int32_t)(b % s.nbx);
+    b /= s.nbx;
+    const int32_t bk = (int32_t)(b % s.nbk);
+    const int32_t bl = (int32_t)(b / s.nbk);
+    const int32_t x0 = bx * 64, l0 = bl * BL, k0 = bk * BK;
+    const int32_t x = x0 + lane;
+
+    // voxel values as Q.22 (32 bits suffice unless some value is >= 1024)
+    uint32_t q[LW][BK];
+    bool big = false;
 #pragma unroll
-    for (int j = 0; j < J; ++j) {
-        const int32_t x = xb + 64 * j;
-        px[j] = __builtin_fmaf((float)(x + a.vmin[0]), a.voxel[0], a.origin[0]);
-        q[j] = (x < s.nx) ? q22<AccT>(VS::load(vol, rowoff + x)) : (AccT)0;
+    for (int j = 0; j < LW; ++j) {
+        const int32_t l = l0 + wave + kWaves * j;
+#pragma unroll
+        for (int i = 0; i < BK; ++i) {
+            const int32_t k = k0 + i;
+            uint32_t qq = 0;
+            if (x < s.nx && l < s.nl && k < s.nk) {
+                const float v = VS::load(vol, ((uint64_t)l * s.nk + k) * s.nx + x);
+                if (sizeof(AccT) == 8) {
+                    const uint64_t q64 = q22_u64(v);
+                    big |= (q64 >> 32) != 0;
+                    qq = (uint32_t)q64;
+                } else {
+                    qq = q22_u32(v);
+                }
+            }
+            q[j][i] = qq;
+        }
     }
+    const bool any_big = __syncthreads_or(big);
+    const float px = __builtin_fmaf((float)(x + a.vmin[0]), a.voxel[0], a.origin[0]);
+
+    // this lane's brick corner (lanes form identical groups of 8 corners)
+    const int32_t xe = min(x0 + 63, s.nx - 1), le = min(l0 + BL - 1, s.nl - 1),
+                  ke = min(k0 + BK - 1, s.nk - 1);
+    const int32_t cxi = (lane & 1) ? xe : x0, cli = (lane & 2) ? le : l0,
+                  cki = (lane & 4) ? ke : k0;
+    const float cpx = __builtin_fmaf((float)(cxi + a.vmin[0]), a.voxel[0], a.origin[0]);
+    const float cpy = __builtin_fmaf((float)(cli + a.vmin[1]), a.voxel[1], a.origin[1]);
+    const float cpz = __builtin_fmaf((float)(cki + a.vmin[2]), a.voxel[2], a.origin[2]);
     const float line_off = (float)a.line_offset;
+    const uint64_t view_stride = (uint64_t)a.det_width * (uint64_t)a.line_rows;
+
     for (uint32_t view = 0, m = 0; view < a.n_views; ++view)
     for (uint32_t mm = 0; mm < a.modules; ++mm, ++m) {
-        const ViewConsts c = view_consts(pd + 20u * m, py, pz, a, view);
+        const float *__restrict__ p = pd + 20u * m;
+        const uint64_t base = (uint64_t)(a.view_index ? a.view_index[view] : view) * view_stride;
+        // ---- footprint of the brick ----------------------------------------
+        const float X = __builtin_fmaf(p[0], cpx, __builtin_fmaf(p[1], cpy, __builtin_fmaf(p[2], cpz, p[3])));
+        const float Y = __builtin_fmaf(p[4], cpx, __builtin_fmaf(p[5], cpy, __builtin_fmaf(p[6], cpz, p[7])));
+        const float Wc = __builtin_fmaf(p[12], cpx, __builtin_fmaf(p[13], cpy, __builtin_fmaf(p[14], cpz, p[15])));
+        const float r = 1.0f / Wc;
+        float umin = X * r, vmin = Y * r - line_off;
+        float umax = umin, vmax = vmin;
+        int ok = (Wc > 0.0f) & (umin == umin) & (vmin == vmin) & (fabsf(umin) < 1e7f) &
+                 (fabsf(vmin) < 1e7f);
 #pragma unroll
-        for (int j = 0; j < J; ++j) {
-            uint64_t idx;
-            if (project(c, px[j], line_off, a.det_width, idx) && q[j] != 0)
-                __hip_atomic_fetch_add(sums + idx, q[j], __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
+        for (int off = 1; off < 8; off <<= 1) {
+            umin = fminf(umin, __shfl_xor(umin, off));
+            umax = fmaxf(umax, __shfl_xor(umax, off));
+            vmin = fminf(vmin, __shfl_xor(vmin, off));
+            vmax = fmaxf(vmax, __shfl_xor(vmax, off));
+            ok &= __shfl_xor(ok, off);
+        }
+        const float vpl = fmaxf(p[16], 0.0f), vpr = fminf(p[17], (float)a.det_width);
+        const float vlo = fmaxf(p[18], 0.0f), vhi = fminf(p[19], (float)a.line_rows);
+        int32_t u0 = 0, v0 = 0, tw = 0, th = 0;
+        if (ok) {
+            u0 = (int32_t)fmaxf(floorf(umin) - 1.0f, vpl);
+            v0 = (int32_t)fmaxf(floorf(vmin) - 1.0f, vlo);
+            const int32_t u1 = (int32_t)fminf(ceilf(umax) + 1.0f, vpr - 1.0f);
+            const int32_t v1 = (int32_t)fminf(ceilf(vmax) + 1.0f, vhi - 1.0f);
+            tw = u1 - u0 + 1;
+            th = v1 - v0 + 1;
+            if (tw <= 0 || th <= 0) continue;         // the brick misses this view
+        }
+        const bool use_tile = ok && !any_big && tw * th <= s.cap;
+        if (use_tile) {
+            for (int rr = wave; rr < th; rr += kWaves)
+                for (int cc = lane; cc < tw; cc += 64) tile[rr * tw + cc] = 0ull;
+            __syncthreads();
+        }
+        // ---- splat ------------------------------------------------------------
+#pragma unroll
+        for (int j = 0; j < LW; ++j) {
+            const int32_t l = l0 + wave + kWaves * j;
+            const float py = __builtin_fmaf((float)(l + a.vmin[1]), a.voxel[1], a.origin[1]);
+#pragma unroll
+            for (int i = 0; i < BK; ++i) {
+                const float pz = __builtin_fmaf((float)(k0 + i + a.vmin[2]), a.voxel[2], a.origin[2]);
+                const float Xv = __builtin_fmaf(p[0], px, __builtin_fmaf(p[1], py, __builtin_fmaf(p[2], pz, p[3])));
+                const float Yv = __builtin_fmaf(p[4], px, __builtin_fmaf(p[5], py, __builtin_fmaf(p[6], pz, p[7])));
+                const float Wv = __builtin_fmaf(p[12], px, __builtin_fmaf(p[13], py, __builtin_fmaf(p[14], pz, p[15])));
+                const float rv = 1.0f / Wv;
+                const float cu = __builtin_roundf(Xv * rv);
+                const float cv = __builtin_roundf(Yv * rv - line_off);
+                const bool inb = (cu >= vpl) & (cu < vpr) & (cv >= vlo) & (cv < vhi);
+                const uint32_t qq = q[j][i];
+                if (inb && qq != 0u) {
+                    const int32_t iu = (int32_t)cu, iv = (int32_t)cv;
+                    const uint32_t tu = (uint32_t)(iu - u0), tv = (uint32_t)(iv - v0);
+                    if (use_tile && tu < (uint32_t)tw && tv < (uint32_t)th) {
+                        atomicAdd(&tile[tv * tw + tu], (unsigned long long)qq);
+                    } else {
+                        AccT qa = (AccT)qq;
+                        if (sizeof(AccT) == 8 && any_big) {
+                            // exact 64-bit value: reload and convert again
+                            qa = (AccT)q22_u64(VS::load(vol, ((uint64_t)l * s.nk + k0 + i) * s.nx + x));
+                        }
+                        global_add(sums + base + (uint64_t)((uint32_t)iv * a.det_width + (uint32_t)iu), qa);
+                    }
+                }
+            }
+        }
+        if (use_tile) {
+            __syncthreads();
+            for (int rr = wave; rr < th; rr += kWaves) {
+                AccT *row = sums + base + (uint64_t)(uint32_t)(v0 + rr) * a.det_width + (uint32_t)u0;
+                for (int cc = lane; cc < tw; cc += 64) {
+                    const unsigned long long t = tile[rr * tw + cc];
+                    if (t != 0ull) global_add(row + cc, (AccT)t);
+                }
+            }
+            __syncthreads();
         }
     }
 }
@@ -285,6 +417,15 @@ __global__ __launch_bounds__(256) void k_error(const typename IS::T *__restrict_
         const float e = logf(fmaxf(ref / fmaxf(got, kQ22Eps), kQ22Eps)) * weight;
         ES::store(err, i, e);
     }
+}
+
+template <typename AccT>
+__global__ __launch_bounds__(256) void k_q22_to_float(const AccT *__restrict__ sums,
+                                                      float *__restrict__ out, uint64_t n)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        out[i] = (float)sums[i] / kQ22One;
 }
 
 template <typename T>
@@ -428,14 +569,42 @@ int backward_impl(hipStream_t st, void *vol, const void *err, const float *pd,
     });
 }
 
+constexpr int kFwdLW = 2;   // layers per wave
+constexpr int kFwdBK = 16;  // depths per brick
+
+int forward_tile_cap()
+{
+    // tuning knob: tile entries in use; 0 sends every voxel-view to the direct path
+    const char *e = std::getenv("XPR_FORWARD_TILE_CAP");
+    const int c = e ? std::atoi(e) : kTileCap;
+    return c < 0 ? 0 : (c > kTileCap ? kTileCap : c);
+}
+
 template <typename VS, typename AccT>
 int forward_impl(hipStream_t st, const void *vol, void *sums, const float *pd,
                  const xpr_proj_args &a, SlabShape s)
 {
-    return launch_rows(s, [&](unsigned blocks, const SlabShape &t) {
-        hipLaunchKernelGGL((k_forward_global<VS, AccT, kJ>), dim3(blocks), dim3(kThreads), 0,
-                           st, (const typename VS::T *)vol, (AccT *)sums, pd, a, t);
-    });
+    constexpr int BL = kWaves * kFwdLW;
+    BrickShape b;
+    b.nx = s.nx;
+    b.nl = s.nl;
+    b.nk = s.nk;
+    b.nbx = (s.nx + 63) / 64;
+    b.nbk = (s.nk + kFwdBK - 1) / kFwdBK;
+    const int64_t nbl = (s.nl + BL - 1) / BL;
+    b.nbricks = (int64_t)b.nbx * b.nbk * nbl;
+    b.cap = forward_tile_cap();
+    const int64_t max_blocks = (int64_t(1) << 31) / kThreads;
+    for (int64_t b0 = 0; b0 < b.nbricks; b0 += max_blocks) {
+        BrickShape t = b;
+        t.brick_base = b0;
+        const int64_t blocks = std::min<int64_t>(max_blocks, b.nbricks - b0);
+        hipLaunchKernelGGL((k_forward_lds<VS, AccT, kFwdLW, kFwdBK>), dim3((unsigned)blocks),
+                           dim3(kThreads), 0, st, (const typename VS::T *)vol, (AccT *)sums, pd,
+                           a, t);
+        XPR_LAUNCH_CHECK();
+    }
+    return XPR_OK;
 }
 
 template <typename IS, typename AccT>
@@ -588,6 +757,46 @@ int xpr_stream_sync(xpr_stream s)
     return XPR_OK;
 }
 
+int xpr_event_create(xpr_event *e)
+{
+    if (!e) return fail(XPR_E_INVALID, "null output");
+    hipEvent_t h;
+    XPR_HIP_CHECK(hipEventCreate(&h));
+    *e = (xpr_event)h;
+    return XPR_OK;
+}
+
+int xpr_event_destroy(xpr_event e)
+{
+    XPR_HIP_CHECK(hipEventDestroy((hipEvent_t)e));
+    return XPR_OK;
+}
+
+int xpr_event_record(xpr_event e, xpr_stream s)
+{
+    XPR_HIP_CHECK(hipEventRecord((hipEvent_t)e, (hipStream_t)s));
+    return XPR_OK;
+}
+
+int xpr_stream_wait_event(xpr_stream s, xpr_event e)
+{
+    XPR_HIP_CHECK(hipStreamWaitEvent((hipStream_t)s, (hipEvent_t)e, 0));
+    return XPR_OK;
+}
+
+int xpr_event_sync(xpr_event e)
+{
+    XPR_HIP_CHECK(hipEventSynchronize((hipEvent_t)e));
+    return XPR_OK;
+}
+
+int xpr_event_elapsed_ms(xpr_event start, xpr_event stop, float *ms)
+{
+    if (!ms) return fail(XPR_E_INVALID, "null output");
+    XPR_HIP_CHECK(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return XPR_OK;
+}
+
 int xpr_fill_u32(xpr_stream s, uint32_t *dst, uint32_t value, uint64_t n)
 {
     if (n == 0) return XPR_OK;
@@ -717,6 +926,22 @@ int xpr_backward(xpr_stream s, void *volume, int vol_dtype, const void *err, int
     default: return XPR_BW(BF16, BF16);
     }
 #undef XPR_BW
+}
+
+int xpr_q22_to_float(xpr_stream s, const void *sums, int acc_bits, float *out, uint64_t n)
+{
+    if (acc_bits != XPR_ACC_U32 && acc_bits != XPR_ACC_U64)
+        return fail(XPR_E_INVALID, "acc_bits must be 32 or 64");
+    if (n == 0) return XPR_OK;
+    if (!sums || !out) return fail(XPR_E_INVALID, "null buffer");
+    if (acc_bits == XPR_ACC_U32)
+        hipLaunchKernelGGL(k_q22_to_float<uint32_t>, dim3(grid_for(n)), dim3(256), 0,
+                           (hipStream_t)s, (const uint32_t *)sums, out, n);
+    else
+        hipLaunchKernelGGL(k_q22_to_float<uint64_t>, dim3(grid_for(n)), dim3(256), 0,
+                           (hipStream_t)s, (const uint64_t *)sums, out, n);
+    XPR_LAUNCH_CHECK();
+    return XPR_OK;
 }
 
 int xpr_minmax_init(xpr_stream s, int32_t *minmax)
