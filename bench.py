@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run one plain 1 GiB float copy (k_convert) so hardware "
+                         "byte counters can be scaled against a known size")
     return ap.parse_args()
 
 
@@ -199,6 +202,15 @@ def main():
                       "backward_launch_ms": round(bp_launch, 4),
                       "backward_frac": round(bp_bytes / (bp_launch * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                       "error_ms": round(er_ms, 4)}}
+
+    if args.calibrate:
+        n = 1 << 28                      # 1 GiB read + 1 GiB written
+        a = torch.empty(n, dtype=torch.float32, device=dev)
+        b = torch.empty(n, dtype=torch.float32, device=dev)
+        X.fill(a, 1.0)
+        X.convert(b, a)
+        torch.cuda.synchronize()
+        del a, b
 
     cpu = None
     if not args.no_cpu_baseline:

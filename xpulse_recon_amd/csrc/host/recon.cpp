@@ -730,9 +730,11 @@ private:
             ops.setVolumeParameters({0, int(l0), 0}, {int(W), int(l1), int(W)}, ds.vx());
             if (cull_) {
                 const SlabViews &sv = slab_pd[size_t(b)];
+                // (a slab no view reaches runs with zero views: the forward adds
+                // nothing and the backward leaves the slab unchanged)
+                cpd_[i].alloc(std::max<size_t>(sv.pd.size(), 20) * 4);
+                cidx_[i].alloc(std::max<size_t>(sv.index.size(), 1) * 4);
                 if (!sv.index.empty()) {
-                    cpd_[i].alloc(sv.pd.size() * 4);
-                    cidx_[i].alloc(sv.index.size() * 4);
                     check(xpr_upload(stream_, cpd_[i].get(), sv.pd.data(), sv.pd.size() * 4), "upload");
                     check(xpr_upload(stream_, cidx_[i].get(), sv.index.data(), sv.index.size() * 4), "upload");
                 }
