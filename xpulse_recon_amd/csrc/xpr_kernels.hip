@@ -91,57 +91,156 @@ struct BF16 {
 };
 
 // ---------------------------------------------------------------------------
-// projection of one voxel by one ProjData record
+// Exact fast forms of the canonical projection.  Each helper returns exactly
+// what the canonical expression returns; the two hardware identities they
+// rely on were checked over every float32 input on gfx950:
+//   * v_rcp_f32 followed by one FMA Newton step is the IEEE quotient 1/w for
+//     every normal |w| in [1e-6, 1e6] (w outside that range takes 1.0f/w);
+//   * v_cvt_rpi_i32_f32 (floor(u + 0.5)) equals (int)roundf(u) for every u in
+//     (-0.5, 2^23), which holds for every in-bounds coordinate.
+// In-bounds tests compare the unrounded coordinate with half-integer
+// thresholds: for an integer bound L >= 1, roundf(u) >= L <=> u >= L - 0.5,
+// roundf(u) >= 0 <=> u > -0.5 (ties round away from zero), and for R >= 1
+// roundf(u) < R <=> u < R - 0.5.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ float recip(float w)
+{
+    const float aw = fabsf(w);
+    if (__builtin_expect(aw >= 1e-6f && aw <= 1e6f, 1)) {
+        const float r0 = __builtin_amdgcn_rcpf(w);
+        const float e = __builtin_fmaf(-w, r0, 1.0f);
+        return __builtin_fmaf(e, r0, r0);
+    }
+    return 1.0f / w;
+}
+
+__device__ __forceinline__ int32_t pixel_of(float u)
+{
+    int32_t r;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(u));
+    return r;
+}
+
+// thresholds on the unrounded coordinate for roundf(u) in [lo, hi) with real
+// bounds lo >= 0 (an integer n satisfies n >= lo <=> n >= ceil(lo), n < hi <=>
+// n < ceil(hi))
+__device__ __forceinline__ float lo_threshold(float lo)
+{
+    const float L = ceilf(lo);
+    return L >= 1.0f ? L - 0.5f : -0.49999997f;       // the float just above -0.5
+}
+__device__ __forceinline__ float hi_threshold(float hi) { return ceilf(hi) - 0.5f; }
+
+// per (row, view) constants
 struct ViewConsts {
     float c0x, c1x, c3x;       // x coefficients of glm columns 0, 1, 3
-    float k0, k1, k3;          // dot of the (y, z, w) part with the row's (py, pz, 1)
-    float vpl, vpr, vlo, vhi;  // half-open column and row bounds
-    uint64_t base;             // view offset in the detector buffers
+    float k0, k1, k3;          // the (y, z, w) part of the three dot products for (py, pz)
+    float ulo, uhi, vlo, vhi;  // thresholds of the half-open column and row bounds
 };
 
-__device__ __forceinline__ ViewConsts view_consts(const float *__restrict__ pd, float py,
-                                                  float pz, const xpr_proj_args &a,
-                                                  uint32_t view)
+// view bounds, clamped to the buffers the call holds
+__device__ __forceinline__ void view_thresholds(const float *__restrict__ pd,
+                                                const xpr_proj_args &a, ViewConsts &c)
 {
-    ViewConsts c;
+    c.ulo = lo_threshold(fmaxf(pd[16], 0.0f));
+    c.uhi = hi_threshold(fminf(pd[17], (float)a.det_width));
+    c.vlo = lo_threshold(fmaxf(pd[18], 0.0f));
+    c.vhi = hi_threshold(fminf(pd[19], (float)a.line_rows));
+}
+
+__device__ __forceinline__ void row_terms(const float *__restrict__ pd, float py, float pz,
+                                          ViewConsts &c)
+{
     c.c0x = pd[0];
     c.c1x = pd[4];
     c.c3x = pd[12];
     c.k0 = __builtin_fmaf(pd[1], py, __builtin_fmaf(pd[2], pz, pd[3]));
     c.k1 = __builtin_fmaf(pd[5], py, __builtin_fmaf(pd[6], pz, pd[7]));
     c.k3 = __builtin_fmaf(pd[13], py, __builtin_fmaf(pd[14], pz, pd[15]));
-    c.vpl = fmaxf(pd[16], 0.0f);                      // never outside the buffers
-    c.vpr = fminf(pd[17], (float)a.det_width);
-    c.vlo = fmaxf(pd[18], 0.0f);
-    c.vhi = fminf(pd[19], (float)a.line_rows);
-    const uint32_t dview = a.view_index ? a.view_index[view] : view;
-    c.base = (uint64_t)dview * ((uint64_t)a.det_width * (uint64_t)a.line_rows);
-    return c;
 }
 
-// Returns true and the detector index when voxel x-position px lands in bounds.
+// The exact W of a voxel is affine in its position, so over a box of voxels
+// it lies between its values at the box corners; the three FMAs that
+// evaluate it err by at most T * 2^-22, T = |c3x px| + |c3y py| + |c3z pz| + |c3w|.
+// Corner values in [1e-2, 1e5] with T <= 1e4 therefore keep every W of the
+// box inside [1e-6, 1e6], where the fast reciprocal is exact.
+__device__ __forceinline__ bool fast_rcp_box(float w, float t)
+{
+    return (w >= 1e-2f) & (w <= 1e5f) & (t <= 1e4f);
+}
+
+// row segment [px_first, px_last] at fixed (py, pz): the two ends are its corners
+__device__ __forceinline__ bool fast_rcp_range(const float *__restrict__ p, const ViewConsts &c,
+                                               float px_first, float px_last, float py, float pz)
+{
+    const float w0 = __builtin_fmaf(c.c3x, px_first, c.k3);
+    const float w1 = __builtin_fmaf(c.c3x, px_last, c.k3);
+    const float t = fabsf(c.c3x) * fmaxf(fabsf(px_first), fabsf(px_last)) + fabsf(p[13] * py) +
+                    fabsf(p[14] * pz) + fabsf(p[15]);
+    return fast_rcp_box(w0, t) & fast_rcp_box(w1, t);
+}
+
+// Projects the voxel at x-position px: true when in bounds, with its pixel
+// column iu and row iv (meaningful when in bounds).  FAST: the caller has
+// checked with fast_rcp_range that W is in the fast reciprocal's range.
+template <bool CROP, bool FAST>
 __device__ __forceinline__ bool project(const ViewConsts &c, float px, float line_off,
-                                        uint32_t det_width, uint64_t &idx)
+                                        uint32_t &iu, uint32_t &iv)
 {
     const float X = __builtin_fmaf(c.c0x, px, c.k0);
     const float Y = __builtin_fmaf(c.c1x, px, c.k1);
     const float W = __builtin_fmaf(c.c3x, px, c.k3);
-    const float r = 1.0f / W;
+    float r;
+    if (FAST) {
+        const float r0 = __builtin_amdgcn_rcpf(W);
+        r = __builtin_fmaf(__builtin_fmaf(-W, r0, 1.0f), r0, r0);
+    } else {
+        r = recip(W);
+    }
     const float u = X * r;
-    const float v = Y * r - line_off;
-    const float cu = __builtin_roundf(u);
-    const float cv = __builtin_roundf(v);
-    const bool inb = (cu >= c.vpl) & (cu < c.vpr) & (cv >= c.vlo) & (cv < c.vhi);
-    idx = c.base + (uint64_t)((uint32_t)(int32_t)cv * det_width + (uint32_t)(int32_t)cu);
+    const float v = CROP ? Y * r - line_off : Y * r;   // subtracting 0 is exact
+    const bool inb = (u >= c.ulo) & (u < c.uhi) & (v >= c.vlo) & (v < c.vhi);
+    iu = (uint32_t)pixel_of(u);
+    iv = (uint32_t)pixel_of(v);
     return inb;
 }
 
+__device__ __forceinline__ uint32_t view_of(const xpr_proj_args &a, uint32_t view)
+{
+    return a.view_index ? a.view_index[view] : view;
+}
+
+// raw buffer over one detector view: out-of-range offsets read as 0
+constexpr int kBufferWord3 = 0x00020000;     // gfx9 raw buffer, no format conversion
+constexpr uint32_t kOutOfRange = 0x80000000u;
+
+template <typename T>
+__device__ __forceinline__ float load_or_zero(__amdgpu_buffer_rsrc_t r, uint32_t byte_off);
+template <>
+__device__ __forceinline__ float load_or_zero<F32>(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+template <>
+__device__ __forceinline__ float load_or_zero<F16>(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    const uint16_t h = __builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0);
+    return (float)__builtin_bit_cast(_Float16, h);
+}
+template <>
+__device__ __forceinline__ float load_or_zero<BF16>(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    const uint16_t h = __builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0);
+    return __builtin_bit_cast(float, (uint32_t)h << 16);
+}
+
 // ---------------------------------------------------------------------------
-// projector kernels: one wave per (layer, depth) row, lane owns J voxels at
-// x = seg*64*J + lane + 64*j.  The (y, z) part of the three dot products is
-// hoisted per row and view; per voxel-view the work is 3 FMA, 1 reciprocal,
-// 2 MUL, 2 round, 4 compares and one gather (backward) or atomic (forward).
+// Backward: one wave per (layer, depth) row, lane owns J voxels at
+// x = seg*64*J + lane + 64*j.  Per view the (y, z, w) part of the three dot
+// products is computed once per row; a voxel-view then costs 3 FMA, the
+// reciprocal, 2 MUL, 4 compares, 2 conversions, the index and one gather.
+// The gather reads through a raw buffer over the view, so out-of-bounds
+// voxels read 0 without a branch and all J loads stay in flight.
 // ---------------------------------------------------------------------------
 constexpr int kWaves = 4;
 constexpr int kThreads = 64 * kWaves;
@@ -153,7 +252,7 @@ struct SlabShape {
     int64_t nrows;         // rows in the slab
 };
 
-template <typename VS, typename ES, int J>
+template <typename VS, typename ES, int J, bool CROP>
 __global__ __launch_bounds__(kThreads) void k_backward(
     typename VS::T *__restrict__ vol, const typename ES::T *__restrict__ err,
     const float *__restrict__ pd, xpr_proj_args a, SlabShape s,
@@ -163,7 +262,7 @@ __global__ __launch_bounds__(kThreads) void k_backward(
     const int wave = threadIdx.x >> 6;
     const int seg = blockIdx.x % s.nseg;
     const int64_t row = s.row_base + (int64_t)(blockIdx.x / s.nseg) * kWaves + wave;
-    if (row >= s.nrows) return;
+    if (row >= s.nrows) return;                       // whole wave leaves together
     const int32_t l = (int32_t)(row / s.nk), k = (int32_t)(row % s.nk);
     const float py = __builtin_fmaf((float)(l + a.vmin[1]), a.voxel[1], a.origin[1]);
     const float pz = __builtin_fmaf((float)(k + a.vmin[2]), a.voxel[2], a.origin[2]);
@@ -176,18 +275,41 @@ __global__ __launch_bounds__(kThreads) void k_backward(
         sum[j] = 0.0f;
         cnt[j] = 0;
     }
+    // first and last voxel positions of the row segment (they bound W over it)
+    const float px_first = __builtin_fmaf((float)(seg * 64 * J + a.vmin[0]), a.voxel[0], a.origin[0]);
+    const float px_last = __builtin_fmaf((float)(min(seg * 64 * J + 64 * J, s.nx) - 1 + a.vmin[0]),
+                                         a.voxel[0], a.origin[0]);
     const float line_off = (float)a.line_offset;
-    for (uint32_t view = 0, m = 0; view < a.n_views; ++view)
-    for (uint32_t mm = 0; mm < a.modules; ++mm, ++m) {
-        const ViewConsts c = view_consts(pd + 20u * m, py, pz, a, view);
+    const uint32_t view_elems = a.det_width * (uint32_t)a.line_rows;
+    const uint32_t esz = (uint32_t)sizeof(typename ES::T);
+    for (uint32_t view = 0, m = 0; view < a.n_views; ++view) {
+        const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(err + (uint64_t)view_of(a, view) * view_elems), 0, (int)(view_elems * esz),
+            kBufferWord3);
+        for (uint32_t mm = 0; mm < a.modules; ++mm, ++m) {
+            const float *__restrict__ p = pd + 20u * m;
+            ViewConsts c;
+            view_thresholds(p, a, c);
+            row_terms(p, py, pz, c);
+            if (fast_rcp_range(p, c, px_first, px_last, py, pz)) {
 #pragma unroll
-        for (int j = 0; j < J; ++j) {
-            uint64_t idx;
-            const bool inb = project(c, px[j], line_off, a.det_width, idx);
-            // unconditional load from a valid address keeps J loads in flight
-            const float e = ES::load(err, inb ? idx : c.base);
-            sum[j] += inb ? e : 0.0f;
-            cnt[j] += inb ? 1u : 0u;
+                for (int j = 0; j < J; ++j) {
+                    uint32_t iu, iv;
+                    const bool inb = project<CROP, true>(c, px[j], line_off, iu, iv);
+                    const uint32_t off = (__umul24(iv, a.det_width) + iu) * esz;
+                    sum[j] += load_or_zero<ES>(rsrc, inb ? off : kOutOfRange);
+                    cnt[j] += inb ? 1u : 0u;
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < J; ++j) {
+                    uint32_t iu, iv;
+                    const bool inb = project<CROP, false>(c, px[j], line_off, iu, iv);
+                    const uint32_t off = (__umul24(iv, a.det_width) + iu) * esz;
+                    sum[j] += load_or_zero<ES>(rsrc, inb ? off : kOutOfRange);
+                    cnt[j] += inb ? 1u : 0u;
+                }
+            }
         }
     }
     uint32_t empty = 0;
@@ -197,7 +319,7 @@ __global__ __launch_bounds__(kThreads) void k_backward(
         const int32_t x = xb + 64 * j;
         if (x < s.nx) {
             if (cnt[j] == 0) {
-                ++empty;
+                ++empty;              // the reference computes exp(0/0) here
             } else {
                 const float v = VS::load(vol, rowoff + x);
                 VS::store(vol, rowoff + x, v * expf(sum[j] / (float)cnt[j]));
@@ -205,7 +327,6 @@ __global__ __launch_bounds__(kThreads) void k_backward(
         }
     }
     if (empty_count != nullptr) {
-        // wave-level sum, one atomic per wave
         uint32_t e = empty;
         for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off);
         if (lane == 0 && e) atomicAdd(empty_count, (unsigned long long)e);
@@ -228,25 +349,20 @@ __device__ __forceinline__ uint64_t q22_u64(float v)
     return (uint64_t)s;
 }
 
-template <typename AccT>
-__device__ __forceinline__ AccT q22(float v);
-template <>
-__device__ __forceinline__ uint32_t q22<uint32_t>(float v) { return q22_u32(v); }
-template <>
-__device__ __forceinline__ uint64_t q22<uint64_t>(float v) { return q22_u64(v); }
-
 // ---------------------------------------------------------------------------
-// Forward with LDS binning.  A workgroup owns a brick of 64 (x) x 4*LW
-// (layers) x BK (depths) voxels; wave w holds layers w, w+4, ... and lane l the
-// column x0+l.  Per view, the brick's 8 corners bound its detector footprint
+// Forward with LDS binning.  A workgroup owns a brick of 64*JX (x) x 4
+// (layers, one per wave) x BK (depths); lane l holds the columns
+// x0 + l + 64 j.  Per view the brick's 8 corners bound its detector footprint
 // (one pixel of margin covers float rounding); voxel-views accumulate there
-// with LDS u64 atomics and the non-zero footprint pixels are added to the
-// detector sums with one global atomic each.  Footprints larger than the tile,
-// corners behind the source and bricks holding Q.22 values above 2^32 take the
-// direct path (one global atomic per voxel-view).  All paths add the same
-// integers, so the sums are bit-identical whatever path or launch order.
+// with LDS u64 atomics and the non-zero footprint pixels go to the detector
+// sums with one global atomic each.  Two tiles alternate: the flush of view v
+// also clears its tile, so one barrier per view separates splat and flush.
+// Footprints above the tile capacity, corners behind the source and bricks
+// holding Q.22 values above 2^32 take the direct path (one global atomic per
+// voxel-view).  Every path adds the same integers: the sums are bit-identical
+// whatever the path, the tiling or the launch order.
 // ---------------------------------------------------------------------------
-constexpr int kTileCap = 2048;
+constexpr int kTileCap = 1536;
 
 struct BrickShape {
     int32_t nx, nl, nk;      // slab extent (x, layers, depths)
@@ -262,13 +378,109 @@ __device__ __forceinline__ void global_add(AccT *p, AccT v)
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <typename VS, typename AccT, int LW, int BK>
+struct Footprint {
+    int32_t u0, v0, tw, th;
+    bool ok;       // corners in front of the source, finite
+    bool miss;     // the brick cannot reach the view's bounds
+    bool fast;     // every voxel's W is inside the fast reciprocal's range
+};
+
+template <bool CROP>
+__device__ __forceinline__ Footprint footprint(const float *__restrict__ p, float cpx, float cpy,
+                                               float cpz, float line_off, const xpr_proj_args &a)
+{
+    // lanes form identical groups of 8 corners; every lane ends with the result
+    const float X = __builtin_fmaf(p[0], cpx, __builtin_fmaf(p[1], cpy, __builtin_fmaf(p[2], cpz, p[3])));
+    const float Y = __builtin_fmaf(p[4], cpx, __builtin_fmaf(p[5], cpy, __builtin_fmaf(p[6], cpz, p[7])));
+    const float Wc = __builtin_fmaf(p[12], cpx, __builtin_fmaf(p[13], cpy, __builtin_fmaf(p[14], cpz, p[15])));
+    const float r = 1.0f / Wc;
+    float umin = X * r, vmin = CROP ? Y * r - line_off : Y * r;
+    float umax = umin, vmax = vmin;
+    int ok = (Wc > 0.0f) & (umin == umin) & (vmin == vmin) & (fabsf(umin) < 1e7f) &
+             (fabsf(vmin) < 1e7f);
+    // every voxel W of the brick lies between the corner values (fast_rcp_box)
+    const float tmag = fabsf(p[12] * cpx) + fabsf(p[13] * cpy) + fabsf(p[14] * cpz) + fabsf(p[15]);
+    int fast = fast_rcp_box(Wc, tmag);
+#pragma unroll
+    for (int off = 1; off < 8; off <<= 1) {
+        umin = fminf(umin, __shfl_xor(umin, off));
+        umax = fmaxf(umax, __shfl_xor(umax, off));
+        vmin = fminf(vmin, __shfl_xor(vmin, off));
+        vmax = fmaxf(vmax, __shfl_xor(vmax, off));
+        ok &= __shfl_xor(ok, off);
+        fast &= __shfl_xor(fast, off);
+    }
+    Footprint f{0, 0, 0, 0, ok != 0, false, fast != 0};
+    if (f.ok) {
+        const float vpl = fmaxf(p[16], 0.0f), vpr = fminf(p[17], (float)a.det_width);
+        const float vlo = fmaxf(p[18], 0.0f), vhi = fminf(p[19], (float)a.line_rows);
+        f.u0 = (int32_t)fmaxf(floorf(umin) - 1.0f, vpl);
+        f.v0 = (int32_t)fmaxf(floorf(vmin) - 1.0f, vlo);
+        const int32_t u1 = (int32_t)fminf(ceilf(umax) + 1.0f, vpr - 1.0f);
+        const int32_t v1 = (int32_t)fminf(ceilf(vmax) + 1.0f, vhi - 1.0f);
+        f.tw = u1 - f.u0 + 1;
+        f.th = v1 - f.v0 + 1;
+        f.miss = f.tw <= 0 || f.th <= 0;
+    }
+    return f;
+}
+
+// Workgroup barrier that orders LDS only.  The tile protocol needs no
+// ordering of the global atomics of a flush, so the wave does not wait for
+// them here (__syncthreads would also drain every outstanding global access).
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+template <typename VS, typename AccT, int JX, int BK, bool CROP>
+__device__ __noinline__ void forward_direct_u64(const typename VS::T *__restrict__ vol,
+                                                AccT *__restrict__ sums,
+                                                const float *__restrict__ pd,
+                                                const xpr_proj_args &a, const BrickShape &s,
+                                                int32_t x0, int32_t l0, int32_t k0)
+{
+    const int lane = threadIdx.x & 63;
+    const int32_t l = l0 + (threadIdx.x >> 6);
+    if (l >= s.nl) return;
+    const float py = __builtin_fmaf((float)(l + a.vmin[1]), a.voxel[1], a.origin[1]);
+    const float line_off = (float)a.line_offset;
+    const uint32_t view_elems = a.det_width * (uint32_t)a.line_rows;
+    for (int j = 0; j < JX; ++j) {
+        const int32_t x = x0 + lane + 64 * j;
+        const float px = __builtin_fmaf((float)(x + a.vmin[0]), a.voxel[0], a.origin[0]);
+        for (int i = 0; i < BK; ++i) {
+            const int32_t k = k0 + i;
+            if (x >= s.nx || k >= s.nk) continue;
+            const float v = VS::load(vol, ((uint64_t)l * s.nk + k) * s.nx + x);
+            const AccT q = sizeof(AccT) == 8 ? (AccT)q22_u64(v) : (AccT)q22_u32(v);
+            if (q == 0) continue;
+            const float pz = __builtin_fmaf((float)(k + a.vmin[2]), a.voxel[2], a.origin[2]);
+            for (uint32_t view = 0, m = 0; view < a.n_views; ++view) {
+                AccT *vsum = sums + (uint64_t)view_of(a, view) * view_elems;
+                for (uint32_t mm = 0; mm < a.modules; ++mm, ++m) {
+                    const float *__restrict__ p = pd + 20u * m;
+                    ViewConsts c;
+                    view_thresholds(p, a, c);
+                    row_terms(p, py, pz, c);
+                    uint32_t iu, iv;
+                    if (project<CROP, false>(c, px, line_off, iu, iv))
+                        global_add(vsum + (__umul24(iv, a.det_width) + iu), q);
+                }
+            }
+        }
+    }
+}
+
+template <typename VS, typename AccT, int JX, int BK, bool CROP>
 __global__ __launch_bounds__(kThreads) void k_forward_lds(
     const typename VS::T *__restrict__ vol, AccT *__restrict__ sums,
     const float *__restrict__ pd, xpr_proj_args a, BrickShape s)
 {
-    constexpr int BL = kWaves * LW;
-    __shared__ unsigned long long tile[kTileCap];
+    // two tiles, then one private slot per lane: voxel-views outside the
+    // footprint add 0 to their lane's slot, so the splat needs no branch
+    __shared__ unsigned long long tile[2][kTileCap];
+    __shared__ unsigned long long sink[kThreads];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     int64_t b = s.brick_base + blockIdx.x;
@@ -277,15 +489,15 @@ __global__ __launch_bounds__(kThreads) void k_forward_lds(
     b /= s.nbx;
     const int32_t bk = (int32_t)(b % s.nbk);
     const int32_t bl = (int32_t)(b / s.nbk);
-    const int32_t x0 = bx * 64, l0 = bl * BL, k0 = bk * BK;
-    const int32_t x = x0 + lane;
+    const int32_t x0 = bx * 64 * JX, l0 = bl * kWaves, k0 = bk * BK;
+    const int32_t l = l0 + wave;
 
-    // voxel values as Q.22 (32 bits suffice unless some value is >= 1024)
-    uint32_t q[LW][BK];
+    // Q.22 voxel values (32 bits suffice unless some value is >= 1024)
+    uint32_t q[JX][BK];
     bool big = false;
 #pragma unroll
-    for (int j = 0; j < LW; ++j) {
-        const int32_t l = l0 + wave + kWaves * j;
+    for (int j = 0; j < JX; ++j) {
+        const int32_t x = x0 + lane + 64 * j;
 #pragma unroll
         for (int i = 0; i < BK; ++i) {
             const int32_t k = k0 + i;
@@ -303,101 +515,76 @@ __global__ __launch_bounds__(kThreads) void k_forward_lds(
             q[j][i] = qq;
         }
     }
-    const bool any_big = __syncthreads_or(big);
-    const float px = __builtin_fmaf((float)(x + a.vmin[0]), a.voxel[0], a.origin[0]);
+    if (__syncthreads_or(big)) {
+        // some value needs more than 32 bits: one global atomic per voxel-view
+        // with the exact 64-bit value (values >= 1024 do not occur in practice)
+        forward_direct_u64<VS, AccT, JX, BK, CROP>(vol, sums, pd, a, s, x0, l0, k0);
+        return;
+    }
+    float px[JX];
+#pragma unroll
+    for (int j = 0; j < JX; ++j)
+        px[j] = __builtin_fmaf((float)(x0 + lane + 64 * j + a.vmin[0]), a.voxel[0], a.origin[0]);
+    const float py = __builtin_fmaf((float)(l + a.vmin[1]), a.voxel[1], a.origin[1]);
 
-    // this lane's brick corner (lanes form identical groups of 8 corners)
-    const int32_t xe = min(x0 + 63, s.nx - 1), le = min(l0 + BL - 1, s.nl - 1),
+    // this lane's brick corner
+    const int32_t xe = min(x0 + 64 * JX - 1, s.nx - 1), le = min(l0 + kWaves - 1, s.nl - 1),
                   ke = min(k0 + BK - 1, s.nk - 1);
-    const int32_t cxi = (lane & 1) ? xe : x0, cli = (lane & 2) ? le : l0,
-                  cki = (lane & 4) ? ke : k0;
-    const float cpx = __builtin_fmaf((float)(cxi + a.vmin[0]), a.voxel[0], a.origin[0]);
-    const float cpy = __builtin_fmaf((float)(cli + a.vmin[1]), a.voxel[1], a.origin[1]);
-    const float cpz = __builtin_fmaf((float)(cki + a.vmin[2]), a.voxel[2], a.origin[2]);
+    const float cpx = __builtin_fmaf((float)(((lane & 1) ? xe : x0) + a.vmin[0]), a.voxel[0], a.origin[0]);
+    const float cpy = __builtin_fmaf((float)(((lane & 2) ? le : l0) + a.vmin[1]), a.voxel[1], a.origin[1]);
+    const float cpz = __builtin_fmaf((float)(((lane & 4) ? ke : k0) + a.vmin[2]), a.voxel[2], a.origin[2]);
     const float line_off = (float)a.line_offset;
-    const uint64_t view_stride = (uint64_t)a.det_width * (uint64_t)a.line_rows;
+    const uint32_t view_elems = a.det_width * (uint32_t)a.line_rows;
 
-    for (uint32_t view = 0, m = 0; view < a.n_views; ++view)
-    for (uint32_t mm = 0; mm < a.modules; ++mm, ++m) {
-        const float *__restrict__ p = pd + 20u * m;
-        const uint64_t base = (uint64_t)(a.view_index ? a.view_index[view] : view) * view_stride;
-        // ---- footprint of the brick ----------------------------------------
-        const float X = __builtin_fmaf(p[0], cpx, __builtin_fmaf(p[1], cpy, __builtin_fmaf(p[2], cpz, p[3])));
-        const float Y = __builtin_fmaf(p[4], cpx, __builtin_fmaf(p[5], cpy, __builtin_fmaf(p[6], cpz, p[7])));
-        const float Wc = __builtin_fmaf(p[12], cpx, __builtin_fmaf(p[13], cpy, __builtin_fmaf(p[14], cpz, p[15])));
-        const float r = 1.0f / Wc;
-        float umin = X * r, vmin = Y * r - line_off;
-        float umax = umin, vmax = vmin;
-        int ok = (Wc > 0.0f) & (umin == umin) & (vmin == vmin) & (fabsf(umin) < 1e7f) &
-                 (fabsf(vmin) < 1e7f);
-#pragma unroll
-        for (int off = 1; off < 8; off <<= 1) {
-            umin = fminf(umin, __shfl_xor(umin, off));
-            umax = fmaxf(umax, __shfl_xor(umax, off));
-            vmin = fminf(vmin, __shfl_xor(vmin, off));
-            vmax = fmaxf(vmax, __shfl_xor(vmax, off));
-            ok &= __shfl_xor(ok, off);
-        }
-        const float vpl = fmaxf(p[16], 0.0f), vpr = fminf(p[17], (float)a.det_width);
-        const float vlo = fmaxf(p[18], 0.0f), vhi = fminf(p[19], (float)a.line_rows);
-        int32_t u0 = 0, v0 = 0, tw = 0, th = 0;
-        if (ok) {
-            u0 = (int32_t)fmaxf(floorf(umin) - 1.0f, vpl);
-            v0 = (int32_t)fmaxf(floorf(vmin) - 1.0f, vlo);
-            const int32_t u1 = (int32_t)fminf(ceilf(umax) + 1.0f, vpr - 1.0f);
-            const int32_t v1 = (int32_t)fminf(ceilf(vmax) + 1.0f, vhi - 1.0f);
-            tw = u1 - u0 + 1;
-            th = v1 - v0 + 1;
-            if (tw <= 0 || th <= 0) continue;         // the brick misses this view
-        }
-        const bool use_tile = ok && !any_big && tw * th <= s.cap;
-        if (use_tile) {
-            for (int rr = wave; rr < th; rr += kWaves)
-                for (int cc = lane; cc < tw; cc += 64) tile[rr * tw + cc] = 0ull;
-            __syncthreads();
-        }
-        // ---- splat ------------------------------------------------------------
-#pragma unroll
-        for (int j = 0; j < LW; ++j) {
-            const int32_t l = l0 + wave + kWaves * j;
-            const float py = __builtin_fmaf((float)(l + a.vmin[1]), a.voxel[1], a.origin[1]);
+    for (int t = threadIdx.x; t < 2 * kTileCap; t += kThreads) (&tile[0][0])[t] = 0ull;
+    __syncthreads();
+    int buf = 0;
+    for (uint32_t view = 0, m = 0; view < a.n_views; ++view) {
+        AccT *__restrict__ vsum = sums + (uint64_t)view_of(a, view) * view_elems;
+        for (uint32_t mm = 0; mm < a.modules; ++mm, ++m) {
+            const float *__restrict__ p = pd + 20u * m;
+            const Footprint f = footprint<CROP>(p, cpx, cpy, cpz, line_off, a);
+            if (f.ok && f.miss) continue;                  // uniform: no voxel lands in bounds
+            const bool use_tile = f.ok && f.tw * f.th <= s.cap;
+            unsigned long long *__restrict__ tl = tile[buf];
+            ViewConsts c;
+            view_thresholds(p, a, c);
 #pragma unroll
             for (int i = 0; i < BK; ++i) {
                 const float pz = __builtin_fmaf((float)(k0 + i + a.vmin[2]), a.voxel[2], a.origin[2]);
-                const float Xv = __builtin_fmaf(p[0], px, __builtin_fmaf(p[1], py, __builtin_fmaf(p[2], pz, p[3])));
-                const float Yv = __builtin_fmaf(p[4], px, __builtin_fmaf(p[5], py, __builtin_fmaf(p[6], pz, p[7])));
-                const float Wv = __builtin_fmaf(p[12], px, __builtin_fmaf(p[13], py, __builtin_fmaf(p[14], pz, p[15])));
-                const float rv = 1.0f / Wv;
-                const float cu = __builtin_roundf(Xv * rv);
-                const float cv = __builtin_roundf(Yv * rv - line_off);
-                const bool inb = (cu >= vpl) & (cu < vpr) & (cv >= vlo) & (cv < vhi);
-                const uint32_t qq = q[j][i];
-                if (inb && qq != 0u) {
-                    const int32_t iu = (int32_t)cu, iv = (int32_t)cv;
-                    const uint32_t tu = (uint32_t)(iu - u0), tv = (uint32_t)(iv - v0);
-                    if (use_tile && tu < (uint32_t)tw && tv < (uint32_t)th) {
-                        atomicAdd(&tile[tv * tw + tu], (unsigned long long)qq);
-                    } else {
-                        AccT qa = (AccT)qq;
-                        if (sizeof(AccT) == 8 && any_big) {
-                            // exact 64-bit value: reload and convert again
-                            qa = (AccT)q22_u64(VS::load(vol, ((uint64_t)l * s.nk + k0 + i) * s.nx + x));
+                row_terms(p, py, pz, c);
+                const bool fast = f.fast;
+#pragma unroll
+                for (int j = 0; j < JX; ++j) {
+                    const uint32_t qq = q[j][i];
+                    uint32_t iu, iv;
+                    const bool inb = fast ? project<CROP, true>(c, px[j], line_off, iu, iv)
+                                          : project<CROP, false>(c, px[j], line_off, iu, iv);
+                    const uint32_t tu = iu - (uint32_t)f.u0;
+                    const uint32_t tv = iv - (uint32_t)f.v0;
+                    const bool in_tile = use_tile & (tu < (uint32_t)f.tw) & (tv < (uint32_t)f.th);
+                    unsigned long long *slot = (inb & in_tile) ? &tl[tv * f.tw + tu]
+                                                               : &sink[threadIdx.x];
+                    atomicAdd(slot, (unsigned long long)((inb & in_tile) ? qq : 0u));
+                    if (inb & !in_tile & (qq != 0u))   // off the tile: direct global add
+                        global_add(vsum + (__umul24(iv, a.det_width) + iu), (AccT)qq);
+                }
+            }
+            if (use_tile) {
+                lds_barrier();                // splat of this view complete
+                // flush and clear; the next view splats into the other tile
+                for (int rr = wave; rr < f.th; rr += kWaves) {
+                    AccT *dst = vsum + (uint32_t)(f.v0 + rr) * a.det_width + (uint32_t)f.u0;
+                    for (int cc = lane; cc < f.tw; cc += 64) {
+                        const unsigned long long t = tl[rr * f.tw + cc];
+                        if (t != 0ull) {
+                            global_add(dst + cc, (AccT)t);
+                            tl[rr * f.tw + cc] = 0ull;
                         }
-                        global_add(sums + base + (uint64_t)((uint32_t)iv * a.det_width + (uint32_t)iu), qa);
                     }
                 }
+                buf ^= 1;
             }
-        }
-        if (use_tile) {
-            __syncthreads();
-            for (int rr = wave; rr < th; rr += kWaves) {
-                AccT *row = sums + base + (uint64_t)(uint32_t)(v0 + rr) * a.det_width + (uint32_t)u0;
-                for (int cc = lane; cc < tw; cc += 64) {
-                    const unsigned long long t = tile[rr * tw + cc];
-                    if (t != 0ull) global_add(row + cc, (AccT)t);
-                }
-            }
-            __syncthreads();
         }
     }
 }
@@ -537,13 +724,11 @@ int check_args(const xpr_proj_args *a, SlabShape &s)
     return XPR_OK;
 }
 
-constexpr int kJ = 4;
-
 template <typename Launch>
-int launch_rows(SlabShape s, Launch &&launch)
+int launch_rows(SlabShape s, int J, Launch &&launch)
 {
     if (s.nx == 0 || s.nrows == 0) return XPR_OK;
-    s.nseg = (s.nx + 64 * kJ - 1) / (64 * kJ);
+    s.nseg = (s.nx + 64 * J - 1) / (64 * J);
     // keep grid * block below 2^31 per launch
     const int64_t max_blocks = (int64_t(1) << 31) / kThreads;
     const int64_t rows_per_launch = (max_blocks / s.nseg) * kWaves;
@@ -558,19 +743,26 @@ int launch_rows(SlabShape s, Launch &&launch)
     return XPR_OK;
 }
 
+constexpr int kBwdJ = 8;
+
 template <typename VS, typename ES>
 int backward_impl(hipStream_t st, void *vol, const void *err, const float *pd,
                   const xpr_proj_args &a, SlabShape s, uint64_t *empty)
 {
-    return launch_rows(s, [&](unsigned blocks, const SlabShape &t) {
-        hipLaunchKernelGGL((k_backward<VS, ES, kJ>), dim3(blocks), dim3(kThreads), 0, st,
-                           (typename VS::T *)vol, (const typename ES::T *)err, pd, a, t,
-                           (unsigned long long *)empty);
+    if ((uint64_t)a.det_width * (uint64_t)a.line_rows * sizeof(typename ES::T) >= 0x80000000ull)
+        return fail(XPR_E_INVALID, "a detector view above 2 GiB");
+    const bool crop = a.line_offset != 0;
+    return launch_rows(s, kBwdJ, [&](unsigned blocks, const SlabShape &t) {
+        if (crop)
+            hipLaunchKernelGGL((k_backward<VS, ES, kBwdJ, true>), dim3(blocks), dim3(kThreads), 0,
+                               st, (typename VS::T *)vol, (const typename ES::T *)err, pd, a, t,
+                               (unsigned long long *)empty);
+        else
+            hipLaunchKernelGGL((k_backward<VS, ES, kBwdJ, false>), dim3(blocks), dim3(kThreads), 0,
+                               st, (typename VS::T *)vol, (const typename ES::T *)err, pd, a, t,
+                               (unsigned long long *)empty);
     });
 }
-
-constexpr int kFwdLW = 2;   // layers per wave
-constexpr int kFwdBK = 16;  // depths per brick
 
 int forward_tile_cap()
 {
@@ -580,31 +772,62 @@ int forward_tile_cap()
     return c < 0 ? 0 : (c > kTileCap ? kTileCap : c);
 }
 
-template <typename VS, typename AccT>
-int forward_impl(hipStream_t st, const void *vol, void *sums, const float *pd,
-                 const xpr_proj_args &a, SlabShape s)
+// brick shapes compiled: 64*JX columns x 4 layers x BK depths
+enum class Brick { b2x16, b1x32, b2x32, b1x16 };
+
+Brick forward_brick()
 {
-    constexpr int BL = kWaves * kFwdLW;
+    // tuning knob: brick shape "JXxBK"
+    const char *e = std::getenv("XPR_FORWARD_BRICK");
+    const std::string v = e ? e : "1x16";
+    if (v == "1x32") return Brick::b1x32;
+    if (v == "2x32") return Brick::b2x32;
+    if (v == "2x16") return Brick::b2x16;
+    return Brick::b1x16;
+}
+
+template <typename VS, typename AccT, int JX, int BK>
+int forward_launch(hipStream_t st, const void *vol, void *sums, const float *pd,
+                   const xpr_proj_args &a, SlabShape s)
+{
     BrickShape b;
     b.nx = s.nx;
     b.nl = s.nl;
     b.nk = s.nk;
-    b.nbx = (s.nx + 63) / 64;
-    b.nbk = (s.nk + kFwdBK - 1) / kFwdBK;
-    const int64_t nbl = (s.nl + BL - 1) / BL;
+    b.nbx = (s.nx + 64 * JX - 1) / (64 * JX);
+    b.nbk = (s.nk + BK - 1) / BK;
+    const int64_t nbl = (s.nl + kWaves - 1) / kWaves;
     b.nbricks = (int64_t)b.nbx * b.nbk * nbl;
     b.cap = forward_tile_cap();
+    const bool crop = a.line_offset != 0;
     const int64_t max_blocks = (int64_t(1) << 31) / kThreads;
     for (int64_t b0 = 0; b0 < b.nbricks; b0 += max_blocks) {
         BrickShape t = b;
         t.brick_base = b0;
         const int64_t blocks = std::min<int64_t>(max_blocks, b.nbricks - b0);
-        hipLaunchKernelGGL((k_forward_lds<VS, AccT, kFwdLW, kFwdBK>), dim3((unsigned)blocks),
-                           dim3(kThreads), 0, st, (const typename VS::T *)vol, (AccT *)sums, pd,
-                           a, t);
+        if (crop)
+            hipLaunchKernelGGL((k_forward_lds<VS, AccT, JX, BK, true>), dim3((unsigned)blocks),
+                               dim3(kThreads), 0, st, (const typename VS::T *)vol, (AccT *)sums,
+                               pd, a, t);
+        else
+            hipLaunchKernelGGL((k_forward_lds<VS, AccT, JX, BK, false>), dim3((unsigned)blocks),
+                               dim3(kThreads), 0, st, (const typename VS::T *)vol, (AccT *)sums,
+                               pd, a, t);
         XPR_LAUNCH_CHECK();
     }
     return XPR_OK;
+}
+
+template <typename VS, typename AccT>
+int forward_impl(hipStream_t st, const void *vol, void *sums, const float *pd,
+                 const xpr_proj_args &a, SlabShape s)
+{
+    switch (forward_brick()) {
+    case Brick::b1x32: return forward_launch<VS, AccT, 1, 32>(st, vol, sums, pd, a, s);
+    case Brick::b2x32: return forward_launch<VS, AccT, 2, 32>(st, vol, sums, pd, a, s);
+    case Brick::b1x16: return forward_launch<VS, AccT, 1, 16>(st, vol, sums, pd, a, s);
+    default: return forward_launch<VS, AccT, 2, 16>(st, vol, sums, pd, a, s);
+    }
 }
 
 template <typename IS, typename AccT>
