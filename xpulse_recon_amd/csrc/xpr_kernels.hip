@@ -472,6 +472,37 @@ __device__ __noinline__ void forward_direct_u64(const typename VS::T *__restrict
     }
 }
 
+// splat of one view over the brick: every voxel-view adds its Q.22 value to
+// its footprint tile entry (voxel-views out of bounds add 0 to the lane's
+// sink slot); one off the tile goes straight to the detector sums
+template <bool FAST, bool CROP, typename AccT, int JX, int BK>
+__device__ __forceinline__ void splat_brick(ViewConsts &c, const float *__restrict__ p,
+                                            const uint32_t (&q)[JX][BK], const float (&px)[JX],
+                                            float py, int32_t k0, float line_off, const Footprint &f,
+                                            bool use_tile, unsigned long long *__restrict__ tl,
+                                            unsigned long long *__restrict__ sink,
+                                            AccT *__restrict__ vsum, const xpr_proj_args &a)
+{
+#pragma unroll
+    for (int i = 0; i < BK; ++i) {
+        const float pz = __builtin_fmaf((float)(k0 + i + a.vmin[2]), a.voxel[2], a.origin[2]);
+        row_terms(p, py, pz, c);
+#pragma unroll
+        for (int j = 0; j < JX; ++j) {
+            const uint32_t qq = q[j][i];
+            uint32_t iu, iv;
+            const bool inb = project<CROP, FAST>(c, px[j], line_off, iu, iv);
+            const uint32_t tu = iu - (uint32_t)f.u0;
+            const uint32_t tv = iv - (uint32_t)f.v0;
+            const bool in_tile = use_tile & (tu < (uint32_t)f.tw) & (tv < (uint32_t)f.th);
+            unsigned long long *slot = (inb & in_tile) ? &tl[tv * f.tw + tu] : &sink[threadIdx.x];
+            atomicAdd(slot, (unsigned long long)((inb & in_tile) ? qq : 0u));
+            if (inb & !in_tile & (qq != 0u))   // off the tile: direct global add
+                global_add(vsum + (__umul24(iv, a.det_width) + iu), (AccT)qq);
+        }
+    }
+}
+
 template <typename VS, typename AccT, int JX, int BK, bool CROP>
 __global__ __launch_bounds__(kThreads) void k_forward_lds(
     const typename VS::T *__restrict__ vol, AccT *__restrict__ sums,
@@ -549,27 +580,12 @@ __global__ __launch_bounds__(kThreads) void k_forward_lds(
             unsigned long long *__restrict__ tl = tile[buf];
             ViewConsts c;
             view_thresholds(p, a, c);
-#pragma unroll
-            for (int i = 0; i < BK; ++i) {
-                const float pz = __builtin_fmaf((float)(k0 + i + a.vmin[2]), a.voxel[2], a.origin[2]);
-                row_terms(p, py, pz, c);
-                const bool fast = f.fast;
-#pragma unroll
-                for (int j = 0; j < JX; ++j) {
-                    const uint32_t qq = q[j][i];
-                    uint32_t iu, iv;
-                    const bool inb = fast ? project<CROP, true>(c, px[j], line_off, iu, iv)
-                                          : project<CROP, false>(c, px[j], line_off, iu, iv);
-                    const uint32_t tu = iu - (uint32_t)f.u0;
-                    const uint32_t tv = iv - (uint32_t)f.v0;
-                    const bool in_tile = use_tile & (tu < (uint32_t)f.tw) & (tv < (uint32_t)f.th);
-                    unsigned long long *slot = (inb & in_tile) ? &tl[tv * f.tw + tu]
-                                                               : &sink[threadIdx.x];
-                    atomicAdd(slot, (unsigned long long)((inb & in_tile) ? qq : 0u));
-                    if (inb & !in_tile & (qq != 0u))   // off the tile: direct global add
-                        global_add(vsum + (__umul24(iv, a.det_width) + iu), (AccT)qq);
-                }
-            }
+            if (f.fast)
+                splat_brick<true, CROP, AccT, JX, BK>(c, p, q, px, py, k0, line_off, f, use_tile, tl,
+                                                      sink, vsum, a);
+            else
+                splat_brick<false, CROP, AccT, JX, BK>(c, p, q, px, py, k0, line_off, f, use_tile, tl,
+                                                       sink, vsum, a);
             if (use_tile) {
                 lds_barrier();                // splat of this view complete
                 // flush and clear; the next view splats into the other tile
