@@ -48,7 +48,7 @@ def parse():
     ap.add_argument("--det", type=int, default=768)
     ap.add_argument("--views", type=int, default=720)
     ap.add_argument("--sit", type=int, default=20)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
@@ -84,6 +84,23 @@ def cpu_baseline(pd_sit, images_host, W, H, A_s, orig, vx, target_s):
             "sample": f"one sub-iteration (Forward + Error + Backward, {A_s} views) over a "
                       f"{L}-layer central slab of the {W}x{H}x{W} grid, FP32 canonical oracle, "
                       f"{t:.1f} s"}
+
+
+def measured_traffic(kernel, args):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary of this workload (calibrated against a copy
+    of known size), or None when the summary does not cover it."""
+    if (args.det, args.views, args.sit, args.slab) != (768, 720, 20, 64):
+        return None
+    path = os.path.join(HERE, "profiles", "pmc_traffic_768cube_f32.md")
+    try:
+        for line in open(path):
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic" and parts[1] == kernel:
+                return int(parts[2])
+    except OSError:
+        return None
+    return None
 
 
 def main():
@@ -193,8 +210,9 @@ def main():
     else:
         name, launch_ms, nbytes = "k_forward", fp_launch, fp_bytes
     ach = nbytes / (launch_ms * 1e-3) / 1e9
+    traffic = measured_traffic(name, args) if args.dtype == "f32" else None
     roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": name,
             "launch_ms": round(launch_ms, 4),
             "algorithmic_bytes_per_launch": int(nbytes),
             "other": {"forward_launch_ms": round(fp_launch, 4),
