@@ -51,6 +51,13 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve", action="store_true",
+                    help="serving run instead: Poisson arrivals of reconstructions of "
+                         "mixed sizes, p50/p99 time-to-reconstruction")
+    ap.add_argument("--jobs", type=int, default=24, help="jobs of the serving run")
+    ap.add_argument("--rate", type=float, default=4.0, help="mean arrivals per second")
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--max-concurrent", type=int, default=8)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one plain 1 GiB float copy (k_convert) so hardware "
                          "byte counters can be scaled against a known size")
@@ -103,6 +110,89 @@ def measured_traffic(kernel, args):
     return None
 
 
+SERVE_CLASSES = [(128, 180, 128), (384, 360, 256), (768, 720, 512)]   # detector, views, phantom
+
+
+def serve_main(args):
+    """Serving workload: jobs drawn uniformly from three sizes (128^2 detector /
+    180 views, 384^2 / 360, 768^2 / 720; grid = detector width), so ~ U[80,150],
+    sd ~ U[800,1200] mm, 12-ellipsoid phantoms, I0 = 1e4 Poisson, it=1, sit=10,
+    FP16 storage with a final FP32 sub-iteration, admitted by an HBM budget.
+    Arrivals are a Poisson process; latency = completion - arrival."""
+    import torch
+    from xpulse_recon_amd import synth
+    from xpulse_recon_amd.geometry import Geometry
+    from xpulse_recon_amd.serve import Server
+
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(2024)
+    jobs = []
+    t0 = time.perf_counter()
+    for j in range(args.jobs):
+        D, N, n_ph = SERVE_CLASSES[int(rng.integers(0, 3))]
+        so, sd = float(rng.uniform(80, 150)), float(rng.uniform(800, 1200))
+        geom = Geometry(dwidth=D, dheight=D, projections=N, so=so, sd=sd, px=0.1)
+        _, grid = geom.projdata()
+        ell = synth.random_ellipsoids(3000 + j, 12, grid["vwidth"] * grid["vx"], n_ph)
+        proj = np.empty((N, D, D), np.float32)
+        for v0 in range(0, N, 90):
+            views = list(range(v0, min(N, v0 + 90)))
+            p = synth.line_integrals(geom, views, ell, dev)
+            proj[v0:v0 + len(views)] = synth.transmission(p, 1e4, 5000 + j * 8 + v0).cpu().numpy()
+            del p
+        jobs.append((geom, proj, D, N))
+    torch.cuda.synchronize()
+    gaps = rng.exponential(1.0 / args.rate, args.jobs)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    log(f"{args.jobs} jobs synthesised in {time.perf_counter() - t0:.1f} s "
+        f"(sizes {[d for (_, _, d, _) in jobs]})")
+    kw = dict(it=1, sit=10, storage=args.dtype if args.dtype != "f32" else "f16",
+              final_fp32=True, jitter_seed=7)
+    budget = int(args.budget_gb * 1e9)
+    with Server(budget, args.max_concurrent) as sv:
+        g, p, _, _ = jobs[0]
+        sv.wait(sv.submit(g, p, **kw))                 # module load, pools
+        log("warm-up job done")
+        ids = []
+        start = time.perf_counter()
+        for (geom, proj, D, N), t in zip(jobs, arrivals):
+            dt = start + t - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            ids.append(sv.submit(geom, proj, **kw))
+        res = [sv.wait(i) for i in ids]
+    for r in res:
+        if not np.isfinite(r.volume).all():
+            raise SystemExit("non-finite volume in a served job")
+    lat = np.array([r.latency_ms for r in res])
+    t_first = min(r.submit_ms for r in res)
+    makespan = max(r.end_ms for r in res) - t_first
+    vv = sum(r.voxel_views for r in res)
+    by = {}
+    for (_, _, D, _), r in zip(jobs, res):
+        by.setdefault(D, []).append(r.latency_ms)
+    out = {
+        "metric": METRIC, "mode": "serve", "value": vv / (makespan * 1e-3),
+        "unit": "voxel-views/s", "n_gpus": 1, "higher_is_better": True,
+        "p50_ms": round(float(np.percentile(lat, 50)), 1),
+        "p99_ms": round(float(np.percentile(lat, 99)), 1),
+        "mean_ms": round(float(lat.mean()), 1), "makespan_ms": round(makespan, 1),
+        "device_busy_ms": round(sum(r.device_ms for r in res), 1),
+        "queue_ms_mean": round(float(np.mean([r.start_ms - r.submit_ms for r in res])), 1),
+        "per_size_p50_ms": {f"{d}^3": round(float(np.percentile(v, 50)), 1) for d, v in sorted(by.items())},
+        "per_size_jobs": {f"{d}^3": len(v) for d, v in sorted(by.items())},
+        "streamed_jobs": sum(r.streamed for r in res),
+        "dtype": kw["storage"], "data": "synthetic: 12 random ellipsoids per job, FP64 line "
+                                        "integrals, Poisson I0=1e4",
+        "config": {"workload": f"{args.jobs} jobs, Poisson arrivals {args.rate}/s, sizes "
+                               "{128^3/180 views, 384^3/360, 768^3/720}, so U[80,150], sd "
+                               "U[800,1200], it=1 sit=10, f16 storage + final FP32 "
+                               "sub-iteration", "budget_gb": args.budget_gb,
+                   "max_concurrent": args.max_concurrent},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -113,6 +203,8 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if args.serve:
+        return serve_main(args)
     import xpulse_recon_amd as X
     from xpulse_recon_amd import synth
     from xpulse_recon_amd.geometry import Geometry

@@ -83,6 +83,10 @@ int xpr_mem_info(size_t *free_bytes, size_t *total_bytes);
 int xpr_alloc(void **dptr, size_t bytes);
 int xpr_free(void *dptr);
 int xpr_host_alloc(void **hptr, size_t bytes);   /* pinned host memory */
+/* Stream-ordered allocation (no device-wide synchronisation), for work that
+ * runs concurrently on several streams. */
+int xpr_alloc_async(void **dptr, size_t bytes, xpr_stream s);
+int xpr_free_async(void *dptr, xpr_stream s);
 int xpr_host_free(void *hptr);
 int xpr_upload(xpr_stream s, void *dst, const void *src, size_t bytes);   /* async */
 int xpr_download(xpr_stream s, void *dst, const void *src, size_t bytes); /* async */
@@ -219,6 +223,50 @@ int xpr_reconstruct(const char *params, char *record, size_t record_cap);
 /* The default parameters as JSON (XPRecon -template).  0 on success, -1 if
  * cap is too small. */
 int xpr_template_json(char *out, size_t cap);
+
+/* ---- serving: concurrent reconstructions under an HBM budget ----------------
+ * (no counterpart in the reference, which runs one reconstruction per process)
+ *
+ * Jobs are admitted in arrival order while the device memory they need fits in
+ * the budget and a concurrency slot is free; each admitted job runs the naive
+ * schedule on its own stream, overlapping the jobs already running.  A job that
+ * cannot fit the budget even alone runs with slab streaming (volume held on the
+ * host).  Every job owns its buffers and its seeded jitter, so its result does
+ * not depend on what else runs. */
+typedef struct xpr_server xpr_server;
+
+typedef struct {
+    xpr_geometry_params geometry;    /* detector size and projection count included */
+    int64_t it, sit;
+    float weight, weight_factor;
+    int32_t mlog, normalize;
+    int32_t storage;                 /* XPR_F32 / XPR_F16 / XPR_BF16 */
+    int32_t final_fp32;              /* last sub-iteration in FP32 storage */
+    int32_t acc_bits;                /* XPR_ACC_U32 / XPR_ACC_U64 */
+    int32_t jitter;                  /* 0: no origin jitter */
+    int64_t jitter_seed;
+    int64_t slab_height;             /* used by the streaming fallback */
+    const float *projections;        /* host, [projections][dheight][dwidth], raw (as read from TIFF) */
+    float *volume_out;               /* host, [vheight][vwidth][vwidth] float32 */
+} xpr_job;
+
+typedef struct {
+    double submit_ms, start_ms, end_ms;   /* steady clock, ms */
+    double device_ms;                     /* sum of sub-iteration device times */
+    uint64_t device_bytes;                /* memory reserved against the budget */
+    uint64_t voxel_views;                 /* projector work (both projectors) */
+    int32_t streamed;                     /* 1: ran with the slab-streaming fallback */
+    int32_t status;                       /* XPR_OK or the job's error code */
+} xpr_job_stats;
+
+int xpr_server_create(xpr_server **out, uint64_t budget_bytes, int max_concurrent, int device);
+/* The job description is copied; its host buffers must stay valid until
+ * xpr_server_wait returns for that job. */
+int xpr_server_submit(xpr_server *sv, const xpr_job *job, uint64_t *job_id);
+int xpr_server_wait(xpr_server *sv, uint64_t job_id, xpr_job_stats *stats);
+/* Device memory a job needs when resident (the admission figure). */
+int xpr_job_device_bytes(const xpr_job *job, uint64_t *bytes);
+int xpr_server_destroy(xpr_server *sv);   /* waits for every job */
 
 #ifdef __cplusplus
 }

@@ -78,6 +78,28 @@ class Ellipsoid(C.Structure):
                 ("mu", C.c_double)]
 
 
+class Job(C.Structure):
+    """xpr_job: one reconstruction submitted to a server (host buffers)."""
+
+    _fields_ = [
+        ("geometry", GeometryParams),
+        ("it", C.c_int64), ("sit", C.c_int64),
+        ("weight", C.c_float), ("weight_factor", C.c_float),
+        ("mlog", C.c_int32), ("normalize", C.c_int32),
+        ("storage", C.c_int32), ("final_fp32", C.c_int32), ("acc_bits", C.c_int32),
+        ("jitter", C.c_int32), ("jitter_seed", C.c_int64), ("slab_height", C.c_int64),
+        ("projections", C.c_void_p), ("volume_out", C.c_void_p),
+    ]
+
+
+class JobStats(C.Structure):
+    _fields_ = [
+        ("submit_ms", C.c_double), ("start_ms", C.c_double), ("end_ms", C.c_double),
+        ("device_ms", C.c_double), ("device_bytes", C.c_uint64), ("voxel_views", C.c_uint64),
+        ("streamed", C.c_int32), ("status", C.c_int32),
+    ]
+
+
 # name -> (restype, argtypes)
 _P = C.c_void_p
 _SIGS = {
@@ -89,6 +111,8 @@ _SIGS = {
     "xpr_free": (C.c_int, [_P]),
     "xpr_host_alloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
     "xpr_host_free": (C.c_int, [_P]),
+    "xpr_alloc_async": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t, _P]),
+    "xpr_free_async": (C.c_int, [_P, _P]),
     "xpr_upload": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "xpr_download": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "xpr_stream_create": (C.c_int, [C.POINTER(C.c_void_p)]),
@@ -117,6 +141,11 @@ _SIGS = {
     "xpr_geometry_frames": (C.c_int, [C.POINTER(GeometryParams), _P]),
     "xpr_reconstruct": (C.c_int, [C.c_char_p, C.c_char_p, C.c_size_t]),
     "xpr_template_json": (C.c_int, [C.c_char_p, C.c_size_t]),
+    "xpr_server_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint64, C.c_int, C.c_int]),
+    "xpr_server_submit": (C.c_int, [_P, C.POINTER(Job), C.POINTER(C.c_uint64)]),
+    "xpr_server_wait": (C.c_int, [_P, C.c_uint64, C.POINTER(JobStats)]),
+    "xpr_job_device_bytes": (C.c_int, [C.POINTER(Job), C.POINTER(C.c_uint64)]),
+    "xpr_server_destroy": (C.c_int, [_P]),
     "xpr_synth_line_integrals": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, C.c_uint32,
                                            C.c_uint32, _P, C.c_int, _P]),
 }

@@ -33,7 +33,7 @@ def _run(cmd):
 
 BIN_DIR = os.path.join(HERE, "bin")
 CLI = os.path.join(BIN_DIR, "XPRecon")
-HOST_SRCS = ["json.cpp", "params.cpp", "io.cpp", "recon.cpp", "capi.cpp"]
+HOST_SRCS = ["json.cpp", "params.cpp", "io.cpp", "recon.cpp", "capi.cpp", "serve.cpp"]
 
 
 def build_hip_lib() -> str:

@@ -75,6 +75,11 @@ struct XprSection {
     int threads = 0;                 // host IO threads (0: hardware concurrency, max 16)
     std::string run_log = "";        // JSON run record (jitter, timings, memory)
     int device = 0;
+    // in-process jobs (not JSON): projections from memory, volume to memory
+    const float *mem_projections = nullptr;   // [n][h][w] raw images
+    int64_t mem_width = 0, mem_height = 0, mem_count = 0;
+    float *mem_volume_out = nullptr;          // [vheight][vwidth][vwidth]
+    bool quiet = false;                       // no progress output
 };
 
 struct Parameters {

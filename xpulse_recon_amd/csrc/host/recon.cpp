@@ -26,6 +26,12 @@ void check(int rc, const char *what)
 // ---------------------------------------------------------------------------
 // buffers
 // ---------------------------------------------------------------------------
+namespace {
+thread_local xpr_stream t_alloc_stream = nullptr;
+}
+AllocStreamScope::AllocStreamScope(xpr_stream s) : prev_(t_alloc_stream) { t_alloc_stream = s; }
+AllocStreamScope::~AllocStreamScope() { t_alloc_stream = prev_; }
+
 DevBuf::~DevBuf() { reset(); }
 DevBuf &DevBuf::operator=(DevBuf &&o) noexcept
 {
@@ -33,6 +39,7 @@ DevBuf &DevBuf::operator=(DevBuf &&o) noexcept
         reset();
         p_ = o.p_;
         n_ = o.n_;
+        s_ = o.s_;
         o.p_ = nullptr;
         o.n_ = 0;
     }
@@ -41,12 +48,25 @@ DevBuf &DevBuf::operator=(DevBuf &&o) noexcept
 void DevBuf::alloc(size_t bytes)
 {
     reset();
-    check(xpr_alloc(&p_, bytes), "device allocation");
+    s_ = t_alloc_stream;
+    if (s_) {
+        check(xpr_alloc_async(&p_, bytes, s_), "device allocation");
+        check(xpr_stream_sync(s_), "sync");
+    } else {
+        check(xpr_alloc(&p_, bytes), "device allocation");
+    }
     n_ = bytes;
 }
 void DevBuf::reset()
 {
-    if (p_) xpr_free(p_);
+    if (p_) {
+        if (s_) {
+            xpr_stream_sync(s_);            // work using the buffer is done
+            xpr_free_async(p_, s_);
+        } else {
+            xpr_free(p_);
+        }
+    }
     p_ = nullptr;
     n_ = 0;
 }
@@ -117,9 +137,17 @@ struct Event {
 // ---------------------------------------------------------------------------
 Dataset::Dataset(Parameters &p) : p_(p)
 {
+    if (p.xpr.mem_projections) {                  // in-process job
+        dwidth_ = p.xpr.mem_width;
+        dheight_ = p.xpr.mem_height;
+        nproj_ = p.xpr.mem_count;
+        if (dwidth_ <= 0 || dheight_ <= 0 || nproj_ <= 0)
+            throw std::runtime_error("empty projection stack");
+    }
     // collectFromDirectory + checkTiffFileInfos (dataset.hpp:214-269)
-    if (!fs::is_directory(p.recon.input))
+    if (!p.xpr.mem_projections && !fs::is_directory(p.recon.input))
         throw std::runtime_error("input directory not found: " + p.recon.input);
+    if (!p.xpr.mem_projections)
     for (const std::string &f : list_tiffs(p.recon.input)) {
         TiffInfo i = tiff_info(f);
         if (i.sample_format != 3 || i.bits != 32)
@@ -133,8 +161,10 @@ Dataset::Dataset(Parameters &p) : p_(p)
         }
         if (i.width == dwidth_ && i.height == dheight_) files_.push_back(f);
     }
-    if (files_.empty()) throw std::runtime_error("No valid TIFF file found in the directory.");
-    nproj_ = int64_t(files_.size());
+    if (!p.xpr.mem_projections) {
+        if (files_.empty()) throw std::runtime_error("No valid TIFF file found in the directory.");
+        nproj_ = int64_t(files_.size());
+    }
     if (nproj_ / p.recon.sit < 1)
         throw std::runtime_error("fewer projections than sub-iterations");
     if (!p.geometry.angle_list.empty() && int64_t(p.geometry.angle_list.size()) != nproj_)
@@ -172,17 +202,30 @@ Dataset::Dataset(Parameters &p) : p_(p)
     vwidth_ = grid.vwidth;
     vheight_ = grid.vheight;
     for (int i = 0; i < 3; ++i) orig_[i] = float(grid.orig[i]);
-    fs::create_directories(p.recon.output);
-    if (!p.recon.proj_output.empty()) fs::create_directories(p.recon.proj_output);
-    proj_dir_ = (fs::path(p.recon.output) / "projs").string();
-    std::cout << "Volume dimension : " << vwidth_ << "x" << vheight_ << "x" << vwidth_ << "\n"
-              << "Pixel size : " << p.detector.px << ", Voxel size : " << vx_ << std::endl;
+    if (!p.xpr.mem_volume_out) {
+        fs::create_directories(p.recon.output);
+        if (!p.recon.proj_output.empty()) fs::create_directories(p.recon.proj_output);
+        proj_dir_ = (fs::path(p.recon.output) / "projs").string();
+    }
+    if (!p.xpr.quiet)
+        std::cout << "Volume dimension : " << vwidth_ << "x" << vheight_ << "x" << vwidth_ << "\n"
+                  << "Pixel size : " << p.detector.px << ", Voxel size : " << vx_ << std::endl;
 }
 
 Dataset::~Dataset()
 {
+    if (proj_dir_.empty()) return;
     std::error_code ec;
     fs::remove_all(proj_dir_, ec);     // cleanTempImages (data_loader.hpp:107-111)
+}
+
+std::vector<float> Dataset::raw_image(int64_t i) const
+{
+    if (p_.xpr.mem_projections) {
+        const float *src = p_.xpr.mem_projections + size_t(i) * size_t(dwidth_ * dheight_);
+        return std::vector<float>(src, src + dwidth_ * dheight_);
+    }
+    return tiff_read_f32(files_[size_t(i)]);
 }
 
 std::vector<int64_t> Dataset::sit_views(int64_t sit) const
@@ -202,18 +245,25 @@ std::vector<float> Dataset::sit_projdata(int64_t sit) const
     return out;
 }
 
-void Dataset::initialize(int)
+void Dataset::initialize()
 {
+    if (initialized_) return;
+    initialized_ = true;
     const int64_t P = dwidth_ * dheight_;
     images_.assign(size_t(nproj_), {});
     xpr_stream s;
     check(xpr_stream_create(&s), "stream");
+    struct StreamGuard {
+        xpr_stream s;
+        ~StreamGuard() { xpr_stream_destroy(s); }
+    } guard{s};
+    AllocStreamScope scope(s);
     DevBuf d_in(size_t(P) * 4), d_out(size_t(P) * 4), d_mm(8);
     HostBuf h(size_t(P) * 4);
     const int threads = io_threads(p_);
-    // read all TIFFs (host threads), then preprocess on the device
+    // read all images (host threads), then preprocess on the device
     std::vector<std::vector<float>> raw(static_cast<size_t>(nproj_));
-    parallel_for(nproj_, threads, [&](int64_t i) { raw[size_t(i)] = tiff_read_f32(files_[size_t(i)]); });
+    parallel_for(nproj_, threads, [&](int64_t i) { raw[size_t(i)] = raw_image(i); });
     if (p_.recon.normalize) {
         check(xpr_minmax_init(s, d_mm.as<int32_t>()), "minmax init");
         for (int64_t i = 0; i < nproj_; ++i) {
@@ -238,7 +288,6 @@ void Dataset::initialize(int)
         raw[size_t(i)].clear();
         raw[size_t(i)].shrink_to_fit();
     }
-    xpr_stream_destroy(s);
 }
 
 std::vector<float> Dataset::sit_images(int64_t sit, int64_t row0, int64_t rows) const
@@ -293,6 +342,13 @@ void Dataset::load_layer(int64_t layer, float *out) const
 void Dataset::save_final_layers(const float *vol, int64_t l0, int64_t l1)
 {
     const size_t n = size_t(vwidth_ * vwidth_);
+    if (p_.xpr.mem_volume_out) {                  // in-process job: into the caller's buffer
+        float *dst = p_.xpr.mem_volume_out + size_t(l0) * n;
+        const float scale = mx_ - mn_;
+        for (size_t k = 0; k < size_t(l1 - l0) * n; ++k)
+            dst[k] = p_.recon.normalize ? vol[k] * scale : vol[k];
+        return;
+    }
     parallel_for(l1 - l0, io_threads(p_), [&](int64_t k) {
         const float *src = vol + size_t(k) * n;
         std::vector<float> tmp;
@@ -405,7 +461,8 @@ std::array<float, 3> Reconstruction::next_origin()
 void Reconstruction::check_memory(uint64_t need, const char *what)
 {
     note("required_device_bytes", Json::number(double(need)));
-    std::cout << "Checking GPU memory requierments : " << need / (1024 * 1024) << "Mo" << std::endl;
+    if (!p_.xpr.quiet)
+        std::cout << "Checking GPU memory requierments : " << need / (1024 * 1024) << "Mo" << std::endl;
     if (need > budget_)
         throw std::runtime_error(std::string("Not enough GPU memory (") + what + ": " +
                                  std::to_string(need >> 20) + " MiB needed, budget " +
@@ -477,7 +534,6 @@ public:
     explicit ReconstructionN(Parameters &p) : Reconstruction(p)
     {
         check_memory(requiredGPUMemory(), "naive");
-        ds_->initialize();
     }
     uint64_t requiredGPUMemory() override
     {
@@ -491,6 +547,8 @@ public:
     }
     void exec() override
     {
+        ds_->initialize();
+        AllocStreamScope scope(stream_);
         auto &ds = *ds_;
         const int st = storage();
         const int64_t W = ds.vwidth(), H = ds.vheight(), P = ds.width() * ds.height();
@@ -572,7 +630,6 @@ public:
     ReconstructionSlab(Parameters &p, int threads, bool cull) : Reconstruction(p), threads_(threads), cull_(cull)
     {
         check_memory(requiredGPUMemory(), method_name(p.recon.method));
-        ds_->initialize();
     }
     uint64_t requiredGPUMemory() override
     {
@@ -586,6 +643,8 @@ public:
 
     void exec() override
     {
+        ds_->initialize();
+        AllocStreamScope scope(stream_);
         auto &ds = *ds_;
         const int st = storage();
         const int64_t W = ds.vwidth(), H = ds.vheight(), P = ds.width() * ds.height();
@@ -892,7 +951,6 @@ public:
         std::cout << std::endl;
         note("chunks", cj);
         check_memory(requiredGPUMemory(), "method_2");
-        ds_->initialize();
     }
     uint64_t requiredGPUMemory() override
     {
@@ -903,6 +961,8 @@ public:
     }
     void exec() override
     {
+        ds_->initialize();
+        AllocStreamScope scope(stream_);
         auto &ds = *ds_;
         const int st = storage();
         const int64_t W = ds.vwidth(), As = ds.concurrent(), S = p_.recon.sit, dw = ds.width();
@@ -968,6 +1028,7 @@ private:
 // ---------------------------------------------------------------------------
 void Reconstruction::forward_exec()
 {
+    AllocStreamScope scope(stream_);
     auto &ds = *ds_;
     const int64_t W = ds.vwidth(), H = ds.vheight(), As = ds.concurrent(), S = p_.recon.sit;
     const int64_t dw = ds.width(), dh = ds.height();

@@ -23,6 +23,18 @@ struct XprFailure : std::runtime_error {
 };
 void check(int rc, const char *what);
 
+// Stream for the device allocations of the calling thread: while a scope is
+// alive, DevBuf allocates and frees stream-ordered on it (no device-wide
+// synchronisation, so concurrent jobs on other streams keep running).
+class AllocStreamScope {
+public:
+    explicit AllocStreamScope(xpr_stream s);
+    ~AllocStreamScope();
+
+private:
+    xpr_stream prev_;
+};
+
 // Device allocation owned by the host engine.
 class DevBuf {
 public:
@@ -31,7 +43,7 @@ public:
     ~DevBuf();
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DevBuf(DevBuf &&o) noexcept : p_(o.p_), n_(o.n_), s_(o.s_) { o.p_ = nullptr; o.n_ = 0; }
     DevBuf &operator=(DevBuf &&o) noexcept;
     void alloc(size_t bytes);
     void reset();
@@ -42,6 +54,7 @@ public:
 private:
     void *p_ = nullptr;
     size_t n_ = 0;
+    xpr_stream s_ = nullptr;        // stream it was allocated on (stream-ordered)
 };
 
 // Pinned host allocation.
@@ -89,8 +102,9 @@ public:
 
     // Preprocess every projection (normalise, -log) on the device and keep
     // the float32 result in host memory (initializeTempImages,
-    // data_loader.hpp:48-101).  Rows [row0, row0+rows) only when cropped.
-    void initialize(int device_stream_dtype_unused = 0);
+    // data_loader.hpp:48-101).  Idempotent.
+    void initialize();
+    bool quiet() const { return p_.xpr.quiet; }
     // Preprocessed images of one sub-iteration, rows [row0, row0+rows) of each.
     std::vector<float> sit_images(int64_t sit, int64_t row0, int64_t rows) const;
     float norm_min() const { return mn_; }
@@ -115,6 +129,8 @@ private:
     std::vector<std::vector<float>> ram_store_; // layer store when "ram"
     float mn_ = 0.0f, mx_ = 1.0f;
     std::string proj_dir_;
+    bool initialized_ = false;
+    std::vector<float> raw_image(int64_t i) const;
 };
 
 // ---------------------------------------------------------------------------

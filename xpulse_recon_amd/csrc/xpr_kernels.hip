@@ -957,6 +957,25 @@ int xpr_host_alloc(void **hptr, size_t bytes)
     return XPR_OK;
 }
 
+int xpr_alloc_async(void **dptr, size_t bytes, xpr_stream s)
+{
+    if (!dptr) return fail(XPR_E_INVALID, "null output");
+    *dptr = nullptr;
+    hipError_t e = hipMallocAsync(dptr, bytes ? bytes : 1, (hipStream_t)s);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(XPR_E_NOMEM, "hipMallocAsync(" + std::to_string(bytes) + "): " +
+                                     hipGetErrorString(e));
+    }
+    return XPR_OK;
+}
+
+int xpr_free_async(void *dptr, xpr_stream s)
+{
+    XPR_HIP_CHECK(hipFreeAsync(dptr, (hipStream_t)s));
+    return XPR_OK;
+}
+
 int xpr_host_free(void *hptr)
 {
     XPR_HIP_CHECK(hipHostFree(hptr));
