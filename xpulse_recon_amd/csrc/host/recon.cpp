@@ -214,6 +214,8 @@ Dataset::Dataset(Parameters &p) : p_(p)
 
 Dataset::~Dataset()
 {
+    for (auto &e : slot_done_)
+        if (e) xpr_event_destroy(e);
     if (proj_dir_.empty()) return;
     std::error_code ec;
     fs::remove_all(proj_dir_, ec);     // cleanTempImages (data_loader.hpp:107-111)
@@ -245,59 +247,81 @@ std::vector<float> Dataset::sit_projdata(int64_t sit) const
     return out;
 }
 
-void Dataset::initialize()
+const float *Dataset::raw(int64_t i) const
 {
-    if (initialized_) return;
-    initialized_ = true;
-    const int64_t P = dwidth_ * dheight_;
-    images_.assign(size_t(nproj_), {});
-    xpr_stream s;
-    check(xpr_stream_create(&s), "stream");
-    struct StreamGuard {
-        xpr_stream s;
-        ~StreamGuard() { xpr_stream_destroy(s); }
-    } guard{s};
+    if (p_.xpr.mem_projections)
+        return p_.xpr.mem_projections + size_t(i) * size_t(dwidth_ * dheight_);
+    return raw_[size_t(i)].data();
+}
+
+void Dataset::stream_views(xpr_stream s, const std::vector<int64_t> &views, int64_t row0,
+                           int64_t rows, const std::function<void(const float *, size_t, size_t)> &fn)
+{
+    if (row0 < 0 || rows <= 0 || row0 + rows > dheight_)
+        throw std::runtime_error("detector row window outside the projections");
+    const size_t n = size_t(rows * dwidth_);                  // floats per view
+    const size_t per = std::max<size_t>(1, (size_t(16) << 20) / (n * 4));
+    const size_t chunk = std::min(per, views.size());
+    for (auto &h : pin_)
+        if (h.bytes() < chunk * n * 4) h.alloc(chunk * n * 4);
+    for (auto &e : slot_done_)
+        if (!e) check(xpr_event_create(&e), "event");
     AllocStreamScope scope(s);
-    DevBuf d_in(size_t(P) * 4), d_out(size_t(P) * 4), d_mm(8);
-    HostBuf h(size_t(P) * 4);
-    const int threads = io_threads(p_);
-    // read all images (host threads), then preprocess on the device
-    std::vector<std::vector<float>> raw(static_cast<size_t>(nproj_));
-    parallel_for(nproj_, threads, [&](int64_t i) { raw[size_t(i)] = raw_image(i); });
-    if (p_.recon.normalize) {
-        check(xpr_minmax_init(s, d_mm.as<int32_t>()), "minmax init");
-        for (int64_t i = 0; i < nproj_; ++i) {
-            std::memcpy(h.get(), raw[size_t(i)].data(), size_t(P) * 4);
-            check(xpr_upload(s, d_in.get(), h.get(), size_t(P) * 4), "upload");
-            check(xpr_minmax_normal(s, d_in.as<float>(), uint64_t(P), d_mm.as<int32_t>()), "minmax");
-            check(xpr_stream_sync(s), "sync");
-        }
-        int32_t keys[2];
-        check(xpr_download(s, keys, d_mm.get(), 8), "download");
-        check(xpr_stream_sync(s), "sync");
-        check(xpr_minmax_read(keys, &mn_, &mx_), "minmax read");
-    }
-    for (int64_t i = 0; i < nproj_; ++i) {
-        std::memcpy(h.get(), raw[size_t(i)].data(), size_t(P) * 4);
-        check(xpr_upload(s, d_in.get(), h.get(), size_t(P) * 4), "upload");
-        check(xpr_preprocess(s, d_in.as<float>(), d_out.get(), XPR_F32, uint64_t(P),
-                             p_.recon.normalize, p_.recon.mlog, mn_, mx_), "preprocess");
-        check(xpr_download(s, h.get(), d_out.get(), size_t(P) * 4), "download");
-        check(xpr_stream_sync(s), "sync");
-        images_[size_t(i)].assign(h.as<float>(), h.as<float>() + P);
-        raw[size_t(i)].clear();
-        raw[size_t(i)].shrink_to_fit();
+    DevBuf dev[2] = {DevBuf(chunk * n * 4), DevBuf(chunk * n * 4)};
+    for (size_t j0 = 0, k = 0; j0 < views.size(); j0 += chunk, k ^= 1) {
+        const size_t c = std::min(chunk, views.size() - j0);
+        check(xpr_event_sync(slot_done_[k]), "sync");       // slot's previous transfer is done
+        for (size_t j = 0; j < c; ++j)
+            std::memcpy(pin_[k].as<float>() + j * n, raw(views[j0 + j]) + row0 * dwidth_, n * 4);
+        check(xpr_upload(s, dev[k].get(), pin_[k].get(), c * n * 4), "upload");
+        check(xpr_event_record(slot_done_[k], s), "event");
+        fn(dev[k].as<float>(), j0, c);
     }
 }
 
-std::vector<float> Dataset::sit_images(int64_t sit, int64_t row0, int64_t rows) const
+void Dataset::load(xpr_stream s)
 {
-    const auto views = sit_views(sit);
-    std::vector<float> out(size_t(views.size()) * size_t(rows * dwidth_));
-    for (size_t j = 0; j < views.size(); ++j)
-        std::memcpy(out.data() + j * size_t(rows * dwidth_),
-                    images_[size_t(views[j])].data() + row0 * dwidth_, size_t(rows * dwidth_) * 4);
-    return out;
+    if (loaded_) return;
+    loaded_ = true;
+    if (!p_.xpr.mem_projections) {
+        raw_.assign(size_t(nproj_), {});
+        parallel_for(nproj_, io_threads(p_), [&](int64_t i) { raw_[size_t(i)] = raw_image(i); });
+    }
+    if (!p_.recon.normalize) return;
+    const size_t n = size_t(dwidth_ * dheight_);
+    AllocStreamScope scope(s);
+    DevBuf d_mm(8);
+    check(xpr_minmax_init(s, d_mm.as<int32_t>()), "minmax init");
+    std::vector<int64_t> all(static_cast<size_t>(nproj_));
+    for (int64_t i = 0; i < nproj_; ++i) all[size_t(i)] = i;
+    stream_views(s, all, 0, dheight_, [&](const float *d, size_t, size_t c) {
+        check(xpr_minmax_normal(s, d, uint64_t(c * n), d_mm.as<int32_t>()), "minmax");
+    });
+    int32_t keys[2];
+    check(xpr_download(s, keys, d_mm.get(), 8), "download");
+    check(xpr_stream_sync(s), "sync");
+    check(xpr_minmax_read(keys, &mn_, &mx_), "minmax read");
+}
+
+void Dataset::upload_sit_images(xpr_stream s, int64_t sit, int64_t row0, int64_t rows, void *dst,
+                                int dt)
+{
+    load(s);
+    const size_t n = size_t(rows * dwidth_);
+    char *out = static_cast<char *>(dst);
+    stream_views(s, sit_views(sit), row0, rows, [&](const float *d, size_t j0, size_t c) {
+        check(xpr_preprocess(s, d, out + j0 * n * dtype_bytes(dt), dt, uint64_t(c * n),
+                             p_.recon.normalize, p_.recon.mlog, mn_, mx_), "preprocess");
+    });
+}
+
+void Dataset::finish_volume_sink()
+{
+    if (!p_.xpr.mem_volume_out || !p_.recon.normalize) return;
+    const float scale = mx_ - mn_;
+    float *v = p_.xpr.mem_volume_out;
+    const size_t n = size_t(vwidth_ * vwidth_ * vheight_);
+    for (size_t k = 0; k < n; ++k) v[k] = v[k] * scale;
 }
 
 void Dataset::save_layer(const float *data, int64_t layer, bool finalize)
@@ -547,25 +571,46 @@ public:
     }
     void exec() override
     {
-        ds_->initialize();
         AllocStreamScope scope(stream_);
         auto &ds = *ds_;
+        ds.load(stream_);
         const int st = storage();
         const int64_t W = ds.vwidth(), H = ds.vheight(), P = ds.width() * ds.height();
         const int64_t As = ds.concurrent(), S = p_.recon.sit;
         const uint64_t V = uint64_t(W * W * H);
-        DevBuf stage;
+        const bool final32 = st != XPR_F32 && p_.xpr.final_fp32;
+        // device-resident images of every sub-iteration, uploaded and
+        // preprocessed on the copy stream ahead of the sub-iteration that
+        // first reads them (the host staging of sit s+1 overlaps compute of s)
+        DevBuf stage, img32, err32, vol32;
         std::vector<DevBuf> images(static_cast<size_t>(S)), pds(static_cast<size_t>(S));
+        std::vector<std::vector<float>> pd_host(static_cast<size_t>(S));
         for (int64_t s = 0; s < S; ++s) {
-            auto img = ds.sit_images(s, 0, ds.height());
-            images[size_t(s)].alloc(img.size() * dtype_bytes(st));
-            upload_as(stream_, images[size_t(s)], st, img.data(), img.size(), stage);
-            auto pd = ds.sit_projdata(s);
-            pds[size_t(s)].alloc(pd.size() * 4);
-            check(xpr_upload(stream_, pds[size_t(s)].get(), pd.data(), pd.size() * 4), "upload");
+            images[size_t(s)].alloc(uint64_t(As * P) * dtype_bytes(st));
+            pd_host[size_t(s)] = ds.sit_projdata(s);
+            pds[size_t(s)].alloc(pd_host[size_t(s)].size() * 4);
+            check(xpr_upload(stream_, pds[size_t(s)].get(), pd_host[size_t(s)].data(),
+                             pd_host[size_t(s)].size() * 4), "upload");
+        }
+        if (final32) {
+            img32.alloc(uint64_t(As * P) * 4);
+            err32.alloc(uint64_t(As * P) * 4);
+            vol32.alloc(V * 4);
         }
         DevBuf vol(V * dtype_bytes(st)), sums(uint64_t(As * P) * uint64_t(p_.xpr.accumulator / 8)),
             err(uint64_t(As * P) * dtype_bytes(st));
+        Event ready, up32;
+        std::vector<Event> up(static_cast<size_t>(S));
+        check(xpr_event_record(ready.e, stream_), "event");
+        check(xpr_stream_wait_event(copy_, ready.e), "wait");     // buffers allocated
+        auto upload = [&](int64_t s) {
+            ds.upload_sit_images(copy_, s, 0, ds.height(), images[size_t(s)].get(), st);
+            check(xpr_event_record(up[size_t(s)].e, copy_), "event");
+            if (final32 && s == S - 1) {
+                ds.upload_sit_images(copy_, s, 0, ds.height(), img32.get(), XPR_F32);
+                check(xpr_event_record(up32.e, copy_), "event");
+            }
+        };
         Operators ops(stream_, p_.xpr.accumulator);
         ops.setSumImagesBuffer(sums.get());
         ops.setErrorBuffer(err.get(), st);
@@ -576,25 +621,22 @@ public:
         ops.setAngleNumber(As, p_.detector.module_number);
         check(xpr_fill(stream_, vol.get(), st, 1.0f, V), "fill");
         fill_sums(stream_, sums, p_.xpr.accumulator, 1ull << 22, uint64_t(As * P));
-        DevBuf vol32, img32, err32;
+        upload(0);
         int vdt = st;
         Event e0, e1;
         for (int64_t mit = 0; mit < p_.recon.it; ++mit) {
             ops.setWeight(next_weight(mit));
             for (int64_t s = 0; s < S; ++s) {
                 const bool last = mit == p_.recon.it - 1 && s == S - 1;
+                if (mit == 0) check(xpr_stream_wait_event(stream_, up[size_t(s)].e), "wait");
                 check(xpr_event_record(e0.e, stream_), "event");
                 ops.setOrigin(next_origin());
                 ops.setImagesBuffer(images[size_t(s)].get(), st);
                 ops.setProjDataBuffer(pds[size_t(s)].as<float>());
-                if (last && st != XPR_F32 && p_.xpr.final_fp32) {
+                if (last && final32) {
                     // final sub-iteration with float32 storage
-                    vol32.alloc(V * 4);
+                    check(xpr_stream_wait_event(stream_, up32.e), "wait");
                     check(xpr_convert(stream_, vol32.get(), XPR_F32, vol.get(), st, V), "convert");
-                    auto img = ds.sit_images(s, 0, ds.height());
-                    img32.alloc(img.size() * 4);
-                    upload_as(stream_, img32, XPR_F32, img.data(), img.size(), stage);
-                    err32.alloc(uint64_t(As * P) * 4);
                     ops.setVolumeBuffer(vol32.get(), XPR_F32);
                     ops.setImagesBuffer(img32.get(), XPR_F32);
                     ops.setErrorBuffer(err32.get(), XPR_F32);
@@ -605,15 +647,22 @@ public:
                 ops.backward();
                 fill_sums(stream_, sums, p_.xpr.accumulator, 0, uint64_t(As * P));
                 check(xpr_event_record(e1.e, stream_), "event");
+                if (mit == 0 && s + 1 < S) upload(s + 1);
                 check(xpr_event_sync(e1.e), "sync");
                 float ms = 0;
                 check(xpr_event_elapsed_ms(e0.e, e1.e, &ms), "elapsed");
                 log_sit(mit, s, ms);
             }
         }
-        std::vector<float> out(V);
-        download_f32(stream_, out.data(), vdt == XPR_F32 && vol32.get() ? vol32 : vol, vdt, V, stage);
-        ds.save_final_layers(out.data(), 0, H);
+        const DevBuf &result = vdt == XPR_F32 && final32 ? vol32 : vol;
+        if (float *sink = ds.volume_sink()) {
+            download_f32(stream_, sink, result, vdt, V, stage);
+            ds.finish_volume_sink();
+        } else {
+            std::vector<float> out(V);
+            download_f32(stream_, out.data(), result, vdt, V, stage);
+            ds.save_final_layers(out.data(), 0, H);
+        }
     }
 };
 
@@ -643,9 +692,9 @@ public:
 
     void exec() override
     {
-        ds_->initialize();
         AllocStreamScope scope(stream_);
         auto &ds = *ds_;
+        ds.load(stream_);
         const int st = storage();
         const int64_t W = ds.vwidth(), H = ds.vheight(), P = ds.width() * ds.height();
         const int64_t As = ds.concurrent(), S = p_.recon.sit;
@@ -672,8 +721,7 @@ public:
                 const bool last = mit == p_.recon.it - 1 && s == S - 1;
                 check(xpr_event_record(e0.e, stream_), "event");
                 ops.setOrigin(next_origin());
-                auto im = ds.sit_images(s, 0, ds.height());
-                upload_as(stream_, img, st, im.data(), im.size(), stage);
+                ds.upload_sit_images(stream_, s, 0, ds.height(), img.get(), st);
                 auto pd = ds.sit_projdata(s);
                 pdb.alloc(pd.size() * 4);
                 check(xpr_upload(stream_, pdb.get(), pd.data(), pd.size() * 4), "upload");
@@ -961,9 +1009,9 @@ public:
     }
     void exec() override
     {
-        ds_->initialize();
         AllocStreamScope scope(stream_);
         auto &ds = *ds_;
+        ds.load(stream_);
         const int st = storage();
         const int64_t W = ds.vwidth(), As = ds.concurrent(), S = p_.recon.sit, dw = ds.width();
         const size_t layer = size_t(W * W);
@@ -996,8 +1044,7 @@ public:
                 ops.setWeight(next_weight(mit));
                 for (int64_t s = 0; s < S; ++s) {
                     check(xpr_event_record(e0.e, stream_), "event");
-                    auto im = ds.sit_images(s, c.iOffset, c.iSize);
-                    upload_as(stream_, img, st, im.data(), im.size(), stage);
+                    ds.upload_sit_images(stream_, s, c.iOffset, c.iSize, img.get(), st);
                     ops.setOrigin(next_origin());
                     ops.setProjDataBuffer(pds[size_t(s)].as<float>());
                     if (s > 0 || mit > 0) ops.forward();

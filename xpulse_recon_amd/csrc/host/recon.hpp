@@ -6,6 +6,7 @@
 #pragma once
 
 #include <array>
+#include <functional>
 #include <cstdint>
 #include <memory>
 #include <random>
@@ -100,13 +101,22 @@ public:
     std::vector<float> sit_projdata(int64_t sit) const;
     std::vector<int64_t> sit_views(int64_t sit) const;
 
-    // Preprocess every projection (normalise, -log) on the device and keep
-    // the float32 result in host memory (initializeTempImages,
-    // data_loader.hpp:48-101).  Idempotent.
-    void initialize();
+    // Raw projections in host memory (TIFF sources are read once by the IO
+    // threads; in-process jobs use the caller's buffer) and, when normalising,
+    // the global min/max over isnormal samples, reduced on the device
+    // (initializeTempImages, data_loader.hpp:48-101).  Idempotent.
+    void load(xpr_stream s);
     bool quiet() const { return p_.xpr.quiet; }
-    // Preprocessed images of one sub-iteration, rows [row0, row0+rows) of each.
-    std::vector<float> sit_images(int64_t sit, int64_t row0, int64_t rows) const;
+    // Rows [row0, row0+rows) of the views of one sub-iteration, preprocessed
+    // (normalise, -log) on the device into dst [view][row][col] of dtype dt.
+    // Uploads go through two pinned staging slots so the host copy of one
+    // chunk overlaps the transfer of the other; everything is ordered on s.
+    void upload_sit_images(xpr_stream s, int64_t sit, int64_t row0, int64_t rows, void *dst, int dt);
+    // The caller's volume buffer of an in-process job ([vheight][vwidth][vwidth]),
+    // or null; after writing it, finish_volume_sink() applies the
+    // de-normalisation of saveLayer(finalize) (data_loader.hpp:135-139).
+    float *volume_sink() const { return p_.xpr.mem_volume_out; }
+    void finish_volume_sink();
     float norm_min() const { return mn_; }
     float norm_max() const { return mx_; }
 
@@ -125,12 +135,19 @@ private:
     float vx_ = 0.0f;
     std::array<float, 3> orig_{};
     std::vector<float> pd_;
-    std::vector<std::vector<float>> images_;    // preprocessed, one per projection
+    std::vector<std::vector<float>> raw_;       // raw projections of TIFF sources
+    HostBuf pin_[2];                            // pinned staging slots
+    xpr_event slot_done_[2] = {nullptr, nullptr};
     std::vector<std::vector<float>> ram_store_; // layer store when "ram"
     float mn_ = 0.0f, mx_ = 1.0f;
     std::string proj_dir_;
-    bool initialized_ = false;
+    bool loaded_ = false;
     std::vector<float> raw_image(int64_t i) const;
+    const float *raw(int64_t i) const;
+    // stream the rows of `views` through the staging slots into device
+    // memory; fn(device rows, first view, views in chunk) runs after each upload
+    void stream_views(xpr_stream s, const std::vector<int64_t> &views, int64_t row0, int64_t rows,
+                      const std::function<void(const float *, size_t, size_t)> &fn);
 };
 
 // ---------------------------------------------------------------------------
