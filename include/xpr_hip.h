@@ -212,6 +212,13 @@ typedef struct {
 int xpr_synth_line_integrals(xpr_stream s, const double *frames, uint32_t n_views,
                              uint32_t modules, uint32_t det_width, uint32_t det_height,
                              const xpr_ellipsoid *ellipsoids, int n_ellipsoids, double *out);
+/* The phantom on the reconstruction grid: out[l][k][i] (device float, vwidth^2
+ * * vheight) = sum of mu over the ellipsoids (device array) that contain the
+ * centre of voxel (i, l, k), orig + (i, l, k)*vx + vx/2 (ground truth for
+ * reconstruction-error figures). */
+int xpr_synth_phantom(xpr_stream s, double vx, int64_t vwidth, int64_t vheight,
+                      const double orig[3], const xpr_ellipsoid *ellipsoids, int n_ellipsoids,
+                      float *out);
 
 /* ---- the XPRecon drop-in (src/main.cpp:62-137) ----------------------------- */
 

@@ -329,3 +329,34 @@ def reconstruct_naive(images: np.ndarray, projdata: np.ndarray, grid: dict, it: 
             sums[...] = 0
         wgt = np.float32(wgt * np.float32(weight_factor))
     return vol
+
+
+def reconstruct_naive_f64(images: np.ndarray, projdata64: np.ndarray, grid: dict, it: int,
+                          sit: int, weight: float, weight_factor: float, jitter,
+                          modules: int = 1):
+    """FP64 model of the naive schedule (reconstruction_n.hpp:51-78): the same
+    voxel-driven NN operators in double precision with real-valued ray sums
+    (no Q10.22 quantisation), FP64 ProjData and the same jitter offsets.
+    The accuracy reference for storage-precision studies."""
+    n, h, w = images.shape
+    W, Hv = grid["vwidth"], grid["vheight"]
+    a_s = n // sit
+    vol = np.ones((Hv, W, W), np.float64)
+    sums = np.ones((a_s, h, w), np.float64)
+    wgt = float(weight)
+    vx = grid["vx"]
+    for mit in range(it):
+        for s in range(sit):
+            views = sit_views(n, sit, s)
+            pd = np.ascontiguousarray(
+                np.concatenate([projdata64[v * modules:(v + 1) * modules] for v in views]))
+            img = np.ascontiguousarray(images[views], np.float64)
+            o = [grid["orig"][i] + float(jitter[mit * sit + s][i]) for i in range(3)]
+            g = make_geom(w, a_s, modules, (0, 0, 0), (W, Hv, W), 0, h, o, (vx, vx, vx), fp64=True)
+            if s > 0 or mit > 0:
+                forward_f64(vol, sums, pd, g)
+            err = error_f64(img, sums, wgt)
+            backward_f64(vol, err, pd, g)
+            sums[...] = 0.0
+        wgt = wgt * weight_factor
+    return vol

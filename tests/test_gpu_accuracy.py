@@ -1,0 +1,105 @@
+"""Storage precision against the FP64 model: reconstructions with FP32, FP16
+and BF16 storage (FP32 arithmetic, final FP32 sub-iteration) through the
+XPRecon drop-in, against the oracle's FP64 naive schedule fed the same
+preprocessed images, jitter and ProjData matrices (the float32 matrices the
+device uses, so that only arithmetic and storage differ; FP64-built matrices
+move nearest-neighbour decisions and differ from either by ~1%), and against
+the phantom.
+
+Scan: 64^3 grid from a 64x64 detector, 90 views over 360 deg, sit=10 (9 views
+per sub-iteration), it=1..3, 12-ellipsoid phantom, Poisson I0 = 1e4.
+
+Reported per iteration k: RMSE of each reconstruction against the FP64 model
+after k iterations (numerical error of the storage) and against the phantom
+mu*vx (reconstruction error; storage error is a small part of it)."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.tiff_util import read_f32, write_f32
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "xpulse_recon_amd", "bin", "XPRecon")
+W = H = 64
+N, SIT, IT = 90, 10, 3
+
+
+def _rmse(a, b):
+    return float(np.sqrt(np.mean((np.asarray(a, np.float64) - np.asarray(b, np.float64)) ** 2)))
+
+
+@pytest.fixture(scope="module")
+def study(tmp_path_factory, cuda):
+    import torch
+    from xpulse_recon_amd import synth
+    t = tmp_path_factory.mktemp("acc")
+    spec = O.GeometrySpec(dwidth=W, dheight=H, projections=N)
+    pd32, grid = O.geometry(spec, False)
+    pd64 = pd32.astype(np.float64)
+    ell = synth.random_ellipsoids(1001, 12, W * grid["vx"], 128)
+    oe = [O.make_ellipsoid(e["c"], e["a"], e["rot"], e["mu"]) for e in ell]
+    truth = O.phantom_voxels(grid, oe) * grid["vx"]
+    rng = np.random.default_rng(1002)
+    d = t / "in"
+    d.mkdir()
+    raw = np.empty((N, H, W), np.float32)
+    for a in range(N):
+        p = O.line_integrals(spec, a, oe)
+        raw[a] = (rng.poisson(1e4 * np.exp(-p)) / 1e4).astype(np.float32)
+        write_f32(d / f"proj_{a:04d}.tif", raw[a])
+    vols, jit = {}, None
+    for storage in ("f32", "f16", "bf16"):
+        for it in range(1, IT + 1):
+            out = t / f"{storage}_{it}"
+            log = t / f"{storage}_{it}.json"
+            params = {"recon": {"input": str(d), "output": str(out), "it": it, "sit": SIT,
+                                "method": "naive"},
+                      "xpr": {"storage": storage, "jitter_seed": 21, "run_log": str(log)}}
+            pf = t / f"{storage}_{it}_params.json"
+            pf.write_text(json.dumps(params))
+            print(f"XPRecon storage={storage} it={it}", flush=True)
+            r = subprocess.run([CLI, str(pf)], capture_output=True, text=True, timeout=300)
+            assert r.returncode == 0, r.stdout + r.stderr
+            vols[storage, it] = np.stack([read_f32(f) for f in sorted(out.glob("layer_*.tif"))])
+            if storage == "f32" and it == IT:
+                jit = np.array(json.loads(log.read_text())["jitter"], np.float64)
+    imgs = raw.copy()
+    O.preprocess(imgs.reshape(-1), False, True)
+    ref = {}
+    print("FP64 oracle", flush=True)
+    for it in range(1, IT + 1):
+        ref[it] = O.reconstruct_naive_f64(imgs.astype(np.float64), pd64, grid, it, SIT, 1.0, 0.9,
+                                          jit[:it * SIT])
+    del torch
+    return vols, ref, truth
+
+
+def test_storage_precision_table(study):
+    vols, ref, truth = study
+    rows = []
+    for it in range(1, IT + 1):
+        scale = float(np.sqrt(np.mean(ref[it] ** 2)))
+        row = {"it": it}
+        for s in ("f32", "f16", "bf16"):
+            row[f"{s}_vs_fp64_rel"] = _rmse(vols[s, it], ref[it]) / scale
+            row[f"{s}_vs_truth"] = _rmse(vols[s, it], truth)
+        row["fp64_vs_truth"] = _rmse(ref[it], truth)
+        rows.append(row)
+        print(json.dumps({k: (round(v, 8) if isinstance(v, float) else v) for k, v in row.items()}),
+              flush=True)
+    for row in rows:
+        # FP32 storage: the FP64 model up to Q10.22 truncation and float rounding
+        assert row["f32_vs_fp64_rel"] < 1e-4
+        # 16-bit storage: numerical error well below the reconstruction error
+        assert row["f16_vs_fp64_rel"] < 2e-2
+        assert row["bf16_vs_fp64_rel"] < 4e-2
+        # reconstruction error against the phantom: 16-bit storage within 1%
+        # (f16) / 2% (bf16) of the FP32 figure
+        assert row["f16_vs_truth"] <= 1.01 * row["f32_vs_truth"]
+        assert row["bf16_vs_truth"] <= 1.02 * row["f32_vs_truth"]

@@ -261,6 +261,19 @@ def test_synthetic_line_integrals_match_oracle(cuda):
     assert got.max() > 0.1
 
 
+def test_synthetic_phantom_matches_oracle(cuda):
+    from xpulse_recon_amd import synth
+    from xpulse_recon_amd.geometry import Geometry
+    geom = Geometry(dwidth=40, dheight=24, projections=4, heli_offset=0.05)
+    _, grid = geom.projdata()
+    ell = synth.random_ellipsoids(8, 9, grid["vwidth"] * grid["vx"], 40)
+    got = synth.phantom(grid, ell, cuda).cpu().numpy()
+    oe = [O.make_ellipsoid(e["c"], e["a"], e["rot"], e["mu"]) for e in ell]
+    ref = O.phantom_voxels(grid, oe).astype(np.float32)
+    assert got.shape == ref.shape and (ref > 0).mean() > 0.05
+    np.testing.assert_array_equal(got, ref)
+
+
 @pytest.mark.parametrize("cap", ["0", "40", "2048"])
 @pytest.mark.parametrize("acc", [32, 64])
 def test_forward_paths_agree_with_large_values(cuda, monkeypatch, cap, acc):

@@ -240,3 +240,61 @@ def test_fp64_model_agrees_with_fp32_canonical():
     O.forward_f64(vol.astype(np.float64), s64, pd64, oracle_geom(d64, fp64=True))
     rel = np.linalg.norm(s32 / Q - s64) / np.linalg.norm(s64)
     assert rel < 2e-2          # only NN flips at rounding boundaries differ
+
+
+def test_fp64_schedule_single_pass_closed_form():
+    """it=1, sit=1: one Backward with got = 1, so every voxel seen by at least
+    one view becomes exp(mean log(c)) = c for constant images c; voxels no view
+    reaches keep 1."""
+    spec = small_spec(20, 16, 8)
+    pd64, grid = O.geometry(spec, True)
+    c = 0.37
+    images = np.full((8, 16, 20), c, np.float64)
+    vol = O.reconstruct_naive_f64(images, pd64, grid, 1, 1, 1.0, 0.9, np.zeros((1, 3)))
+    seen = vol != 1.0
+    assert seen.mean() > 0.5
+    np.testing.assert_allclose(vol[seen], c, rtol=1e-12)
+
+
+def test_fp64_schedule_tracks_fp32_canonical():
+    """The FP64 model of the naive schedule against the FP32 canonical one on
+    the same (float32-valued) matrices and jitter: they differ by float
+    rounding, Q10.22 truncation and rare NN flips only."""
+    spec = small_spec(24, 16, 12)
+    pd32, grid = O.geometry(spec, False)
+    rng = np.random.default_rng(5)
+    images = rng.uniform(0.05, 0.6, (12, 16, 24)).astype(np.float32)
+    jit = rng.uniform(0.0, grid["vx"], (6, 3)).astype(np.float32)
+    v32 = O.reconstruct_naive(images, pd32, grid, 2, 3, 1.0, 0.9, jit)
+    v64 = O.reconstruct_naive_f64(images.astype(np.float64), pd32.astype(np.float64), grid, 2, 3,
+                                  1.0, 0.9, jit.astype(np.float64))
+    rel = np.linalg.norm(v32 - v64) / np.linalg.norm(v64)
+    assert rel < 1e-4
+    assert np.max(np.abs(v32 / v64 - 1.0)) < 1e-4
+
+
+def test_phantom_voxels_quadratic_form():
+    """Phantom sampling against the quadratic form x^T R^T diag(a^-2) R x <= 1."""
+    spec = small_spec(24, 20, 4)
+    _, grid = O.geometry(spec, False)
+    rng = np.random.default_rng(9)
+    ells = []
+    for mu in (0.3, 0.7):
+        q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        c = rng.uniform(-0.05, 0.05, 3)
+        a = rng.uniform(0.04, 0.1, 3)
+        ells.append((c, a, q, mu))
+    got = O.phantom_voxels(grid, [O.make_ellipsoid(c, a, q, mu) for c, a, q, mu in ells])
+    W, Hv, vx = grid["vwidth"], grid["vheight"], grid["vx"]
+    l, k, i = np.meshgrid(np.arange(Hv), np.arange(W), np.arange(W), indexing="ij")
+    pts = np.stack([grid["orig"][0] + (i + 0.5) * vx, grid["orig"][1] + (l + 0.5) * vx,
+                    grid["orig"][2] + (k + 0.5) * vx], -1)
+    want = np.zeros_like(got)
+    for c, a, q, mu in ells:
+        M = q.T @ np.diag(1.0 / a ** 2) @ q
+        x = pts - c
+        r2 = np.einsum("...i,ij,...j->...", x, M, x)
+        want += np.where(r2 <= 1.0, mu, 0.0)
+    assert (got > 0).mean() > 0.01
+    # points within rounding of the surface may fall either side
+    assert np.mean(got != want) < 1e-3

@@ -146,6 +146,8 @@ _SIGS = {
     "xpr_server_wait": (C.c_int, [_P, C.c_uint64, C.POINTER(JobStats)]),
     "xpr_job_device_bytes": (C.c_int, [C.POINTER(Job), C.POINTER(C.c_uint64)]),
     "xpr_server_destroy": (C.c_int, [_P]),
+    "xpr_synth_phantom": (C.c_int, [_P, C.c_double, C.c_int64, C.c_int64,
+                                    C.POINTER(C.c_double), _P, C.c_int, _P]),
     "xpr_synth_line_integrals": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, C.c_uint32,
                                            C.c_uint32, _P, C.c_int, _P]),
 }

@@ -71,7 +71,55 @@ __global__ __launch_bounds__(256) void k_line_integrals(
     }
 }
 
+// Sum of the attenuations of the ellipsoids containing each voxel centre
+// (voxel (i, l, k) centred at orig + (i, l, k) + 0.5 voxels), float32 result.
+__global__ __launch_bounds__(256) void k_phantom(double vx, int64_t vwidth, int64_t vheight,
+                                                 double ox, double oy, double oz,
+                                                 const xpr_ellipsoid *__restrict__ ell, int ne,
+                                                 float *__restrict__ out)
+{
+    const uint64_t total = (uint64_t)vwidth * vwidth * vheight;
+    for (uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; id < total;
+         id += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = id % vwidth, k = (id / vwidth) % vwidth, l = id / ((uint64_t)vwidth * vwidth);
+        const double p[3] = {ox + ((double)i + 0.5) * vx, oy + ((double)l + 0.5) * vx,
+                             oz + ((double)k + 0.5) * vx};
+        double s = 0.0;
+        for (int j = 0; j < ne; ++j) {
+            const xpr_ellipsoid &e = ell[j];
+            const double q0[3] = {p[0] - e.c[0], p[1] - e.c[1], p[2] - e.c[2]};
+            double r2 = 0.0;
+            for (int r = 0; r < 3; ++r) {
+                const double b = (e.rot[3 * r] * q0[0] + e.rot[3 * r + 1] * q0[1] +
+                                  e.rot[3 * r + 2] * q0[2]) / e.a[r];
+                r2 = r2 + b * b;
+            }
+            if (r2 <= 1.0) s += e.mu;
+        }
+        out[id] = (float)s;
+    }
+}
+
 } // namespace
+
+extern "C" int xpr_synth_phantom(xpr_stream s, double vx, int64_t vwidth, int64_t vheight,
+                                 const double orig[3], const xpr_ellipsoid *ellipsoids,
+                                 int n_ellipsoids, float *out)
+{
+    if (!out || !orig || (n_ellipsoids > 0 && !ellipsoids))
+        return xprint_fail(XPR_E_INVALID, "null buffer");
+    if (vwidth <= 0 || vheight <= 0 || n_ellipsoids < 0 || n_ellipsoids > kMaxEllipsoids)
+        return xprint_fail(XPR_E_INVALID, "synth: bad grid or ellipsoid count");
+    const uint64_t total = (uint64_t)vwidth * vwidth * vheight;
+    uint64_t blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_phantom, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)s, vx, vwidth,
+                       vheight, orig[0], orig[1], orig[2], ellipsoids, n_ellipsoids, out);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return xprint_fail(XPR_E_HIP, (std::string("synth launch: ") + hipGetErrorString(e)).c_str());
+    return XPR_OK;
+}
 
 extern "C" int xpr_synth_line_integrals(xpr_stream s, const double *frames, uint32_t n_views,
                                         uint32_t modules, uint32_t det_width,

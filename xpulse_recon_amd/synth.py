@@ -78,6 +78,20 @@ def line_integrals(geom: Geometry, views, ellipsoids, device="cuda", stream=None
     return out
 
 
+def phantom(grid: dict, ellipsoids, device="cuda", stream=None):
+    """Phantom attenuation (1/mm) sampled at the voxel centres of the grid,
+    float32 [vheight, vwidth, vwidth] (torch, on device)."""
+    import torch
+    W, H = grid["vwidth"], grid["vheight"]
+    te = torch.from_numpy(_pack(ellipsoids)).to(device)
+    out = torch.empty((H, W, W), dtype=torch.float32, device=device)
+    orig = (C.c_double * 3)(*grid["orig"])
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream if stream is None else stream)
+    _lib.call("xpr_synth_phantom", s, float(grid["vx"]), W, H, orig, C.c_void_p(te.data_ptr()),
+              len(ellipsoids), C.c_void_p(out.data_ptr()))
+    return out
+
+
 def transmission(p, i0: float, seed: int):
     """Poisson counts N ~ P(I0 exp(-p)), returned as transmission N/I0 (float32)."""
     import torch
