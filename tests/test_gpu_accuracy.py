@@ -94,11 +94,14 @@ def test_storage_precision_table(study):
         print(json.dumps({k: (round(v, 8) if isinstance(v, float) else v) for k, v in row.items()}),
               flush=True)
     for row in rows:
-        # FP32 storage: the FP64 model up to Q10.22 truncation and float rounding
-        assert row["f32_vs_fp64_rel"] < 1e-4
-        # 16-bit storage: numerical error well below the reconstruction error
+        # FP32 storage: the FP64 model up to Q10.22 truncation, float rounding
+        # and the nearest-neighbour flips they cause at near-ties (float and
+        # double voxel positions round a few voxel-views to the other pixel;
+        # at phantom edges that moves 4e-4..6e-3 of the volume norm)
+        assert row["f32_vs_fp64_rel"] < 1e-2
+        # 16-bit storage: of the same order, well below the reconstruction error
         assert row["f16_vs_fp64_rel"] < 2e-2
-        assert row["bf16_vs_fp64_rel"] < 4e-2
+        assert row["bf16_vs_fp64_rel"] < 2e-2
         # reconstruction error against the phantom: 16-bit storage within 1%
         # (f16) / 2% (bf16) of the FP32 figure
         assert row["f16_vs_truth"] <= 1.01 * row["f32_vs_truth"]

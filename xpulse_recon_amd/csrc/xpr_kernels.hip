@@ -350,19 +350,33 @@ __device__ __forceinline__ uint64_t q22_u64(float v)
 }
 
 // ---------------------------------------------------------------------------
-// Forward with LDS binning.  A workgroup owns a brick of 64*JX (x) x 4
-// (layers, one per wave) x BK (depths); lane l holds the columns
-// x0 + l + 64 j.  Per view the brick's 8 corners bound its detector footprint
-// (one pixel of margin covers float rounding); voxel-views accumulate there
-// with LDS u64 atomics and the non-zero footprint pixels go to the detector
-// sums with one global atomic each.  Two tiles alternate: the flush of view v
-// also clears its tile, so one barrier per view separates splat and flush.
-// Footprints above the tile capacity, corners behind the source and bricks
-// holding Q.22 values above 2^32 take the direct path (one global atomic per
-// voxel-view).  Every path adds the same integers: the sums are bit-identical
-// whatever the path, the tiling or the launch order.
+// Forward with LDS binning.  A workgroup owns a brick of 64 (x) x 4 (layers,
+// one per wave) x 16 (depths).  Per view the brick's 8 corners bound its
+// detector footprint (one pixel of margin covers float rounding); voxel-views
+// accumulate there with LDS u64 atomics and the non-zero footprint pixels go
+// to the detector sums with one global atomic each.  Two tiles alternate: the
+// flush of view v also clears its tile, so one barrier per view separates
+// splat and flush.  Footprints above the tile capacity and corners behind the
+// source take the direct path (one global atomic per voxel-view); bricks
+// holding Q.22 values above 2^32 take it with 64-bit values.  Every path adds
+// the same integers: the sums are bit-identical whatever the path, the lane
+// order or the launch order.
+//
+// Lane order per view.  Voxels along x project to neighbouring detector
+// columns when the ray is across x, but onto one or two columns when the ray
+// runs along x (views near 90 and 270 degrees): 64 lanes of one x-row would
+// then hit the same LDS entries in one atomic.  Each wave therefore holds its
+// 1024 voxels twice, as 64 x-lanes by 16 depths and as 16 depth-lanes x 4
+// x-groups by 16 x-steps, and a view uses the order whose lanes spread more
+// across columns.  The per-(layer, depth) part of the dot products is
+// computed once per view by the depth-lanes; x-order reads it with readlane.
+//
+// The footprint, bounds thresholds and order of a chunk of 32 views are
+// computed once per workgroup (8 corner lanes per view) into LDS.
 // ---------------------------------------------------------------------------
 constexpr int kTileCap = 1536;
+constexpr int kBK = 16;          // depths per brick
+constexpr int kViewChunk = 32;   // views per footprint pass (256 threads / 8 corners)
 
 struct BrickShape {
     int32_t nx, nl, nk;      // slab extent (x, layers, depths)
@@ -378,29 +392,47 @@ __device__ __forceinline__ void global_add(AccT *p, AccT v)
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-struct Footprint {
-    int32_t u0, v0, tw, th;
-    bool ok;       // corners in front of the source, finite
-    bool miss;     // the brick cannot reach the view's bounds
-    bool fast;     // every voxel's W is inside the fast reciprocal's range
+enum : uint32_t {
+    kBinSkip = 1u,     // no voxel of the brick lands inside the view's bounds
+    kBinTile = 2u,     // splat into an LDS tile (else direct global atomics)
+    kBinFast = 4u,     // every voxel's W is inside the fast reciprocal's range
+    kBinDepth = 8u,    // depth-lane order
 };
 
+struct ViewBin {                   // one (view, module) record of the brick, in LDS
+    int32_t u0, v0, tw, th;        // tile origin and extent on the detector
+    float ulo, uhi, vlo, vhi;      // in-bounds thresholds (view_thresholds)
+    uint32_t flags;
+    uint32_t pad[3];
+};
+
+// Footprint of the brick for view record m; run by 8 consecutive lanes (one
+// per corner: bit 0 x, bit 1 layer, bit 2 depth), written by the first.
 template <bool CROP>
-__device__ __forceinline__ Footprint footprint(const float *__restrict__ p, float cpx, float cpy,
-                                               float cpz, float line_off, const xpr_proj_args &a)
+__device__ __forceinline__ void bin_view(const float *__restrict__ pd, uint32_t m, bool live,
+                                         int corner, const int32_t (&lo)[3],
+                                         const int32_t (&hi)[3], int cap,
+                                         const xpr_proj_args &a, ViewBin *__restrict__ out)
 {
-    // lanes form identical groups of 8 corners; every lane ends with the result
+    const float *__restrict__ p = pd + 20u * m;
+    const float cpx = __builtin_fmaf((float)(((corner & 1) ? hi[0] : lo[0]) + a.vmin[0]), a.voxel[0], a.origin[0]);
+    const float cpy = __builtin_fmaf((float)(((corner & 2) ? hi[1] : lo[1]) + a.vmin[1]), a.voxel[1], a.origin[1]);
+    const float cpz = __builtin_fmaf((float)(((corner & 4) ? hi[2] : lo[2]) + a.vmin[2]), a.voxel[2], a.origin[2]);
+    const float line_off = (float)a.line_offset;
     const float X = __builtin_fmaf(p[0], cpx, __builtin_fmaf(p[1], cpy, __builtin_fmaf(p[2], cpz, p[3])));
     const float Y = __builtin_fmaf(p[4], cpx, __builtin_fmaf(p[5], cpy, __builtin_fmaf(p[6], cpz, p[7])));
     const float Wc = __builtin_fmaf(p[12], cpx, __builtin_fmaf(p[13], cpy, __builtin_fmaf(p[14], cpz, p[15])));
     const float r = 1.0f / Wc;
-    float umin = X * r, vmin = CROP ? Y * r - line_off : Y * r;
-    float umax = umin, vmax = vmin;
-    int ok = (Wc > 0.0f) & (umin == umin) & (vmin == vmin) & (fabsf(umin) < 1e7f) &
-             (fabsf(vmin) < 1e7f);
+    const float u = X * r, v = CROP ? Y * r - line_off : Y * r;
+    float umin = u, vmin = v, umax = u, vmax = v;
+    int ok = (Wc > 0.0f) & (u == u) & (v == v) & (fabsf(u) < 1e7f) & (fabsf(v) < 1e7f);
     // every voxel W of the brick lies between the corner values (fast_rcp_box)
     const float tmag = fabsf(p[12] * cpx) + fabsf(p[13] * cpy) + fabsf(p[14] * cpz) + fabsf(p[15]);
     int fast = fast_rcp_box(Wc, tmag);
+    // column spread per voxel step along x (corner 1) and along depth (corner 4)
+    const int base = (int)(threadIdx.x & 63) & ~7;     // lane of corner 0
+    const float du_x = fabsf(__shfl(u, base + 1) - __shfl(u, base)) / (float)max(hi[0] - lo[0], 1);
+    const float du_k = fabsf(__shfl(u, base + 4) - __shfl(u, base)) / (float)max(hi[2] - lo[2], 1);
 #pragma unroll
     for (int off = 1; off < 8; off <<= 1) {
         umin = fminf(umin, __shfl_xor(umin, off));
@@ -410,19 +442,30 @@ __device__ __forceinline__ Footprint footprint(const float *__restrict__ p, floa
         ok &= __shfl_xor(ok, off);
         fast &= __shfl_xor(fast, off);
     }
-    Footprint f{0, 0, 0, 0, ok != 0, false, fast != 0};
-    if (f.ok) {
-        const float vpl = fmaxf(p[16], 0.0f), vpr = fminf(p[17], (float)a.det_width);
-        const float vlo = fmaxf(p[18], 0.0f), vhi = fminf(p[19], (float)a.line_rows);
-        f.u0 = (int32_t)fmaxf(floorf(umin) - 1.0f, vpl);
-        f.v0 = (int32_t)fmaxf(floorf(vmin) - 1.0f, vlo);
+    if (!live || corner != 0) return;
+    ViewBin b{};
+    uint32_t flags = fast ? kBinFast : 0u;
+    if (ok && du_k > du_x) flags |= kBinDepth;
+    const float vpl = fmaxf(p[16], 0.0f), vpr = fminf(p[17], (float)a.det_width);
+    const float vlo = fmaxf(p[18], 0.0f), vhi = fminf(p[19], (float)a.line_rows);
+    b.ulo = lo_threshold(vpl);
+    b.uhi = hi_threshold(vpr);
+    b.vlo = lo_threshold(vlo);
+    b.vhi = hi_threshold(vhi);
+    if (ok) {
+        b.u0 = (int32_t)fmaxf(floorf(umin) - 1.0f, vpl);
+        b.v0 = (int32_t)fmaxf(floorf(vmin) - 1.0f, vlo);
         const int32_t u1 = (int32_t)fminf(ceilf(umax) + 1.0f, vpr - 1.0f);
         const int32_t v1 = (int32_t)fminf(ceilf(vmax) + 1.0f, vhi - 1.0f);
-        f.tw = u1 - f.u0 + 1;
-        f.th = v1 - f.v0 + 1;
-        f.miss = f.tw <= 0 || f.th <= 0;
+        b.tw = u1 - b.u0 + 1;
+        b.th = v1 - b.v0 + 1;
+        if (b.tw <= 0 || b.th <= 0)
+            flags |= kBinSkip;
+        else if (b.tw * b.th <= cap)
+            flags |= kBinTile;
     }
-    return f;
+    b.flags = flags;
+    *out = b;
 }
 
 // Workgroup barrier that orders LDS only.  The tile protocol needs no
@@ -433,11 +476,13 @@ __device__ __forceinline__ void lds_barrier()
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-template <typename VS, typename AccT, int JX, int BK, bool CROP>
+// one brick with some Q.22 value of 2^32 or more: one global atomic per
+// voxel-view with the exact 64-bit value (x-lane order)
+template <typename VS, typename AccT, bool CROP>
 __device__ __noinline__ void forward_direct_u64(const typename VS::T *__restrict__ vol,
                                                 AccT *__restrict__ sums,
                                                 const float *__restrict__ pd,
-                                                const xpr_proj_args &a, const BrickShape &s,
+                                                const xpr_proj_args a, const BrickShape s,
                                                 int32_t x0, int32_t l0, int32_t k0)
 {
     const int lane = threadIdx.x & 63;
@@ -446,72 +491,70 @@ __device__ __noinline__ void forward_direct_u64(const typename VS::T *__restrict
     const float py = __builtin_fmaf((float)(l + a.vmin[1]), a.voxel[1], a.origin[1]);
     const float line_off = (float)a.line_offset;
     const uint32_t view_elems = a.det_width * (uint32_t)a.line_rows;
-    for (int j = 0; j < JX; ++j) {
-        const int32_t x = x0 + lane + 64 * j;
-        const float px = __builtin_fmaf((float)(x + a.vmin[0]), a.voxel[0], a.origin[0]);
-        for (int i = 0; i < BK; ++i) {
-            const int32_t k = k0 + i;
-            if (x >= s.nx || k >= s.nk) continue;
-            const float v = VS::load(vol, ((uint64_t)l * s.nk + k) * s.nx + x);
-            const AccT q = sizeof(AccT) == 8 ? (AccT)q22_u64(v) : (AccT)q22_u32(v);
-            if (q == 0) continue;
-            const float pz = __builtin_fmaf((float)(k + a.vmin[2]), a.voxel[2], a.origin[2]);
-            for (uint32_t view = 0, m = 0; view < a.n_views; ++view) {
-                AccT *vsum = sums + (uint64_t)view_of(a, view) * view_elems;
-                for (uint32_t mm = 0; mm < a.modules; ++mm, ++m) {
-                    const float *__restrict__ p = pd + 20u * m;
-                    ViewConsts c;
-                    view_thresholds(p, a, c);
-                    row_terms(p, py, pz, c);
-                    uint32_t iu, iv;
-                    if (project<CROP, false>(c, px, line_off, iu, iv))
-                        global_add(vsum + (__umul24(iv, a.det_width) + iu), q);
-                }
+    const int32_t x = x0 + lane;
+    const float px = __builtin_fmaf((float)(x + a.vmin[0]), a.voxel[0], a.origin[0]);
+    for (int i = 0; i < kBK; ++i) {
+        const int32_t k = k0 + i;
+        if (x >= s.nx || k >= s.nk) continue;
+        const float v = VS::load(vol, ((uint64_t)l * s.nk + k) * s.nx + x);
+        const AccT q = sizeof(AccT) == 8 ? (AccT)q22_u64(v) : (AccT)q22_u32(v);
+        if (q == 0) continue;
+        const float pz = __builtin_fmaf((float)(k + a.vmin[2]), a.voxel[2], a.origin[2]);
+        for (uint32_t view = 0, m = 0; view < a.n_views; ++view) {
+            AccT *vsum = sums + (uint64_t)view_of(a, view) * view_elems;
+            for (uint32_t mm = 0; mm < a.modules; ++mm, ++m) {
+                const float *__restrict__ p = pd + 20u * m;
+                ViewConsts c;
+                view_thresholds(p, a, c);
+                row_terms(p, py, pz, c);
+                uint32_t iu, iv;
+                if (project<CROP, false>(c, px, line_off, iu, iv))
+                    global_add(vsum + (__umul24(iv, a.det_width) + iu), q);
             }
         }
     }
 }
 
-// splat of one view over the brick: every voxel-view adds its Q.22 value to
-// its footprint tile entry (voxel-views out of bounds add 0 to the lane's
-// sink slot); one off the tile goes straight to the detector sums
-template <bool FAST, bool CROP, typename AccT, int JX, int BK>
-__device__ __forceinline__ void splat_brick(ViewConsts &c, const float *__restrict__ p,
-                                            const uint32_t (&q)[JX][BK], const float (&px)[JX],
-                                            float py, int32_t k0, float line_off, const Footprint &f,
-                                            bool use_tile, unsigned long long *__restrict__ tl,
-                                            unsigned long long *__restrict__ sink,
-                                            AccT *__restrict__ vsum, const xpr_proj_args &a)
+// one voxel-view of a splat: its Q.22 value goes to its footprint tile entry
+// (voxel-views out of bounds add 0 to the lane's sink slot); one off the tile
+// goes straight to the detector sums.  Entries are counted from the tile
+// base; sink_e is the sink slot's entry relative to it.
+template <bool CROP, bool FAST, typename AccT>
+__device__ __forceinline__ void splat_one(const ViewConsts &c, float px, uint32_t qq,
+                                          float line_off, int32_t u0, int32_t v0, int32_t tw,
+                                          int32_t th, bool use_tile,
+                                          unsigned long long *__restrict__ tl, uint32_t sink_e,
+                                          AccT *__restrict__ vsum, uint32_t det_width)
 {
-#pragma unroll
-    for (int i = 0; i < BK; ++i) {
-        const float pz = __builtin_fmaf((float)(k0 + i + a.vmin[2]), a.voxel[2], a.origin[2]);
-        row_terms(p, py, pz, c);
-#pragma unroll
-        for (int j = 0; j < JX; ++j) {
-            const uint32_t qq = q[j][i];
-            uint32_t iu, iv;
-            const bool inb = project<CROP, FAST>(c, px[j], line_off, iu, iv);
-            const uint32_t tu = iu - (uint32_t)f.u0;
-            const uint32_t tv = iv - (uint32_t)f.v0;
-            const bool in_tile = use_tile & (tu < (uint32_t)f.tw) & (tv < (uint32_t)f.th);
-            unsigned long long *slot = (inb & in_tile) ? &tl[tv * f.tw + tu] : &sink[threadIdx.x];
-            atomicAdd(slot, (unsigned long long)((inb & in_tile) ? qq : 0u));
-            if (inb & !in_tile & (qq != 0u))   // off the tile: direct global add
-                global_add(vsum + (__umul24(iv, a.det_width) + iu), (AccT)qq);
-        }
-    }
+    uint32_t iu, iv;
+    const bool inb = project<CROP, FAST>(c, px, line_off, iu, iv);
+    const uint32_t tu = iu - (uint32_t)u0;
+    const uint32_t tv = iv - (uint32_t)v0;
+    const bool hit = inb & use_tile & (tu < (uint32_t)tw) & (tv < (uint32_t)th);
+    const uint32_t e = __umul24(tv, (uint32_t)tw) + tu;
+    atomicAdd(&tl[hit ? e : sink_e], (unsigned long long)(hit ? qq : 0u));
+    if (inb & !hit & (qq != 0u))      // off the tile: direct global add
+        global_add(vsum + (__umul24(iv, det_width) + iu), (AccT)qq);
 }
 
-template <typename VS, typename AccT, int JX, int BK, bool CROP>
-__global__ __launch_bounds__(kThreads) void k_forward_lds(
+// a uniform float copied to a VGPR, so that an FMA taking it and a value read
+// from one lane (an SGPR) needs no further copy
+__device__ __forceinline__ float to_vgpr(float x)
+{
+    float v;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "s"(x));
+    return v;
+}
+
+template <typename VS, typename AccT, bool CROP>
+__global__ __launch_bounds__(kThreads, 4) void k_forward_bin(
     const typename VS::T *__restrict__ vol, AccT *__restrict__ sums,
     const float *__restrict__ pd, xpr_proj_args a, BrickShape s)
 {
-    // two tiles, then one private slot per lane: voxel-views outside the
-    // footprint add 0 to their lane's slot, so the splat needs no branch
-    __shared__ unsigned long long tile[2][kTileCap];
-    __shared__ unsigned long long sink[kThreads];
+    // two tiles, one private slot per lane (voxel-views outside the footprint
+    // add 0 there, so the splat needs no branch) and the view records
+    __shared__ unsigned long long lds[2 * kTileCap + kThreads];
+    __shared__ ViewBin bins[kViewChunk];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     int64_t b = s.brick_base + blockIdx.x;
@@ -520,82 +563,141 @@ __global__ __launch_bounds__(kThreads) void k_forward_lds(
     b /= s.nbx;
     const int32_t bk = (int32_t)(b % s.nbk);
     const int32_t bl = (int32_t)(b / s.nbk);
-    const int32_t x0 = bx * 64 * JX, l0 = bl * kWaves, k0 = bk * BK;
+    const int32_t x0 = bx * 64, l0 = bl * kWaves, k0 = bk * kBK;
     const int32_t l = l0 + wave;
+    const int32_t dk = lane & 15, xg = lane >> 4;     // depth-lane order
 
-    // Q.22 voxel values (32 bits suffice unless some value is >= 1024)
-    uint32_t q[JX][BK];
+    // Q.22 voxel values in both lane orders (32 bits suffice unless some
+    // value is >= 1024)
+    uint32_t qx[kBK], qd[kBK];
     bool big = false;
 #pragma unroll
-    for (int j = 0; j < JX; ++j) {
-        const int32_t x = x0 + lane + 64 * j;
-#pragma unroll
-        for (int i = 0; i < BK; ++i) {
-            const int32_t k = k0 + i;
-            uint32_t qq = 0;
-            if (x < s.nx && l < s.nl && k < s.nk) {
-                const float v = VS::load(vol, ((uint64_t)l * s.nk + k) * s.nx + x);
-                if (sizeof(AccT) == 8) {
-                    const uint64_t q64 = q22_u64(v);
-                    big |= (q64 >> 32) != 0;
-                    qq = (uint32_t)q64;
-                } else {
-                    qq = q22_u32(v);
-                }
+    for (int i = 0; i < kBK; ++i) {
+        const int32_t x = x0 + lane, k = k0 + i;
+        uint32_t q = 0;
+        if (x < s.nx && l < s.nl && k < s.nk) {
+            const float v = VS::load(vol, ((uint64_t)l * s.nk + k) * s.nx + x);
+            if (sizeof(AccT) == 8) {
+                const uint64_t q64 = q22_u64(v);
+                big |= (q64 >> 32) != 0;
+                q = (uint32_t)q64;
+            } else {
+                q = q22_u32(v);
             }
-            q[j][i] = qq;
         }
+        qx[i] = q;
+    }
+#pragma unroll
+    for (int j = 0; j < kBK; ++j) {
+        const int32_t x = x0 + xg * 16 + j, k = k0 + dk;
+        uint32_t q = 0;
+        if (x < s.nx && l < s.nl && k < s.nk) {
+            const float v = VS::load(vol, ((uint64_t)l * s.nk + k) * s.nx + x);
+            q = sizeof(AccT) == 8 ? (uint32_t)q22_u64(v) : q22_u32(v);
+        }
+        qd[j] = q;
     }
     if (__syncthreads_or(big)) {
-        // some value needs more than 32 bits: one global atomic per voxel-view
-        // with the exact 64-bit value (values >= 1024 do not occur in practice)
-        forward_direct_u64<VS, AccT, JX, BK, CROP>(vol, sums, pd, a, s, x0, l0, k0);
+        forward_direct_u64<VS, AccT, CROP>(vol, sums, pd, a, s, x0, l0, k0);
         return;
     }
-    float px[JX];
+    const float px_x = __builtin_fmaf((float)(x0 + lane + a.vmin[0]), a.voxel[0], a.origin[0]);
+    float px_d[kBK];
 #pragma unroll
-    for (int j = 0; j < JX; ++j)
-        px[j] = __builtin_fmaf((float)(x0 + lane + 64 * j + a.vmin[0]), a.voxel[0], a.origin[0]);
+    for (int j = 0; j < kBK; ++j)
+        px_d[j] = __builtin_fmaf((float)(x0 + xg * 16 + j + a.vmin[0]), a.voxel[0], a.origin[0]);
     const float py = __builtin_fmaf((float)(l + a.vmin[1]), a.voxel[1], a.origin[1]);
+    const float pz_d = __builtin_fmaf((float)(k0 + dk + a.vmin[2]), a.voxel[2], a.origin[2]);
 
-    // this lane's brick corner
-    const int32_t xe = min(x0 + 64 * JX - 1, s.nx - 1), le = min(l0 + kWaves - 1, s.nl - 1),
-                  ke = min(k0 + BK - 1, s.nk - 1);
-    const float cpx = __builtin_fmaf((float)(((lane & 1) ? xe : x0) + a.vmin[0]), a.voxel[0], a.origin[0]);
-    const float cpy = __builtin_fmaf((float)(((lane & 2) ? le : l0) + a.vmin[1]), a.voxel[1], a.origin[1]);
-    const float cpz = __builtin_fmaf((float)(((lane & 4) ? ke : k0) + a.vmin[2]), a.voxel[2], a.origin[2]);
+    // brick corners (voxel indices inside the slab)
+    const int32_t lo[3] = {x0, l0, k0};
+    const int32_t hi[3] = {min(x0 + 63, s.nx - 1), min(l0 + kWaves - 1, s.nl - 1),
+                           min(k0 + kBK - 1, s.nk - 1)};
     const float line_off = (float)a.line_offset;
     const uint32_t view_elems = a.det_width * (uint32_t)a.line_rows;
+    const uint32_t nm = a.n_views * a.modules;
 
-    for (int t = threadIdx.x; t < 2 * kTileCap; t += kThreads) (&tile[0][0])[t] = 0ull;
-    __syncthreads();
+    for (int t = threadIdx.x; t < 2 * kTileCap; t += kThreads) lds[t] = 0ull;
     int buf = 0;
-    for (uint32_t view = 0, m = 0; view < a.n_views; ++view) {
-        AccT *__restrict__ vsum = sums + (uint64_t)view_of(a, view) * view_elems;
-        for (uint32_t mm = 0; mm < a.modules; ++mm, ++m) {
+    for (uint32_t c0 = 0; c0 < nm; c0 += kViewChunk) {
+        const uint32_t cnt = min((uint32_t)kViewChunk, nm - c0);
+        __syncthreads();                              // previous chunk's records read
+        {
+            const uint32_t vi = threadIdx.x >> 3;
+            const bool live = vi < cnt;
+            bin_view<CROP>(pd, live ? c0 + vi : c0, live, threadIdx.x & 7, lo, hi, s.cap, a,
+                           &bins[vi]);
+        }
+        __syncthreads();
+        for (uint32_t vi = 0; vi < cnt; ++vi) {
+            const uint32_t m = c0 + vi;
+            const uint32_t flags = __builtin_amdgcn_readfirstlane(bins[vi].flags);
+            if (flags & kBinSkip) continue;
+            const int32_t u0 = __builtin_amdgcn_readfirstlane(bins[vi].u0);
+            const int32_t v0 = __builtin_amdgcn_readfirstlane(bins[vi].v0);
+            const int32_t tw = __builtin_amdgcn_readfirstlane(bins[vi].tw);
+            const int32_t th = __builtin_amdgcn_readfirstlane(bins[vi].th);
+            const bool use_tile = (flags & kBinTile) != 0;
             const float *__restrict__ p = pd + 20u * m;
-            const Footprint f = footprint<CROP>(p, cpx, cpy, cpz, line_off, a);
-            if (f.ok && f.miss) continue;                  // uniform: no voxel lands in bounds
-            const bool use_tile = f.ok && f.tw * f.th <= s.cap;
-            unsigned long long *__restrict__ tl = tile[buf];
+            AccT *__restrict__ vsum = sums + (uint64_t)view_of(a, m / a.modules) * view_elems;
+            unsigned long long *__restrict__ tl = lds + buf * kTileCap;
+            const uint32_t sk = (uint32_t)((2 - buf) * kTileCap) + threadIdx.x;
+            // this lane's (layer, depth) row terms: depth k0 + dk
             ViewConsts c;
-            view_thresholds(p, a, c);
-            if (f.fast)
-                splat_brick<true, CROP, AccT, JX, BK>(c, p, q, px, py, k0, line_off, f, use_tile, tl,
-                                                      sink, vsum, a);
-            else
-                splat_brick<false, CROP, AccT, JX, BK>(c, p, q, px, py, k0, line_off, f, use_tile, tl,
-                                                       sink, vsum, a);
+            c.ulo = bins[vi].ulo;
+            c.uhi = bins[vi].uhi;
+            c.vlo = bins[vi].vlo;
+            c.vhi = bins[vi].vhi;
+            row_terms(p, py, pz_d, c);
+            const uint32_t dw = a.det_width;
+            if (flags & kBinDepth) {
+                if (flags & kBinFast) {
+#pragma unroll
+                    for (int j = 0; j < kBK; ++j)
+                        splat_one<CROP, true, AccT>(c, px_d[j], qd[j], line_off, u0, v0, tw, th,
+                                                    use_tile, tl, sk, vsum, dw);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < kBK; ++j)
+                        splat_one<CROP, false, AccT>(c, px_d[j], qd[j], line_off, u0, v0, tw, th,
+                                                     use_tile, tl, sk, vsum, dw);
+                }
+            } else {
+                // x-lane order: the row terms of depth k0 + i sit in lane i
+                ViewConsts ci = c;
+                ci.c0x = to_vgpr(c.c0x);
+                ci.c1x = to_vgpr(c.c1x);
+                ci.c3x = to_vgpr(c.c3x);
+                if (flags & kBinFast) {
+#pragma unroll
+                    for (int i = 0; i < kBK; ++i) {
+                        ci.k0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.k0), i));
+                        ci.k1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.k1), i));
+                        ci.k3 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.k3), i));
+                        splat_one<CROP, true, AccT>(ci, px_x, qx[i], line_off, u0, v0, tw, th,
+                                                    use_tile, tl, sk, vsum, dw);
+                    }
+                } else {
+#pragma unroll
+                    for (int i = 0; i < kBK; ++i) {
+                        ci.k0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.k0), i));
+                        ci.k1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.k1), i));
+                        ci.k3 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.k3), i));
+                        splat_one<CROP, false, AccT>(ci, px_x, qx[i], line_off, u0, v0, tw, th,
+                                                     use_tile, tl, sk, vsum, dw);
+                    }
+                }
+            }
             if (use_tile) {
                 lds_barrier();                // splat of this view complete
                 // flush and clear; the next view splats into the other tile
-                for (int rr = wave; rr < f.th; rr += kWaves) {
-                    AccT *dst = vsum + (uint32_t)(f.v0 + rr) * a.det_width + (uint32_t)f.u0;
-                    for (int cc = lane; cc < f.tw; cc += 64) {
-                        const unsigned long long t = tl[rr * f.tw + cc];
+                for (int rr = wave; rr < th; rr += kWaves) {
+                    AccT *dst = vsum + (uint32_t)(v0 + rr) * dw + (uint32_t)u0;
+                    for (int cc = lane; cc < tw; cc += 64) {
+                        const unsigned long long t = tl[rr * tw + cc];
                         if (t != 0ull) {
                             global_add(dst + cc, (AccT)t);
-                            tl[rr * f.tw + cc] = 0ull;
+                            tl[rr * tw + cc] = 0ull;
                         }
                     }
                 }
@@ -788,30 +890,16 @@ int forward_tile_cap()
     return c < 0 ? 0 : (c > kTileCap ? kTileCap : c);
 }
 
-// brick shapes compiled: 64*JX columns x 4 layers x BK depths
-enum class Brick { b2x16, b1x32, b2x32, b1x16 };
-
-Brick forward_brick()
-{
-    // tuning knob: brick shape "JXxBK"
-    const char *e = std::getenv("XPR_FORWARD_BRICK");
-    const std::string v = e ? e : "1x16";
-    if (v == "1x32") return Brick::b1x32;
-    if (v == "2x32") return Brick::b2x32;
-    if (v == "2x16") return Brick::b2x16;
-    return Brick::b1x16;
-}
-
-template <typename VS, typename AccT, int JX, int BK>
-int forward_launch(hipStream_t st, const void *vol, void *sums, const float *pd,
-                   const xpr_proj_args &a, SlabShape s)
+template <typename VS, typename AccT>
+int forward_impl(hipStream_t st, const void *vol, void *sums, const float *pd,
+                 const xpr_proj_args &a, SlabShape s)
 {
     BrickShape b;
     b.nx = s.nx;
     b.nl = s.nl;
     b.nk = s.nk;
-    b.nbx = (s.nx + 64 * JX - 1) / (64 * JX);
-    b.nbk = (s.nk + BK - 1) / BK;
+    b.nbx = (s.nx + 63) / 64;
+    b.nbk = (s.nk + kBK - 1) / kBK;
     const int64_t nbl = (s.nl + kWaves - 1) / kWaves;
     b.nbricks = (int64_t)b.nbx * b.nbk * nbl;
     b.cap = forward_tile_cap();
@@ -822,28 +910,16 @@ int forward_launch(hipStream_t st, const void *vol, void *sums, const float *pd,
         t.brick_base = b0;
         const int64_t blocks = std::min<int64_t>(max_blocks, b.nbricks - b0);
         if (crop)
-            hipLaunchKernelGGL((k_forward_lds<VS, AccT, JX, BK, true>), dim3((unsigned)blocks),
+            hipLaunchKernelGGL((k_forward_bin<VS, AccT, true>), dim3((unsigned)blocks),
                                dim3(kThreads), 0, st, (const typename VS::T *)vol, (AccT *)sums,
                                pd, a, t);
         else
-            hipLaunchKernelGGL((k_forward_lds<VS, AccT, JX, BK, false>), dim3((unsigned)blocks),
+            hipLaunchKernelGGL((k_forward_bin<VS, AccT, false>), dim3((unsigned)blocks),
                                dim3(kThreads), 0, st, (const typename VS::T *)vol, (AccT *)sums,
                                pd, a, t);
         XPR_LAUNCH_CHECK();
     }
     return XPR_OK;
-}
-
-template <typename VS, typename AccT>
-int forward_impl(hipStream_t st, const void *vol, void *sums, const float *pd,
-                 const xpr_proj_args &a, SlabShape s)
-{
-    switch (forward_brick()) {
-    case Brick::b1x32: return forward_launch<VS, AccT, 1, 32>(st, vol, sums, pd, a, s);
-    case Brick::b2x32: return forward_launch<VS, AccT, 2, 32>(st, vol, sums, pd, a, s);
-    case Brick::b1x16: return forward_launch<VS, AccT, 1, 16>(st, vol, sums, pd, a, s);
-    default: return forward_launch<VS, AccT, 2, 16>(st, vol, sums, pd, a, s);
-    }
 }
 
 template <typename IS, typename AccT>
