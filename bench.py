@@ -369,7 +369,7 @@ def main():
     if bp_ms >= fp_ms:
         name, launch_ms, nbytes = "k_backward", bp_launch, bp_bytes
     else:
-        name, launch_ms, nbytes = "k_forward", fp_launch, fp_bytes
+        name, launch_ms, nbytes = "k_forward_bin", fp_launch, fp_bytes
     ach = nbytes / (launch_ms * 1e-3) / 1e9
     traffic = measured_traffic(name, args) if args.dtype == "f32" else None
     roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
