@@ -25,6 +25,7 @@
 #include <string>
 
 #include "../../include/xpr_hip.h"
+#include "batch.hpp"
 
 namespace {
 
@@ -244,6 +245,7 @@ __device__ __forceinline__ float load_or_zero<BF16>(__amdgpu_buffer_rsrc_t r, ui
 // ---------------------------------------------------------------------------
 constexpr int kWaves = 4;
 constexpr int kThreads = 64 * kWaves;
+constexpr int kBwdJ = 8;          // voxels per lane of the backward
 
 struct SlabShape {
     int32_t nx, nl, nk;
@@ -252,16 +254,20 @@ struct SlabShape {
     int64_t nrows;         // rows in the slab
 };
 
+// One unit of the backward: segment unit % nseg of the kWaves rows of row
+// block unit / nseg (one row per wave).
 template <typename VS, typename ES, int J, bool CROP>
-__global__ __launch_bounds__(kThreads) void k_backward(
-    typename VS::T *__restrict__ vol, const typename ES::T *__restrict__ err,
-    const float *__restrict__ pd, xpr_proj_args a, SlabShape s,
-    unsigned long long *__restrict__ empty_count)
+__device__ __forceinline__ void backward_unit(typename VS::T *__restrict__ vol,
+                                              const typename ES::T *__restrict__ err,
+                                              const float *__restrict__ pd,
+                                              const xpr_proj_args &a, const SlabShape &s,
+                                              int64_t unit,
+                                              unsigned long long *__restrict__ empty_count)
 {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int seg = blockIdx.x % s.nseg;
-    const int64_t row = s.row_base + (int64_t)(blockIdx.x / s.nseg) * kWaves + wave;
+    const int seg = (int)(unit % s.nseg);
+    const int64_t row = s.row_base + (unit / s.nseg) * kWaves + wave;
     if (row >= s.nrows) return;                       // whole wave leaves together
     const int32_t l = (int32_t)(row / s.nk), k = (int32_t)(row % s.nk);
     const float py = __builtin_fmaf((float)(l + a.vmin[1]), a.voxel[1], a.origin[1]);
@@ -331,6 +337,15 @@ __global__ __launch_bounds__(kThreads) void k_backward(
         for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off);
         if (lane == 0 && e) atomicAdd(empty_count, (unsigned long long)e);
     }
+}
+
+template <typename VS, typename ES, int J, bool CROP>
+__global__ __launch_bounds__(kThreads) void k_backward(
+    typename VS::T *__restrict__ vol, const typename ES::T *__restrict__ err,
+    const float *__restrict__ pd, xpr_proj_args a, SlabShape s,
+    unsigned long long *__restrict__ empty_count)
+{
+    backward_unit<VS, ES, J, CROP>(vol, err, pd, a, s, blockIdx.x, empty_count);
 }
 
 // FLOAT_TO_FIXED (constants.cl:22), saturating like the GPU conversion.
@@ -546,19 +561,19 @@ __device__ __forceinline__ float to_vgpr(float x)
     return v;
 }
 
+// One brick of the forward (b < s.nbricks, uniform over the workgroup).  lds
+// holds two tiles and one sink slot per lane ([2*kTileCap + kThreads]); the
+// tiles are zero on entry and left zero.
 template <typename VS, typename AccT, bool CROP>
-__global__ __launch_bounds__(kThreads, 4) void k_forward_bin(
-    const typename VS::T *__restrict__ vol, AccT *__restrict__ sums,
-    const float *__restrict__ pd, xpr_proj_args a, BrickShape s)
+__device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__ vol,
+                                              AccT *__restrict__ sums,
+                                              const float *__restrict__ pd,
+                                              const xpr_proj_args &a, const BrickShape &s,
+                                              int64_t b, unsigned long long *__restrict__ lds,
+                                              ViewBin *__restrict__ bins)
 {
-    // two tiles, one private slot per lane (voxel-views outside the footprint
-    // add 0 there, so the splat needs no branch) and the view records
-    __shared__ unsigned long long lds[2 * kTileCap + kThreads];
-    __shared__ ViewBin bins[kViewChunk];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    int64_t b = s.brick_base + blockIdx.x;
-    if (b >= s.nbricks) return;                       // uniform over the block
     const int32_t bx = (int32_t)(b % s.nbx);
     b /= s.nbx;
     const int32_t bk = (int32_t)(b % s.nbk);
@@ -617,7 +632,6 @@ __global__ __launch_bounds__(kThreads, 4) void k_forward_bin(
     const uint32_t view_elems = a.det_width * (uint32_t)a.line_rows;
     const uint32_t nm = a.n_views * a.modules;
 
-    for (int t = threadIdx.x; t < 2 * kTileCap; t += kThreads) lds[t] = 0ull;
     int buf = 0;
     for (uint32_t c0 = 0; c0 < nm; c0 += kViewChunk) {
         const uint32_t cnt = min((uint32_t)kViewChunk, nm - c0);
@@ -705,6 +719,21 @@ __global__ __launch_bounds__(kThreads, 4) void k_forward_bin(
             }
         }
     }
+}
+
+template <typename VS, typename AccT, bool CROP>
+__global__ __launch_bounds__(kThreads, 4) void k_forward_bin(
+    const typename VS::T *__restrict__ vol, AccT *__restrict__ sums,
+    const float *__restrict__ pd, xpr_proj_args a, BrickShape s)
+{
+    // two tiles, one private slot per lane (voxel-views outside the footprint
+    // add 0 there, so the splat needs no branch) and the view records
+    __shared__ unsigned long long lds[2 * kTileCap + kThreads];
+    __shared__ ViewBin bins[kViewChunk];
+    const int64_t b = s.brick_base + blockIdx.x;
+    if (b >= s.nbricks) return;                       // uniform over the block
+    for (int t = threadIdx.x; t < 2 * kTileCap; t += kThreads) lds[t] = 0ull;
+    forward_brick<VS, AccT, CROP>(vol, sums, pd, a, s, b, lds, bins);
 }
 
 // ---------------------------------------------------------------------------
@@ -803,6 +832,176 @@ __global__ __launch_bounds__(256) void k_preprocess(const float *__restrict__ in
 }
 
 // ---------------------------------------------------------------------------
+// Batched steps: persistent kernels over a table of work items of several
+// reconstructions (one kernel per phase, units taken from an atomic counter).
+// Every unit is the same arithmetic as the corresponding single-job launch, so
+// a job's result does not depend on the other items of its steps.
+// ---------------------------------------------------------------------------
+using xpr_batch::Item;
+using xpr_batch::Table;
+
+// next work unit of the workgroup (uniform); the barrier before the fetch
+// also ends the previous unit's use of shared memory
+__device__ __forceinline__ uint32_t next_unit(uint32_t *counter, uint32_t *slot)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) *slot = atomicAdd(counter, 1u);
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(*slot);
+}
+
+// item of phase ph holding unit u (items are in prefix order)
+__device__ __forceinline__ uint32_t item_of(const Table *__restrict__ t, uint32_t ph, uint32_t u)
+{
+    uint32_t i = 0;
+    const uint32_t n = t->count[ph];
+    while (i + 1 < n && t->items[ph][i + 1].first <= u) ++i;
+    return __builtin_amdgcn_readfirstlane(i);
+}
+
+__device__ __forceinline__ BrickShape batch_brick_shape(const xpr_proj_args &a)
+{
+    BrickShape s;
+    s.nx = a.vmax[0] - a.vmin[0];
+    s.nl = a.vmax[1] - a.vmin[1];
+    s.nk = a.vmax[2] - a.vmin[2];
+    s.nbx = (s.nx + 63) / 64;
+    s.nbk = (s.nk + kBK - 1) / kBK;
+    s.cap = kTileCap;
+    s.brick_base = 0;
+    s.nbricks = (int64_t)s.nbx * s.nbk * ((s.nl + kWaves - 1) / kWaves);
+    return s;
+}
+
+__device__ __forceinline__ SlabShape batch_slab_shape(const xpr_proj_args &a, int J)
+{
+    SlabShape s;
+    s.nx = a.vmax[0] - a.vmin[0];
+    s.nl = a.vmax[1] - a.vmin[1];
+    s.nk = a.vmax[2] - a.vmin[2];
+    s.nseg = (s.nx + 64 * J - 1) / (64 * J);
+    s.row_base = 0;
+    s.nrows = (int64_t)s.nl * s.nk;
+    return s;
+}
+
+__global__ __launch_bounds__(kThreads) void k_batch_reset(uint32_t *counters)
+{
+    if (threadIdx.x < (unsigned)xpr_batch::kPhases) counters[threadIdx.x] = 0u;
+}
+
+__global__ __launch_bounds__(kThreads, 4) void k_batch_forward(const Table *__restrict__ t,
+                                                               uint32_t *counters)
+{
+    __shared__ unsigned long long lds[2 * kTileCap + kThreads];
+    __shared__ ViewBin bins[kViewChunk];
+    __shared__ uint32_t slot;
+    const uint32_t ph = xpr_batch::kForward;
+    const uint32_t total = t->total[ph];
+    if (total == 0) return;
+    for (int i = threadIdx.x; i < 2 * kTileCap; i += kThreads) lds[i] = 0ull;
+    for (;;) {
+        const uint32_t u = next_unit(&counters[ph], &slot);
+        if (u >= total) break;                              // uniform: every workgroup ends here
+        const Item &it = t->items[ph][item_of(t, ph, u)];
+        const BrickShape s = batch_brick_shape(it.a);
+        const int64_t b = (int64_t)(u - it.first);
+        uint64_t *sums = (uint64_t *)it.sums;
+        switch (it.dt_dst) {
+        case XPR_F32:
+            forward_brick<F32, uint64_t, false>((const float *)it.dst, sums, it.pd, it.a, s, b, lds, bins);
+            break;
+        case XPR_F16:
+            forward_brick<F16, uint64_t, false>((const uint16_t *)it.dst, sums, it.pd, it.a, s, b, lds, bins);
+            break;
+        default:
+            forward_brick<BF16, uint64_t, false>((const uint16_t *)it.dst, sums, it.pd, it.a, s, b, lds, bins);
+            break;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_batch_backward(const Table *__restrict__ t,
+                                                             uint32_t *counters)
+{
+    __shared__ uint32_t slot;
+    const uint32_t ph = xpr_batch::kBackward;
+    const uint32_t total = t->total[ph];
+    if (total == 0) return;
+    for (;;) {
+        const uint32_t u = next_unit(&counters[ph], &slot);
+        if (u >= total) break;
+        const Item &it = t->items[ph][item_of(t, ph, u)];
+        const SlabShape s = batch_slab_shape(it.a, kBwdJ);
+        const int64_t unit = (int64_t)(u - it.first);
+        switch (it.dt_dst) {                                // volume and error share the dtype
+        case XPR_F32:
+            backward_unit<F32, F32, kBwdJ, false>((float *)it.dst, (const float *)it.err, it.pd, it.a, s,
+                                                  unit, nullptr);
+            break;
+        case XPR_F16:
+            backward_unit<F16, F16, kBwdJ, false>((uint16_t *)it.dst, (const uint16_t *)it.err, it.pd,
+                                                  it.a, s, unit, nullptr);
+            break;
+        default:
+            backward_unit<BF16, BF16, kBwdJ, false>((uint16_t *)it.dst, (const uint16_t *)it.err, it.pd,
+                                                    it.a, s, unit, nullptr);
+            break;
+        }
+    }
+}
+
+template <typename IS, typename ES>
+__device__ __forceinline__ void error_range(const Item &it, uint64_t i0, uint64_t i1)
+{
+    const typename IS::T *img = (const typename IS::T *)it.src;
+    const uint64_t *sums = (const uint64_t *)it.sums;
+    typename ES::T *err = (typename ES::T *)it.dst;
+    for (uint64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
+        const float ref = IS::load(img, i);
+        const float got = (float)sums[i] / kQ22One;                 // FIXED_TO_FLOAT
+        ES::store(err, i, logf(fmaxf(ref / fmaxf(got, kQ22Eps), kQ22Eps)) * it.weight);
+    }
+}
+
+template <typename SS>
+__device__ __forceinline__ void convert_range(const Item &it, uint64_t i0, uint64_t i1)
+{
+    const typename SS::T *src = (const typename SS::T *)it.src;
+    float *dst = (float *)it.dst;
+    for (uint64_t i = i0 + threadIdx.x; i < i1; i += kThreads) dst[i] = SS::load(src, i);
+}
+
+// convert (to float32), error or fill items, one chunk of kElemUnit elements per unit
+__global__ __launch_bounds__(kThreads) void k_batch_elementwise(const Table *__restrict__ t,
+                                                                uint32_t *counters, uint32_t ph)
+{
+    __shared__ uint32_t slot;
+    const uint32_t total = t->total[ph];
+    if (total == 0) return;
+    for (;;) {
+        const uint32_t u = next_unit(&counters[ph], &slot);
+        if (u >= total) break;
+        const Item &it = t->items[ph][item_of(t, ph, u)];
+        const uint64_t i0 = (uint64_t)(u - it.first) * xpr_batch::kElemUnit;
+        const uint64_t i1 = min(it.n, i0 + xpr_batch::kElemUnit);
+        if (ph == xpr_batch::kFill) {
+            uint64_t *dst = (uint64_t *)it.dst;
+            for (uint64_t i = i0 + threadIdx.x; i < i1; i += kThreads) dst[i] = it.value;
+        } else if (ph == xpr_batch::kConvert) {
+            if (it.dt_src == XPR_F16) convert_range<F16>(it, i0, i1);
+            else if (it.dt_src == XPR_BF16) convert_range<BF16>(it, i0, i1);
+            else convert_range<F32>(it, i0, i1);
+        } else {                                            // error: image and error dtypes
+            if (it.dt_src == XPR_F32 && it.dt_dst == XPR_F32) error_range<F32, F32>(it, i0, i1);
+            else if (it.dt_src == XPR_F16 && it.dt_dst == XPR_F16) error_range<F16, F16>(it, i0, i1);
+            else if (it.dt_src == XPR_BF16 && it.dt_dst == XPR_BF16) error_range<BF16, BF16>(it, i0, i1);
+            else error_range<F32, F32>(it, i0, i1);     // other pairs are rejected by the host
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
 inline unsigned grid_for(uint64_t n, unsigned block = 256)
@@ -860,8 +1059,6 @@ int launch_rows(SlabShape s, int J, Launch &&launch)
     }
     return XPR_OK;
 }
-
-constexpr int kBwdJ = 8;
 
 template <typename VS, typename ES>
 int backward_impl(hipStream_t st, void *vol, const void *err, const float *pd,
@@ -979,6 +1176,126 @@ int convert_from(hipStream_t st, void *dst, const void *src, int src_dtype, uint
 } // namespace
 
 int xprint_fail(int code, const char *msg) { return fail(code, msg); }
+
+// ===========================================================================
+// Batched step engine (library-internal, used by the job server)
+// ===========================================================================
+namespace xpr_batch {
+
+uint32_t forward_units(const xpr_proj_args &a)
+{
+    const int64_t nx = a.vmax[0] - a.vmin[0], nl = a.vmax[1] - a.vmin[1], nk = a.vmax[2] - a.vmin[2];
+    return (uint32_t)(((nx + 63) / 64) * ((nk + kBK - 1) / kBK) * ((nl + kWaves - 1) / kWaves));
+}
+
+uint32_t backward_units(const xpr_proj_args &a)
+{
+    const int64_t nx = a.vmax[0] - a.vmin[0], nl = a.vmax[1] - a.vmin[1], nk = a.vmax[2] - a.vmin[2];
+    return (uint32_t)(((nl * nk + kWaves - 1) / kWaves) * ((nx + 64 * kBwdJ - 1) / (64 * kBwdJ)));
+}
+
+struct Engine {
+    hipStream_t stream = nullptr;
+    Table *host[2] = {nullptr, nullptr};
+    Table *dev[2] = {nullptr, nullptr};
+    uint32_t *counters[2] = {nullptr, nullptr};
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    hipEvent_t start[2] = {nullptr, nullptr}, stop[2] = {nullptr, nullptr};
+    bool pending[2] = {false, false};
+};
+
+namespace {
+
+int capture(Engine &e, int k, int cus)
+{
+    XPR_HIP_CHECK(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
+    const dim3 blk(kThreads);
+    (void)hipMemcpyAsync(e.dev[k], e.host[k], sizeof(Table), hipMemcpyHostToDevice, e.stream);
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), blk, 0, e.stream, e.counters[k]);
+    hipLaunchKernelGGL(k_batch_elementwise, dim3(cus * 4), blk, 0, e.stream, e.dev[k], e.counters[k],
+                       (uint32_t)kConvert);
+    hipLaunchKernelGGL(k_batch_forward, dim3(cus * 4), blk, 0, e.stream, e.dev[k], e.counters[k]);
+    hipLaunchKernelGGL(k_batch_elementwise, dim3(cus * 4), blk, 0, e.stream, e.dev[k], e.counters[k],
+                       (uint32_t)kError);
+    hipLaunchKernelGGL(k_batch_backward, dim3(cus * 6), blk, 0, e.stream, e.dev[k], e.counters[k]);
+    hipLaunchKernelGGL(k_batch_elementwise, dim3(cus * 4), blk, 0, e.stream, e.dev[k], e.counters[k],
+                       (uint32_t)kFill);
+    const hipError_t le = hipGetLastError();
+    const hipError_t ce = hipStreamEndCapture(e.stream, &e.graph[k]);
+    if (le != hipSuccess) return fail(XPR_E_HIP, std::string("batch capture: ") + hipGetErrorString(le));
+    if (ce != hipSuccess) return fail(XPR_E_HIP, std::string("batch capture: ") + hipGetErrorString(ce));
+    XPR_HIP_CHECK(hipGraphInstantiate(&e.exec[k], e.graph[k], nullptr, nullptr, 0));
+    return XPR_OK;
+}
+
+} // namespace
+
+int engine_create(Engine **out)
+{
+    auto *e = new Engine();
+    auto bad = [&](int rc) { engine_destroy(e); return rc; };
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return bad(fail(XPR_E_HIP, "batch engine: device query"));
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
+        return bad(fail(XPR_E_HIP, "batch engine: stream"));
+    for (int k = 0; k < 2; ++k) {
+        if (hipHostMalloc((void **)&e->host[k], sizeof(Table), hipHostMallocDefault) != hipSuccess ||
+            hipMalloc((void **)&e->dev[k], sizeof(Table)) != hipSuccess ||
+            hipMalloc((void **)&e->counters[k], 64) != hipSuccess ||
+            hipEventCreate(&e->start[k]) != hipSuccess || hipEventCreate(&e->stop[k]) != hipSuccess)
+            return bad(fail(XPR_E_NOMEM, "batch engine: allocation"));
+        std::memset(e->host[k], 0, sizeof(Table));
+        const int rc = capture(*e, k, prop.multiProcessorCount);
+        if (rc != XPR_OK) return bad(rc);
+    }
+    *out = e;
+    return XPR_OK;
+}
+
+Table *engine_table(Engine *e, int slot) { return e->host[slot & 1]; }
+
+int engine_launch(Engine *e, int slot)
+{
+    const int k = slot & 1;
+    XPR_HIP_CHECK(hipEventRecord(e->start[k], e->stream));
+    XPR_HIP_CHECK(hipGraphLaunch(e->exec[k], e->stream));
+    XPR_HIP_CHECK(hipEventRecord(e->stop[k], e->stream));
+    e->pending[k] = true;
+    return XPR_OK;
+}
+
+int engine_wait(Engine *e, int slot, float *ms)
+{
+    const int k = slot & 1;
+    if (ms) *ms = 0.0f;
+    if (!e->pending[k]) return XPR_OK;
+    XPR_HIP_CHECK(hipEventSynchronize(e->stop[k]));
+    e->pending[k] = false;
+    if (ms) XPR_HIP_CHECK(hipEventElapsedTime(ms, e->start[k], e->stop[k]));
+    return XPR_OK;
+}
+
+void engine_destroy(Engine *e)
+{
+    if (!e) return;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (int k = 0; k < 2; ++k) {
+        if (e->exec[k]) (void)hipGraphExecDestroy(e->exec[k]);
+        if (e->graph[k]) (void)hipGraphDestroy(e->graph[k]);
+        if (e->host[k]) (void)hipHostFree(e->host[k]);
+        if (e->dev[k]) (void)hipFree(e->dev[k]);
+        if (e->counters[k]) (void)hipFree(e->counters[k]);
+        if (e->start[k]) (void)hipEventDestroy(e->start[k]);
+        if (e->stop[k]) (void)hipEventDestroy(e->stop[k]);
+    }
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+} // namespace xpr_batch
 
 // ===========================================================================
 // C-ABI
