@@ -60,6 +60,8 @@ def parse():
                          "iteration")
     ap.add_argument("--jobs", type=int, default=24, help="jobs of the serving run")
     ap.add_argument("--rate", type=float, default=4.0, help="mean arrivals per second")
+    ap.add_argument("--sizes", default="128,384,768",
+                    help="serving: detector sizes the jobs are drawn from")
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--max-concurrent", type=int, default=8)
     ap.add_argument("--calibrate", action="store_true",
@@ -132,8 +134,9 @@ def serve_main(args):
     rng = np.random.default_rng(2024)
     jobs = []
     t0 = time.perf_counter()
+    classes = [c for c in SERVE_CLASSES if str(c[0]) in args.sizes.split(",")]
     for j in range(args.jobs):
-        D, N, n_ph = SERVE_CLASSES[int(rng.integers(0, 3))]
+        D, N, n_ph = classes[int(rng.integers(0, len(classes)))]
         so, sd = float(rng.uniform(80, 150)), float(rng.uniform(800, 1200))
         geom = Geometry(dwidth=D, dheight=D, projections=N, so=so, sd=sd, px=0.1)
         _, grid = geom.projdata()
@@ -176,7 +179,8 @@ def serve_main(args):
     for (_, _, D, _), r in zip(jobs, res):
         by.setdefault(D, []).append(r.latency_ms)
     out = {
-        "metric": METRIC, "mode": "serve", "value": vv / (makespan * 1e-3),
+        "metric": METRIC, "mode": "serve", "serve_mode": os.environ.get("XPR_SERVE_MODE", "streams"),
+        "value": vv / (makespan * 1e-3),
         "unit": "voxel-views/s", "n_gpus": 1, "higher_is_better": True,
         "p50_ms": round(float(np.percentile(lat, 50)), 1),
         "p99_ms": round(float(np.percentile(lat, 99)), 1),

@@ -7,7 +7,7 @@
 // jitter: its result does not depend on what else runs.
 //
 // Two ways to run admitted jobs:
-//  * batch (default): continuous batching.  Each admitted job prepares its
+//  * batch (XPR_SERVE_MODE=batch): continuous batching.  Each admitted job prepares its
 //    resident buffers on its own stream (raw projections uploaded and
 //    preprocessed on the device), then joins the active set of one scheduler
 //    thread.  The scheduler runs steps: per step every active job gets a share
@@ -17,8 +17,14 @@
 //    persistent kernels replayed from a captured HIP graph run them all.  A
 //    job's pieces run in its own sub-iteration order with its own arguments,
 //    so its volume is the one the naive schedule gives alone.
-//  * streams: each admitted job runs the naive schedule on its own streams
-//    (XPR_SERVE_MODE=streams).
+//    Jobs whose whole work fits in one step (and u32-accumulator jobs) run
+//    on their own streams beside the steps.
+//  * streams (default): each admitted job runs the naive schedule on its own
+//    streams; the device interleaves the workgroups of concurrent jobs.  On
+//    the mixed-size serving workload this gives the lower median latency: the
+//    persistent step kernels hold every CU until their queue drains, so small
+//    jobs beside them wait, while the batch mode's shortest-work-first steps
+//    help mid-size jobs and the tail.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -392,7 +398,7 @@ struct JobRun {
 struct Server {
     uint64_t budget;
     int max_conc, device;
-    bool batch_mode = true;
+    bool batch_mode = false;
     double step_target = 5e9;       // voxel-views per step
     std::mutex m;
     std::condition_variable cv;
@@ -408,7 +414,7 @@ struct Server {
 
     Server(uint64_t b, int mc, int dev) : budget(b), max_conc(mc), device(dev)
     {
-        if (const char *e = std::getenv("XPR_SERVE_MODE")) batch_mode = std::string(e) != "streams";
+        if (const char *e = std::getenv("XPR_SERVE_MODE")) batch_mode = std::string(e) == "batch";
         if (const char *e = std::getenv("XPR_SERVE_STEP")) step_target = std::max(1e6, std::atof(e));
         dispatcher = std::thread([this] { dispatch(); });
         if (batch_mode) scheduler = std::thread([this] { schedule(); });
@@ -444,7 +450,10 @@ struct Server {
             in_use += run->need;
             ++running;
             run->stats.start_ms = now_ms();
-            run->batched = batch_mode && !run->streamed && run->job.acc_bits == XPR_ACC_U64;
+            // a job whose whole work fits in one step gains nothing from
+            // batching: its small launches run at once beside the steps
+            run->batched = batch_mode && !run->streamed && run->job.acc_bits == XPR_ACC_U64 &&
+                           double(run->stats.voxel_views) > step_target;
             if (run->batched)
                 workers.emplace_back([this, run] { execute_batched(run); });
             else

@@ -57,36 +57,41 @@ constexpr float kQ22Eps = 1.0f / 4194304.0f; // EPSILON (constants.cl:23)
 // ---------------------------------------------------------------------------
 // storage types
 // ---------------------------------------------------------------------------
+// Volume, image and error buffers are global memory; accesses go through
+// global-address-space pointers so they stay global instructions (not generic
+// ones, which the LDS wait counter also covers) whatever the pointer's origin.
+template <typename T> using Glob = __attribute__((address_space(1))) T;
+
 struct F32 {
     using T = float;
-    static __device__ __forceinline__ float load(const T *p, uint64_t i) { return p[i]; }
-    static __device__ __forceinline__ void store(T *p, uint64_t i, float v) { p[i] = v; }
+    static __device__ __forceinline__ float load(const T *p, uint64_t i) { return ((const Glob<T> *)p)[i]; }
+    static __device__ __forceinline__ void store(T *p, uint64_t i, float v) { ((Glob<T> *)p)[i] = v; }
 };
 struct F16 {
     using T = uint16_t;
     static __device__ __forceinline__ float load(const T *p, uint64_t i)
     {
-        return (float)__builtin_bit_cast(_Float16, p[i]);
+        return (float)__builtin_bit_cast(_Float16, ((const Glob<T> *)p)[i]);
     }
     static __device__ __forceinline__ void store(T *p, uint64_t i, float v)
     {
-        p[i] = __builtin_bit_cast(uint16_t, (_Float16)v);
+        ((Glob<T> *)p)[i] = __builtin_bit_cast(uint16_t, (_Float16)v);
     }
 };
 struct BF16 {
     using T = uint16_t;
     static __device__ __forceinline__ float load(const T *p, uint64_t i)
     {
-        return __builtin_bit_cast(float, (uint32_t)p[i] << 16);
+        return __builtin_bit_cast(float, (uint32_t)((const Glob<T> *)p)[i] << 16);
     }
     static __device__ __forceinline__ void store(T *p, uint64_t i, float v)
     {
         uint32_t b = __builtin_bit_cast(uint32_t, v);
         if ((b & 0x7fffffffu) > 0x7f800000u) {
-            p[i] = (uint16_t)((b >> 16) | 0x40u);                 // quiet NaN
+            ((Glob<T> *)p)[i] = (uint16_t)((b >> 16) | 0x40u);                 // quiet NaN
         } else {
-            b += 0x7fffu + ((b >> 16) & 1u);                      // nearest even
-            p[i] = (uint16_t)(b >> 16);
+            b += 0x7fffu + ((b >> 16) & 1u);                                   // nearest even
+            ((Glob<T> *)p)[i] = (uint16_t)(b >> 16);
         }
     }
 };
@@ -139,19 +144,27 @@ struct ViewConsts {
     float ulo, uhi, vlo, vhi;  // thresholds of the half-open column and row bounds
 };
 
+// ProjData is read-only for every kernel: reading it through the constant
+// address space lets uniform records go through scalar loads even where the
+// compiler cannot prove that the kernel's own stores leave it unchanged.
+using ConstF = const __attribute__((address_space(4))) float;
+__device__ __forceinline__ ConstF *pd_const(const float *p) { return (ConstF *)p; }
+
 // view bounds, clamped to the buffers the call holds
-__device__ __forceinline__ void view_thresholds(const float *__restrict__ pd,
+__device__ __forceinline__ void view_thresholds(const float *__restrict__ pdg,
                                                 const xpr_proj_args &a, ViewConsts &c)
 {
+    ConstF *pd = pd_const(pdg);
     c.ulo = lo_threshold(fmaxf(pd[16], 0.0f));
     c.uhi = hi_threshold(fminf(pd[17], (float)a.det_width));
     c.vlo = lo_threshold(fmaxf(pd[18], 0.0f));
     c.vhi = hi_threshold(fminf(pd[19], (float)a.line_rows));
 }
 
-__device__ __forceinline__ void row_terms(const float *__restrict__ pd, float py, float pz,
+__device__ __forceinline__ void row_terms(const float *__restrict__ pdg, float py, float pz,
                                           ViewConsts &c)
 {
+    ConstF *pd = pd_const(pdg);
     c.c0x = pd[0];
     c.c1x = pd[4];
     c.c3x = pd[12];
@@ -171,9 +184,10 @@ __device__ __forceinline__ bool fast_rcp_box(float w, float t)
 }
 
 // row segment [px_first, px_last] at fixed (py, pz): the two ends are its corners
-__device__ __forceinline__ bool fast_rcp_range(const float *__restrict__ p, const ViewConsts &c,
+__device__ __forceinline__ bool fast_rcp_range(const float *__restrict__ pg, const ViewConsts &c,
                                                float px_first, float px_last, float py, float pz)
 {
+    ConstF *p = pd_const(pg);
     const float w0 = __builtin_fmaf(c.c3x, px_first, c.k3);
     const float w1 = __builtin_fmaf(c.c3x, px_last, c.k3);
     const float t = fabsf(c.c3x) * fmaxf(fabsf(px_first), fabsf(px_last)) + fabsf(p[13] * py) +
@@ -206,9 +220,10 @@ __device__ __forceinline__ bool project(const ViewConsts &c, float px, float lin
     return inb;
 }
 
+// detector view of launch view `view` (uniform: every caller passes a uniform view)
 __device__ __forceinline__ uint32_t view_of(const xpr_proj_args &a, uint32_t view)
 {
-    return a.view_index ? a.view_index[view] : view;
+    return a.view_index ? __builtin_amdgcn_readfirstlane(a.view_index[view]) : view;
 }
 
 // raw buffer over one detector view: out-of-range offsets read as 0
@@ -254,20 +269,16 @@ struct SlabShape {
     int64_t nrows;         // rows in the slab
 };
 
-// One unit of the backward: segment unit % nseg of the kWaves rows of row
-// block unit / nseg (one row per wave).
+// Segment seg of one (layer, depth) row of the backward, run by one wave.
 template <typename VS, typename ES, int J, bool CROP>
-__device__ __forceinline__ void backward_unit(typename VS::T *__restrict__ vol,
-                                              const typename ES::T *__restrict__ err,
-                                              const float *__restrict__ pd,
-                                              const xpr_proj_args &a, const SlabShape &s,
-                                              int64_t unit,
-                                              unsigned long long *__restrict__ empty_count)
+__device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
+                                             const typename ES::T *__restrict__ err,
+                                             const float *__restrict__ pd,
+                                             const xpr_proj_args &a, const SlabShape &s,
+                                             int64_t row, int seg,
+                                             unsigned long long *__restrict__ empty_count)
 {
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int seg = (int)(unit % s.nseg);
-    const int64_t row = s.row_base + (unit / s.nseg) * kWaves + wave;
     if (row >= s.nrows) return;                       // whole wave leaves together
     const int32_t l = (int32_t)(row / s.nk), k = (int32_t)(row % s.nk);
     const float py = __builtin_fmaf((float)(l + a.vmin[1]), a.voxel[1], a.origin[1]);
@@ -345,7 +356,9 @@ __global__ __launch_bounds__(kThreads) void k_backward(
     const float *__restrict__ pd, xpr_proj_args a, SlabShape s,
     unsigned long long *__restrict__ empty_count)
 {
-    backward_unit<VS, ES, J, CROP>(vol, err, pd, a, s, blockIdx.x, empty_count);
+    // segment blockIdx % nseg of the kWaves rows of row block blockIdx / nseg
+    const int64_t row = s.row_base + (int64_t)(blockIdx.x / s.nseg) * kWaves + (threadIdx.x >> 6);
+    backward_row<VS, ES, J, CROP>(vol, err, pd, a, s, row, (int)(blockIdx.x % s.nseg), empty_count);
 }
 
 // FLOAT_TO_FIXED (constants.cl:22), saturating like the GPU conversion.
@@ -404,7 +417,7 @@ struct BrickShape {
 template <typename AccT>
 __device__ __forceinline__ void global_add(AccT *p, AccT v)
 {
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add((Glob<AccT> *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 enum : uint32_t {
@@ -583,34 +596,41 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
     const int32_t dk = lane & 15, xg = lane >> 4;     // depth-lane order
 
     // Q.22 voxel values in both lane orders (32 bits suffice unless some
-    // value is >= 1024)
+    // value is >= 1024).  Raw buffer loads from this wave's brick corner
+    // (x0, l, k0): out-of-slab voxels read 0 without a branch, so all 32
+    // loads are in flight together.
     uint32_t qx[kBK], qd[kBK];
     bool big = false;
+    {
+        constexpr uint32_t esz = (uint32_t)sizeof(typename VS::T);
+        const int64_t base = ((int64_t)l * s.nk + k0) * s.nx + x0;
+        const int64_t left = (int64_t)s.nl * s.nk * s.nx - base;      // voxels from the corner on
+        const uint32_t bytes = l < s.nl ? (uint32_t)min<int64_t>(left * esz, 0x7fffffff) : 0u;
+        const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(vol + (l < s.nl ? base : 0)), 0, (int)bytes, kBufferWord3);
+        auto q22 = [&](float v, bool track) {
+            const float sc = v * kQ22One;               // FLOAT_TO_FIXED (constants.cl:22)
+            const bool pos = sc > 0.0f, over = sc >= 4294967296.0f;
+            if (sizeof(AccT) == 8 && track) big |= pos & over;
+            return pos ? (over ? 0xffffffffu : (uint32_t)sc) : 0u;
+        };
+        float vx_[kBK], vd_[kBK];
 #pragma unroll
-    for (int i = 0; i < kBK; ++i) {
-        const int32_t x = x0 + lane, k = k0 + i;
-        uint32_t q = 0;
-        if (x < s.nx && l < s.nl && k < s.nk) {
-            const float v = VS::load(vol, ((uint64_t)l * s.nk + k) * s.nx + x);
-            if (sizeof(AccT) == 8) {
-                const uint64_t q64 = q22_u64(v);
-                big |= (q64 >> 32) != 0;
-                q = (uint32_t)q64;
-            } else {
-                q = q22_u32(v);
-            }
+        for (int i = 0; i < kBK; ++i) {
+            const bool in = (x0 + lane < s.nx) & (k0 + i < s.nk);
+            const uint32_t off = (uint32_t)(i * s.nx + lane) * esz;
+            vx_[i] = load_or_zero<VS>(rsrc, in ? off : kOutOfRange);
         }
-        qx[i] = q;
-    }
 #pragma unroll
-    for (int j = 0; j < kBK; ++j) {
-        const int32_t x = x0 + xg * 16 + j, k = k0 + dk;
-        uint32_t q = 0;
-        if (x < s.nx && l < s.nl && k < s.nk) {
-            const float v = VS::load(vol, ((uint64_t)l * s.nk + k) * s.nx + x);
-            q = sizeof(AccT) == 8 ? (uint32_t)q22_u64(v) : q22_u32(v);
+        for (int j = 0; j < kBK; ++j) {
+            const bool in = (x0 + xg * 16 + j < s.nx) & (k0 + dk < s.nk);
+            const uint32_t off = (uint32_t)(dk * s.nx + xg * 16 + j) * esz;
+            vd_[j] = load_or_zero<VS>(rsrc, in ? off : kOutOfRange);
         }
-        qd[j] = q;
+#pragma unroll
+        for (int i = 0; i < kBK; ++i) qx[i] = q22(vx_[i], true);
+#pragma unroll
+        for (int j = 0; j < kBK; ++j) qd[j] = q22(vd_[j], false);
     }
     if (__syncthreads_or(big)) {
         forward_direct_u64<VS, AccT, CROP>(vol, sums, pd, a, s, x0, l0, k0);
@@ -840,23 +860,55 @@ __global__ __launch_bounds__(256) void k_preprocess(const float *__restrict__ in
 using xpr_batch::Item;
 using xpr_batch::Table;
 
+// Work units are dealt per XCD: workgroups are dispatched to the 8 XCDs
+// round-robin by blockIdx, so the workgroups of XCD x = blockIdx % 8 take the
+// units u = x + 8k, k from the phase's counter for x.  Each XCD's L2 then
+// serves the same slice of the volume and detector as in a single-job launch
+// (whose block b runs on XCD b % 8); within an XCD the units balance
+// dynamically.  Every unit is taken exactly once whatever the real dispatch.
+constexpr uint32_t kXcds = 8;
+
+__device__ __forceinline__ uint32_t xcd_of_block() { return blockIdx.x % kXcds; }
+
 // next work unit of the workgroup (uniform); the barrier before the fetch
 // also ends the previous unit's use of shared memory
-__device__ __forceinline__ uint32_t next_unit(uint32_t *counter, uint32_t *slot)
+__device__ __forceinline__ uint32_t next_unit(uint32_t *counters, uint32_t ph, uint32_t *slot)
 {
+    const uint32_t x = xcd_of_block();
     __syncthreads();
-    if (threadIdx.x == 0) *slot = atomicAdd(counter, 1u);
+    if (threadIdx.x == 0) *slot = x + kXcds * atomicAdd(&counters[ph * kXcds + x], 1u);
     __syncthreads();
     return __builtin_amdgcn_readfirstlane(*slot);
 }
+
+// The table is read-only for the kernels: reading it through the constant
+// address space gives scalar loads into registers shared by the wave (the
+// compiler cannot otherwise prove it unchanged by the kernels' stores and
+// would read it with per-lane vector loads).
+using ConstU32 = const __attribute__((address_space(4))) uint32_t;
+
+__device__ __forceinline__ uint32_t uniform_u32(const uint32_t *p) { return *(ConstU32 *)p; }
 
 // item of phase ph holding unit u (items are in prefix order)
 __device__ __forceinline__ uint32_t item_of(const Table *__restrict__ t, uint32_t ph, uint32_t u)
 {
     uint32_t i = 0;
-    const uint32_t n = t->count[ph];
-    while (i + 1 < n && t->items[ph][i + 1].first <= u) ++i;
-    return __builtin_amdgcn_readfirstlane(i);
+    const uint32_t n = uniform_u32(&t->count[ph]);
+    while (i + 1 < n && uniform_u32(&t->items[ph][i + 1].first) <= u) ++i;
+    return i;
+}
+
+__device__ __forceinline__ Item uniform_item(const Item *__restrict__ p)
+{
+    constexpr int kWords = (int)(sizeof(Item) / 4);
+    static_assert(sizeof(Item) % 4 == 0, "item size");
+    uint32_t w[kWords];
+    ConstU32 *src = (ConstU32 *)p;
+#pragma unroll
+    for (int i = 0; i < kWords; ++i) w[i] = src[i];
+    Item it;
+    __builtin_memcpy(&it, w, sizeof(Item));
+    return it;
 }
 
 __device__ __forceinline__ BrickShape batch_brick_shape(const xpr_proj_args &a)
@@ -887,7 +939,7 @@ __device__ __forceinline__ SlabShape batch_slab_shape(const xpr_proj_args &a, in
 
 __global__ __launch_bounds__(kThreads) void k_batch_reset(uint32_t *counters)
 {
-    if (threadIdx.x < (unsigned)xpr_batch::kPhases) counters[threadIdx.x] = 0u;
+    if (threadIdx.x < (unsigned)xpr_batch::kPhases * kXcds) counters[threadIdx.x] = 0u;
 }
 
 __global__ __launch_bounds__(kThreads, 4) void k_batch_forward(const Table *__restrict__ t,
@@ -897,55 +949,62 @@ __global__ __launch_bounds__(kThreads, 4) void k_batch_forward(const Table *__re
     __shared__ ViewBin bins[kViewChunk];
     __shared__ uint32_t slot;
     const uint32_t ph = xpr_batch::kForward;
-    const uint32_t total = t->total[ph];
+    const uint32_t total = uniform_u32(&t->total[ph]);
     if (total == 0) return;
     for (int i = threadIdx.x; i < 2 * kTileCap; i += kThreads) lds[i] = 0ull;
     for (;;) {
-        const uint32_t u = next_unit(&counters[ph], &slot);
+        const uint32_t u = next_unit(counters, ph, &slot);
         if (u >= total) break;                              // uniform: every workgroup ends here
-        const Item &it = t->items[ph][item_of(t, ph, u)];
+        const Item it = uniform_item(&t->items[ph][item_of(t, ph, u)]);
         const BrickShape s = batch_brick_shape(it.a);
         const int64_t b = (int64_t)(u - it.first);
         uint64_t *sums = (uint64_t *)it.sums;
+        const void *vol = it.dst;
         switch (it.dt_dst) {
         case XPR_F32:
-            forward_brick<F32, uint64_t, false>((const float *)it.dst, sums, it.pd, it.a, s, b, lds, bins);
+            forward_brick<F32, uint64_t, false>((const float *)vol, sums, it.pd, it.a, s, b, lds, bins);
             break;
         case XPR_F16:
-            forward_brick<F16, uint64_t, false>((const uint16_t *)it.dst, sums, it.pd, it.a, s, b, lds, bins);
+            forward_brick<F16, uint64_t, false>((const uint16_t *)vol, sums, it.pd, it.a, s, b, lds, bins);
             break;
         default:
-            forward_brick<BF16, uint64_t, false>((const uint16_t *)it.dst, sums, it.pd, it.a, s, b, lds, bins);
+            forward_brick<BF16, uint64_t, false>((const uint16_t *)vol, sums, it.pd, it.a, s, b, lds, bins);
             break;
         }
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_batch_backward(const Table *__restrict__ t,
-                                                             uint32_t *counters)
+// Backward units are the single-job launch's workgroups: segment unit % nseg
+// of the kWaves adjacent rows of row block unit / nseg, one row per wave, run
+// together so the four rows (which gather nearly the same detector pixels)
+// share the CU's L1.
+__global__ __launch_bounds__(kThreads, 7) void k_batch_backward(const Table *__restrict__ t,
+                                                                uint32_t *counters)
 {
     __shared__ uint32_t slot;
     const uint32_t ph = xpr_batch::kBackward;
-    const uint32_t total = t->total[ph];
+    const uint32_t total = uniform_u32(&t->total[ph]);
     if (total == 0) return;
     for (;;) {
-        const uint32_t u = next_unit(&counters[ph], &slot);
+        const uint32_t u = next_unit(counters, ph, &slot);
         if (u >= total) break;
-        const Item &it = t->items[ph][item_of(t, ph, u)];
+        const Item it = uniform_item(&t->items[ph][item_of(t, ph, u)]);
         const SlabShape s = batch_slab_shape(it.a, kBwdJ);
         const int64_t unit = (int64_t)(u - it.first);
+        const int64_t row = (unit / s.nseg) * kWaves + (threadIdx.x >> 6);
+        const int seg = (int)(unit % s.nseg);
         switch (it.dt_dst) {                                // volume and error share the dtype
         case XPR_F32:
-            backward_unit<F32, F32, kBwdJ, false>((float *)it.dst, (const float *)it.err, it.pd, it.a, s,
-                                                  unit, nullptr);
+            backward_row<F32, F32, kBwdJ, false>((float *)it.dst, (const float *)it.err, it.pd, it.a, s,
+                                                 row, seg, nullptr);
             break;
         case XPR_F16:
-            backward_unit<F16, F16, kBwdJ, false>((uint16_t *)it.dst, (const uint16_t *)it.err, it.pd,
-                                                  it.a, s, unit, nullptr);
+            backward_row<F16, F16, kBwdJ, false>((uint16_t *)it.dst, (const uint16_t *)it.err, it.pd,
+                                                 it.a, s, row, seg, nullptr);
             break;
         default:
-            backward_unit<BF16, BF16, kBwdJ, false>((uint16_t *)it.dst, (const uint16_t *)it.err, it.pd,
-                                                    it.a, s, unit, nullptr);
+            backward_row<BF16, BF16, kBwdJ, false>((uint16_t *)it.dst, (const uint16_t *)it.err, it.pd,
+                                                   it.a, s, row, seg, nullptr);
             break;
         }
     }
@@ -955,7 +1014,7 @@ template <typename IS, typename ES>
 __device__ __forceinline__ void error_range(const Item &it, uint64_t i0, uint64_t i1)
 {
     const typename IS::T *img = (const typename IS::T *)it.src;
-    const uint64_t *sums = (const uint64_t *)it.sums;
+    const Glob<uint64_t> *sums = (const Glob<uint64_t> *)it.sums;
     typename ES::T *err = (typename ES::T *)it.dst;
     for (uint64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
         const float ref = IS::load(img, i);
@@ -969,7 +1028,7 @@ __device__ __forceinline__ void convert_range(const Item &it, uint64_t i0, uint6
 {
     const typename SS::T *src = (const typename SS::T *)it.src;
     float *dst = (float *)it.dst;
-    for (uint64_t i = i0 + threadIdx.x; i < i1; i += kThreads) dst[i] = SS::load(src, i);
+    for (uint64_t i = i0 + threadIdx.x; i < i1; i += kThreads) F32::store(dst, i, SS::load(src, i));
 }
 
 // convert (to float32), error or fill items, one chunk of kElemUnit elements per unit
@@ -977,16 +1036,16 @@ __global__ __launch_bounds__(kThreads) void k_batch_elementwise(const Table *__r
                                                                 uint32_t *counters, uint32_t ph)
 {
     __shared__ uint32_t slot;
-    const uint32_t total = t->total[ph];
+    const uint32_t total = uniform_u32(&t->total[ph]);
     if (total == 0) return;
     for (;;) {
-        const uint32_t u = next_unit(&counters[ph], &slot);
+        const uint32_t u = next_unit(counters, ph, &slot);
         if (u >= total) break;
-        const Item &it = t->items[ph][item_of(t, ph, u)];
+        const Item it = uniform_item(&t->items[ph][item_of(t, ph, u)]);
         const uint64_t i0 = (uint64_t)(u - it.first) * xpr_batch::kElemUnit;
         const uint64_t i1 = min(it.n, i0 + xpr_batch::kElemUnit);
         if (ph == xpr_batch::kFill) {
-            uint64_t *dst = (uint64_t *)it.dst;
+            Glob<uint64_t> *dst = (Glob<uint64_t> *)it.dst;
             for (uint64_t i = i0 + threadIdx.x; i < i1; i += kThreads) dst[i] = it.value;
         } else if (ph == xpr_batch::kConvert) {
             if (it.dt_src == XPR_F16) convert_range<F16>(it, i0, i1);
@@ -1211,15 +1270,17 @@ int capture(Engine &e, int k, int cus)
 {
     XPR_HIP_CHECK(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
     const dim3 blk(kThreads);
+    // grids are multiples of kXcds, so every XCD residue class has workgroups
+    auto grid = [&](int per_cu) { return dim3((unsigned)(((cus * per_cu + 7) / 8) * 8)); };
     (void)hipMemcpyAsync(e.dev[k], e.host[k], sizeof(Table), hipMemcpyHostToDevice, e.stream);
     hipLaunchKernelGGL(k_batch_reset, dim3(1), blk, 0, e.stream, e.counters[k]);
-    hipLaunchKernelGGL(k_batch_elementwise, dim3(cus * 4), blk, 0, e.stream, e.dev[k], e.counters[k],
+    hipLaunchKernelGGL(k_batch_elementwise, grid(4), blk, 0, e.stream, e.dev[k], e.counters[k],
                        (uint32_t)kConvert);
-    hipLaunchKernelGGL(k_batch_forward, dim3(cus * 4), blk, 0, e.stream, e.dev[k], e.counters[k]);
-    hipLaunchKernelGGL(k_batch_elementwise, dim3(cus * 4), blk, 0, e.stream, e.dev[k], e.counters[k],
+    hipLaunchKernelGGL(k_batch_forward, grid(4), blk, 0, e.stream, e.dev[k], e.counters[k]);
+    hipLaunchKernelGGL(k_batch_elementwise, grid(4), blk, 0, e.stream, e.dev[k], e.counters[k],
                        (uint32_t)kError);
-    hipLaunchKernelGGL(k_batch_backward, dim3(cus * 6), blk, 0, e.stream, e.dev[k], e.counters[k]);
-    hipLaunchKernelGGL(k_batch_elementwise, dim3(cus * 4), blk, 0, e.stream, e.dev[k], e.counters[k],
+    hipLaunchKernelGGL(k_batch_backward, grid(8), blk, 0, e.stream, e.dev[k], e.counters[k]);
+    hipLaunchKernelGGL(k_batch_elementwise, grid(4), blk, 0, e.stream, e.dev[k], e.counters[k],
                        (uint32_t)kFill);
     const hipError_t le = hipGetLastError();
     const hipError_t ce = hipStreamEndCapture(e.stream, &e.graph[k]);
@@ -1244,7 +1305,7 @@ int engine_create(Engine **out)
     for (int k = 0; k < 2; ++k) {
         if (hipHostMalloc((void **)&e->host[k], sizeof(Table), hipHostMallocDefault) != hipSuccess ||
             hipMalloc((void **)&e->dev[k], sizeof(Table)) != hipSuccess ||
-            hipMalloc((void **)&e->counters[k], 64) != hipSuccess ||
+            hipMalloc((void **)&e->counters[k], sizeof(uint32_t) * kPhases * kXcds) != hipSuccess ||
             hipEventCreate(&e->start[k]) != hipSuccess || hipEventCreate(&e->stop[k]) != hipSuccess)
             return bad(fail(XPR_E_NOMEM, "batch engine: allocation"));
         std::memset(e->host[k], 0, sizeof(Table));
