@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default tile shapes (no xpr_tune_* before the timed steps)")
     ap.add_argument("--serve", action="store_true",
                     help="serving run instead: Poisson arrivals of reconstructions of "
                          "mixed sizes, p50/p99 time-to-reconstruction")
@@ -339,6 +341,17 @@ def main():
         X.fill_u64(sums, 0)
         if ev: ev[4].record()
 
+    tuned = None
+    if not args.no_tune:
+        # the library's tile-shape tuning, on scratch copies of the first slab's buffers
+        a0 = args_for(slabs[0][0], slabs[0][1], [np.float32(grid["orig"][i]) for i in range(3)])
+        cap = X.tune_forward(vol[slabs[0][0]:slabs[0][1]].clone(), torch.zeros_like(sums), pds[0], a0)
+        X.error(images[0], sums, err, weight)
+        jb = X.tune_backward(vol[slabs[0][0]:slabs[0][1]].clone(), err, pds[0], a0)
+        sums.zero_()
+        tuned = {"forward_tile_cap": cap, "backward_voxels_per_lane": jb}
+        log(f"tuned tile shapes: {tuned}")
+
     for k in range(args.warmup):
         step(k)
     torch.cuda.synchronize()
@@ -415,7 +428,7 @@ def main():
                                f"(A_s={A_s}), method_1a slab {args.slab} device-resident, one "
                                f"sub-iteration per step, {args.dtype} storage",
                    "grid": [W, H, W], "views_per_subiteration": A_s, "slab": args.slab,
-                   "voxel_views_per_step": vv_step},
+                   "voxel_views_per_step": vv_step, "tile_shapes": tuned or "defaults"},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -158,6 +158,24 @@ int xpr_minmax_read(const int32_t *minmax_host, float *mn, float *mx);
 int xpr_preprocess(xpr_stream s, const float *in, void *out, int out_dtype, uint64_t n,
                    int normalize, int mlog, float mn, float mx);
 
+/* ---- tile-shape tuning (no counterpart in the reference: its kernels have a
+ *      fixed work-item shape, gpu/kernel.hpp:73-81) ------------------------------
+ *
+ * Time the compiled tile shapes of a projector on the caller's buffers and keep
+ * the fastest in a per-process table keyed by the launch's shape (dtypes, slab
+ * width and depth, views, modules, detector rows); later xpr_forward /
+ * xpr_backward calls of that shape use it.  The shapes change only how work is
+ * spread over lanes and paths: results are identical whatever is chosen.
+ * Forward: LDS tile capacity (entries); the scratch sums receive several
+ * forward passes.  Backward: voxels per lane (2, 4, 8, 16); the scratch volume
+ * receives several updates.  *choice receives the shape kept. */
+int xpr_tune_forward(xpr_stream s, const void *volume, int vol_dtype, void *sums_scratch,
+                     int acc_bits, const float *projdata, const xpr_proj_args *args, int *choice);
+int xpr_tune_backward(xpr_stream s, void *volume_scratch, int vol_dtype, const void *err,
+                      int err_dtype, const float *projdata, const xpr_proj_args *args, int *choice);
+/* Forget every tuned shape (defaults apply again). */
+int xpr_tune_reset(void);
+
 /* ---- geometry (Geometry, dataset/geometry.hpp:35-76, 182-328) ----------- */
 
 /* JSON "geometry" and "detector" sections (global_parameters.hpp:23-54).

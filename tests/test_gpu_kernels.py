@@ -314,3 +314,61 @@ def test_forward_bit_exact_across_tile_widths(cuda, scale):
     sums = torch.zeros(shape, dtype=torch.int64, device=cuda)
     X.forward(torch.from_numpy(vol).to(cuda), sums, torch.from_numpy(pd).to(cuda), _args(d))
     np.testing.assert_array_equal(sums.cpu().numpy().view(np.uint64), ref)
+
+
+@pytest.mark.parametrize("case", [1, 2, 4])
+def test_backward_lane_shapes_bit_identical(cuda, monkeypatch, case):
+    """Every compiled voxels-per-lane shape of the backward (2, 4, 8, 16) gives
+    the same volume bit for bit (widths 100 and 130 leave partial segments),
+    including the cropped band of case 4, and matches the oracle."""
+    import torch
+    import xpulse_recon_amd as X
+    spec, pd, d = _case(case)
+    nx, nl, nk = (d["vmax"][i] - d["vmin"][i] for i in range(3))
+    rng = np.random.default_rng(300 + case)
+    vol = rng.uniform(0.5, 2.0, (nl, nk, nx)).astype(np.float32)
+    err = rng.normal(0.0, 0.4, (d["n_views"], d["line_rows"], d["det_width"])).astype(np.float32)
+    ref = vol.copy()
+    O.backward_f32(ref, err, pd, oracle_geom(d))
+    te, tp = torch.from_numpy(err).to(cuda), torch.from_numpy(pd).to(cuda)
+    outs = []
+    for j in ("2", "4", "8", "16"):
+        monkeypatch.setenv("XPR_BACKWARD_J", j)
+        tv = torch.from_numpy(vol).to(cuda)
+        X.backward(tv, te, tp, _args(d))
+        outs.append(tv.cpu().numpy())
+    for o in outs[1:]:
+        np.testing.assert_array_equal(o, outs[0])
+    np.testing.assert_allclose(outs[0], ref, rtol=4e-7, atol=0)
+
+
+def test_tuned_shapes_keep_results(cuda):
+    """xpr_tune_* time the shapes on scratch buffers and keep one per launch
+    shape; launches after tuning give the untuned results bit for bit."""
+    import torch
+    import xpulse_recon_amd as X
+    spec, pd, d = _case(1)
+    nx, nl, nk = (d["vmax"][i] - d["vmin"][i] for i in range(3))
+    rng = np.random.default_rng(9)
+    vol = rng.uniform(0.5, 2.0, (nl, nk, nx)).astype(np.float32)
+    err = rng.normal(0.0, 0.4, (d["n_views"], d["line_rows"], d["det_width"])).astype(np.float32)
+    tp, te = torch.from_numpy(pd).to(cuda), torch.from_numpy(err).to(cuda)
+    shape = (d["n_views"], d["line_rows"], d["det_width"])
+    X.tune_reset()
+
+    def run():
+        s = torch.zeros(shape, dtype=torch.int64, device=cuda)
+        X.forward(torch.from_numpy(vol).to(cuda), s, tp, _args(d))
+        tv = torch.from_numpy(vol).to(cuda)
+        X.backward(tv, te, tp, _args(d))
+        return s.cpu().numpy(), tv.cpu().numpy()
+
+    s0, v0 = run()
+    cap = X.tune_forward(torch.from_numpy(vol).to(cuda),
+                         torch.zeros(shape, dtype=torch.int64, device=cuda), tp, _args(d))
+    j = X.tune_backward(torch.from_numpy(vol).to(cuda), te, tp, _args(d))
+    assert cap in (384, 768, 1536) and j in (2, 4, 8, 16)
+    s1, v1 = run()
+    np.testing.assert_array_equal(s1, s0)
+    np.testing.assert_array_equal(v1, v0)
+    X.tune_reset()

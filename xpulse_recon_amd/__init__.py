@@ -15,7 +15,7 @@ from ._lib import ProjArgs, XprError, proj_args  # noqa: F401
 
 __all__ = ["forward", "error", "backward", "fill", "fill_u32", "fill_u64", "convert",
            "preprocess", "minmax_normal", "proj_args", "ProjArgs", "XprError",
-           "dtype_code", "acc_bits"]
+           "dtype_code", "acc_bits", "tune_forward", "tune_backward", "tune_reset"]
 
 
 def _stream(stream):
@@ -137,3 +137,31 @@ def preprocess(data, out, normalize: bool, mlog: bool, mn: float = 0.0, mx: floa
         raise XprError("preprocess: output smaller than input")
     _lib.call("xpr_preprocess", _stream(stream), _ptr(data), _ptr(out), dtype_code(out),
               data.numel(), int(normalize), int(mlog), float(mn), float(mx))
+
+
+def tune_forward(volume, sums_scratch, projdata, args: ProjArgs, stream=None) -> int:
+    """Time the forward's tile capacities on these buffers (sums_scratch gets
+    several passes) and keep the fastest for this launch shape; returns it."""
+    _need_cuda(volume, sums_scratch, projdata)
+    _check_sizes(volume, sums_scratch, projdata, args)
+    out = C.c_int()
+    _lib.call("xpr_tune_forward", _stream(stream), _ptr(volume), dtype_code(volume),
+              _ptr(sums_scratch), acc_bits(sums_scratch), _ptr(projdata), C.byref(args),
+              C.byref(out))
+    return out.value
+
+
+def tune_backward(volume_scratch, err, projdata, args: ProjArgs, stream=None) -> int:
+    """Time the backward's voxels-per-lane shapes on these buffers
+    (volume_scratch gets several updates) and keep the fastest; returns it."""
+    _need_cuda(volume_scratch, err, projdata)
+    _check_sizes(volume_scratch, err, projdata, args)
+    out = C.c_int()
+    _lib.call("xpr_tune_backward", _stream(stream), _ptr(volume_scratch),
+              dtype_code(volume_scratch), _ptr(err), dtype_code(err), _ptr(projdata),
+              C.byref(args), C.byref(out))
+    return out.value
+
+
+def tune_reset() -> None:
+    _lib.call("xpr_tune_reset")

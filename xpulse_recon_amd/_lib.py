@@ -131,6 +131,11 @@ _SIGS = {
     "xpr_forward": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, _P, C.POINTER(ProjArgs)]),
     "xpr_error": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, _P, C.c_int, C.c_uint64, C.c_float]),
     "xpr_backward": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, _P, C.POINTER(ProjArgs), _P]),
+    "xpr_tune_forward": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, _P, C.POINTER(ProjArgs),
+                                   C.POINTER(C.c_int)]),
+    "xpr_tune_backward": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, _P, C.POINTER(ProjArgs),
+                                    C.POINTER(C.c_int)]),
+    "xpr_tune_reset": (C.c_int, []),
     "xpr_q22_to_float": (C.c_int, [_P, _P, C.c_int, _P, C.c_uint64]),
     "xpr_minmax_init": (C.c_int, [_P, _P]),
     "xpr_minmax_normal": (C.c_int, [_P, _P, C.c_uint64, _P]),

@@ -178,6 +178,7 @@ Parameters parameters_from_json(const Json &root)
         if (q.layer_store != "zstd" && q.layer_store != "ram") bad("xpr.layer_store", "zstd or ram");
         rdn(x, "threads", q.threads, "xpr");
         rd(x, "run_log", q.run_log, "xpr");
+        rd(x, "autotune", q.autotune, "xpr");
         rdn(x, "device", q.device, "xpr");
     }
     // initializeParameters (parameters.hpp:34-48); rx stays in pixels here and
@@ -255,6 +256,7 @@ Json parameters_to_json(const Parameters &p)
     x.set("layer_store", Json::string(p.xpr.layer_store));
     x.set("threads", Json::number(p.xpr.threads));
     x.set("run_log", Json::string(p.xpr.run_log));
+    x.set("autotune", Json::boolean(p.xpr.autotune));
     x.set("device", Json::number(p.xpr.device));
     root.set("xpr", x);
     return root;

@@ -74,6 +74,7 @@ struct XprSection {
     std::string layer_store = "zstd";// method_1a/1b/1c layer spill: "zstd" or "ram"
     int threads = 0;                 // host IO threads (0: hardware concurrency, max 16)
     std::string run_log = "";        // JSON run record (jitter, timings, memory)
+    bool autotune = true;            // time the projectors' tile shapes once per launch shape
     int device = 0;
     // in-process jobs (not JSON): projections from memory, volume to memory
     const float *mem_projections = nullptr;   // [n][h][w] raw images

@@ -10,6 +10,7 @@
 #include <functional>
 #include <iostream>
 #include <mutex>
+#include <set>
 #include <thread>
 
 #include "io.hpp"
@@ -504,6 +505,29 @@ void Reconstruction::log_sit(int64_t mit, int64_t sit, double ms)
     note("subiteration_ms", sit_ms_);
 }
 
+void Reconstruction::tune_shapes(void *vol_slab, int vdt, void *sums, void *err, int edt,
+                                 const float *pd, const xpr_proj_args &a)
+{
+    if (!p_.xpr.autotune) return;
+    static std::mutex mu;
+    static std::set<std::vector<int64_t>> done;
+    const std::vector<int64_t> key = {p_.xpr.device, vdt, edt, p_.xpr.accumulator,
+                                      a.vmax[0] - a.vmin[0], a.vmax[2] - a.vmin[2], a.n_views,
+                                      a.modules, a.det_width, a.line_rows, a.line_offset != 0};
+    {
+        std::lock_guard<std::mutex> g(mu);
+        if (!done.insert(key).second) return;
+    }
+    const uint64_t n = uint64_t(a.n_views) * uint64_t(a.line_rows) * uint64_t(a.det_width);
+    check(xpr_fill(stream_, err, edt, 0.0f, n), "fill");
+    int c = 0;
+    check(xpr_tune_backward(stream_, vol_slab, vdt, err, edt, pd, &a, &c), "tune backward");
+    note("tuned_backward_voxels_per_lane", Json::number(c));
+    check(xpr_tune_forward(stream_, vol_slab, vdt, sums, p_.xpr.accumulator, pd, &a, &c),
+          "tune forward");
+    note("tuned_forward_tile_cap", Json::number(c));
+}
+
 // ---------------------------------------------------------------------------
 // helpers shared by the schedules
 // ---------------------------------------------------------------------------
@@ -620,6 +644,16 @@ public:
         ops.setImageOffset(0, int(ds.height()));
         ops.setAngleNumber(As, p_.detector.module_number);
         check(xpr_fill(stream_, vol.get(), st, 1.0f, V), "fill");
+        {
+            // tile shapes timed on 32 central layers of the all-1.0 volume
+            const int64_t l0 = std::max<int64_t>(0, H / 2 - 16), l1 = std::min<int64_t>(H, l0 + 32);
+            xpr_proj_args ta = ops.args();
+            ta.vmin[1] = int32_t(l0);
+            ta.vmax[1] = int32_t(l1);
+            for (int i = 0; i < 3; ++i) ta.origin[i] = ds.orig()[size_t(i)];
+            tune_shapes(static_cast<char *>(vol.get()) + size_t(l0) * size_t(W * W) * dtype_bytes(st), st,
+                        sums.get(), err.get(), st, pds[0].as<float>(), ta);
+        }
         fill_sums(stream_, sums, p_.xpr.accumulator, 1ull << 22, uint64_t(As * P));
         upload(0);
         int vdt = st;

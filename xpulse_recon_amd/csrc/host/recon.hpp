@@ -214,6 +214,11 @@ protected:
     float next_weight(int64_t mit) const;
     void note(const std::string &k, Json v) { record_.set(k, std::move(v)); }
     void log_sit(int64_t mit, int64_t sit, double ms);
+    // Times the projectors' tile shapes (xpr_tune_*) for launches of this
+    // shape, once per shape in the process, when xpr.autotune is set.  vol_slab
+    // must hold 1.0 (it keeps it: the error used is 0); sums is overwritten.
+    void tune_shapes(void *vol_slab, int vdt, void *sums, void *err, int edt, const float *pd,
+                     const xpr_proj_args &a);
 
     Parameters &p_;
     std::unique_ptr<Dataset> ds_;

@@ -22,7 +22,11 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
+#include <type_traits>
 
 #include "../../include/xpr_hip.h"
 #include "batch.hpp"
@@ -64,11 +68,13 @@ template <typename T> using Glob = __attribute__((address_space(1))) T;
 
 struct F32 {
     using T = float;
+    static constexpr int code = XPR_F32;
     static __device__ __forceinline__ float load(const T *p, uint64_t i) { return ((const Glob<T> *)p)[i]; }
     static __device__ __forceinline__ void store(T *p, uint64_t i, float v) { ((Glob<T> *)p)[i] = v; }
 };
 struct F16 {
     using T = uint16_t;
+    static constexpr int code = XPR_F16;
     static __device__ __forceinline__ float load(const T *p, uint64_t i)
     {
         return (float)__builtin_bit_cast(_Float16, ((const Glob<T> *)p)[i]);
@@ -80,6 +86,7 @@ struct F16 {
 };
 struct BF16 {
     using T = uint16_t;
+    static constexpr int code = XPR_BF16;
     static __device__ __forceinline__ float load(const T *p, uint64_t i)
     {
         return __builtin_bit_cast(float, (uint32_t)((const Glob<T> *)p)[i] << 16);
@@ -1119,23 +1126,113 @@ int launch_rows(SlabShape s, int J, Launch &&launch)
     return XPR_OK;
 }
 
-template <typename VS, typename ES>
-int backward_impl(hipStream_t st, void *vol, const void *err, const float *pd,
-                  const xpr_proj_args &a, SlabShape s, uint64_t *empty)
+// ---------------------------------------------------------------------------
+// Tile-shape table: the backward's voxels per lane (J) and the forward's tile
+// capacity per launch shape, set by xpr_tune_* from timings on the caller's
+// buffers.  A shape with no entry uses the defaults below.  The choice moves
+// work between lanes and paths only: results are the same integers and the
+// same per-voxel arithmetic whatever is chosen.
+// ---------------------------------------------------------------------------
+enum TuneKind { kTuneForward = 0, kTuneBackward = 1 };
+using TuneKey = std::tuple<int, int, int, int, int32_t, int32_t, uint32_t, uint32_t, uint32_t,
+                           int32_t, int>;    // kind, device, dtypes (2), nx, nk, views, modules,
+                                             // det_width, line_rows, crop
+std::mutex g_tune_mu;
+std::map<TuneKey, int> g_tune;
+
+TuneKey tune_key(int kind, int d0, int d1, const xpr_proj_args &a, const SlabShape &s)
 {
-    if ((uint64_t)a.det_width * (uint64_t)a.line_rows * sizeof(typename ES::T) >= 0x80000000ull)
-        return fail(XPR_E_INVALID, "a detector view above 2 GiB");
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    return TuneKey(kind, dev, d0, d1, s.nx, s.nk, a.n_views, a.modules, a.det_width, a.line_rows,
+                   a.line_offset != 0 ? 1 : 0);
+}
+
+int tune_lookup(const TuneKey &k)
+{
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune.find(k);
+    return it == g_tune.end() ? -1 : it->second;
+}
+
+void tune_store(const TuneKey &k, int v)
+{
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune[k] = v;
+}
+
+// median-free timing: the best of `reps` launches after one warm-up launch
+template <typename F>
+int time_best(hipStream_t st, int reps, F &&launch, float *best)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    XPR_HIP_CHECK(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(XPR_E_HIP, "event");
+    }
+    int rc = launch();
+    *best = 3.4e38f;
+    for (int r = 0; rc == XPR_OK && r < reps; ++r) {
+        (void)hipEventRecord(e0, st);
+        rc = launch();
+        (void)hipEventRecord(e1, st);
+        float ms = 0.0f;
+        if (rc == XPR_OK && hipEventSynchronize(e1) == hipSuccess &&
+            hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+            *best = std::min(*best, ms);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+constexpr int kBwdJs[] = {2, 4, 8, 16};
+
+// default voxels per lane: one 64*J segment should not leave most lanes idle
+int backward_default_j(int32_t nx) { return nx <= 128 ? 2 : nx <= 256 ? 4 : kBwdJ; }
+
+template <typename VS, typename ES, int J>
+int backward_launch(hipStream_t st, void *vol, const void *err, const float *pd,
+                    const xpr_proj_args &a, SlabShape s, uint64_t *empty)
+{
     const bool crop = a.line_offset != 0;
-    return launch_rows(s, kBwdJ, [&](unsigned blocks, const SlabShape &t) {
+    return launch_rows(s, J, [&](unsigned blocks, const SlabShape &t) {
         if (crop)
-            hipLaunchKernelGGL((k_backward<VS, ES, kBwdJ, true>), dim3(blocks), dim3(kThreads), 0,
+            hipLaunchKernelGGL((k_backward<VS, ES, J, true>), dim3(blocks), dim3(kThreads), 0,
                                st, (typename VS::T *)vol, (const typename ES::T *)err, pd, a, t,
                                (unsigned long long *)empty);
         else
-            hipLaunchKernelGGL((k_backward<VS, ES, kBwdJ, false>), dim3(blocks), dim3(kThreads), 0,
+            hipLaunchKernelGGL((k_backward<VS, ES, J, false>), dim3(blocks), dim3(kThreads), 0,
                                st, (typename VS::T *)vol, (const typename ES::T *)err, pd, a, t,
                                (unsigned long long *)empty);
     });
+}
+
+// J < 0: the table's choice for this shape, else the default
+template <typename VS, typename ES>
+int backward_impl(hipStream_t st, void *vol, const void *err, const float *pd,
+                  const xpr_proj_args &a, SlabShape s, uint64_t *empty, int J = -1)
+{
+    if ((uint64_t)a.det_width * (uint64_t)a.line_rows * sizeof(typename ES::T) >= 0x80000000ull)
+        return fail(XPR_E_INVALID, "a detector view above 2 GiB");
+    if constexpr (std::is_same<VS, ES>::value) {       // the schedules' dtype pairs: all J compiled
+        if (J < 0) {
+            // tuning knob: force the voxels per lane
+            if (const char *e = std::getenv("XPR_BACKWARD_J")) J = std::atoi(e);
+            else J = tune_lookup(tune_key(kTuneBackward, VS::code, ES::code, a, s));
+        }
+        if (J < 0) J = backward_default_j(s.nx);
+        switch (J) {
+        case 2: return backward_launch<VS, ES, 2>(st, vol, err, pd, a, s, empty);
+        case 4: return backward_launch<VS, ES, 4>(st, vol, err, pd, a, s, empty);
+        case 16: return backward_launch<VS, ES, 16>(st, vol, err, pd, a, s, empty);
+        default: return backward_launch<VS, ES, kBwdJ>(st, vol, err, pd, a, s, empty);
+        }
+    } else {
+        (void)J;
+        return backward_launch<VS, ES, kBwdJ>(st, vol, err, pd, a, s, empty);
+    }
 }
 
 int forward_tile_cap()
@@ -1146,9 +1243,12 @@ int forward_tile_cap()
     return c < 0 ? 0 : (c > kTileCap ? kTileCap : c);
 }
 
+constexpr int kTileCaps[] = {384, 768, 1536};
+
+// cap < 0: the table's choice for this shape, else the default (XPR_FORWARD_TILE_CAP or kTileCap)
 template <typename VS, typename AccT>
 int forward_impl(hipStream_t st, const void *vol, void *sums, const float *pd,
-                 const xpr_proj_args &a, SlabShape s)
+                 const xpr_proj_args &a, SlabShape s, int cap = -1)
 {
     BrickShape b;
     b.nx = s.nx;
@@ -1158,7 +1258,9 @@ int forward_impl(hipStream_t st, const void *vol, void *sums, const float *pd,
     b.nbk = (s.nk + kBK - 1) / kBK;
     const int64_t nbl = (s.nl + kWaves - 1) / kWaves;
     b.nbricks = (int64_t)b.nbx * b.nbk * nbl;
-    b.cap = forward_tile_cap();
+    if (cap < 0 && !std::getenv("XPR_FORWARD_TILE_CAP"))
+        cap = tune_lookup(tune_key(kTuneForward, VS::code, (int)sizeof(AccT), a, s));
+    b.cap = cap < 0 ? forward_tile_cap() : std::min(cap, kTileCap);
     const bool crop = a.line_offset != 0;
     const int64_t max_blocks = (int64_t(1) << 31) / kThreads;
     for (int64_t b0 = 0; b0 < b.nbricks; b0 += max_blocks) {
@@ -1638,6 +1740,91 @@ int xpr_backward(xpr_stream s, void *volume, int vol_dtype, const void *err, int
     default: return XPR_BW(BF16, BF16);
     }
 #undef XPR_BW
+}
+
+// ---- tile-shape tuning --------------------------------------------------------
+int xpr_tune_forward(xpr_stream s, const void *volume, int vol_dtype, void *sums_scratch,
+                     int acc_bits, const float *projdata, const xpr_proj_args *a, int *choice)
+{
+    SlabShape sh;
+    if (int rc = check_args(a, sh)) return rc;
+    if (int rc = check_dtype(vol_dtype)) return rc;
+    if (acc_bits != XPR_ACC_U32 && acc_bits != XPR_ACC_U64)
+        return fail(XPR_E_INVALID, "acc_bits must be 32 or 64");
+    if (!volume || !sums_scratch || !projdata || !choice) return fail(XPR_E_INVALID, "null buffer");
+    if (a->n_views == 0 || sh.nx == 0 || sh.nrows == 0) return fail(XPR_E_INVALID, "empty launch");
+    hipStream_t st = (hipStream_t)s;
+    float best = 3.4e38f;
+    int pick = kTileCap;
+    for (int cap : kTileCaps) {
+        auto launch = [&]() {
+            if (acc_bits == XPR_ACC_U32) {
+                switch (vol_dtype) {
+                case XPR_F32: return forward_impl<F32, uint32_t>(st, volume, sums_scratch, projdata, *a, sh, cap);
+                case XPR_F16: return forward_impl<F16, uint32_t>(st, volume, sums_scratch, projdata, *a, sh, cap);
+                default: return forward_impl<BF16, uint32_t>(st, volume, sums_scratch, projdata, *a, sh, cap);
+                }
+            }
+            switch (vol_dtype) {
+            case XPR_F32: return forward_impl<F32, uint64_t>(st, volume, sums_scratch, projdata, *a, sh, cap);
+            case XPR_F16: return forward_impl<F16, uint64_t>(st, volume, sums_scratch, projdata, *a, sh, cap);
+            default: return forward_impl<BF16, uint64_t>(st, volume, sums_scratch, projdata, *a, sh, cap);
+            }
+        };
+        float ms = 0.0f;
+        if (int rc = time_best(st, 2, launch, &ms)) return rc;
+        if (ms < best) {
+            best = ms;
+            pick = cap;
+        }
+    }
+    const int acc_bytes = acc_bits / 8;
+    tune_store(tune_key(kTuneForward, vol_dtype, acc_bytes, *a, sh), pick);
+    *choice = pick;
+    return XPR_OK;
+}
+
+int xpr_tune_backward(xpr_stream s, void *volume_scratch, int vol_dtype, const void *err,
+                      int err_dtype, const float *projdata, const xpr_proj_args *a, int *choice)
+{
+    SlabShape sh;
+    if (int rc = check_args(a, sh)) return rc;
+    if (int rc = check_dtype(vol_dtype)) return rc;
+    if (int rc = check_dtype(err_dtype)) return rc;
+    if (!volume_scratch || !err || !projdata || !choice) return fail(XPR_E_INVALID, "null buffer");
+    if (sh.nx == 0 || sh.nrows == 0) return fail(XPR_E_INVALID, "empty launch");
+    if (vol_dtype != err_dtype) {                // one shape compiled for mixed storage pairs
+        *choice = kBwdJ;
+        return XPR_OK;
+    }
+    hipStream_t st = (hipStream_t)s;
+    float best = 3.4e38f;
+    int pick = kBwdJ;
+    for (int J : kBwdJs) {
+        auto launch = [&]() {
+            switch (vol_dtype) {
+            case XPR_F32: return backward_impl<F32, F32>(st, volume_scratch, err, projdata, *a, sh, nullptr, J);
+            case XPR_F16: return backward_impl<F16, F16>(st, volume_scratch, err, projdata, *a, sh, nullptr, J);
+            default: return backward_impl<BF16, BF16>(st, volume_scratch, err, projdata, *a, sh, nullptr, J);
+            }
+        };
+        float ms = 0.0f;
+        if (int rc = time_best(st, 2, launch, &ms)) return rc;
+        if (ms < best) {
+            best = ms;
+            pick = J;
+        }
+    }
+    tune_store(tune_key(kTuneBackward, vol_dtype, err_dtype, *a, sh), pick);
+    *choice = pick;
+    return XPR_OK;
+}
+
+int xpr_tune_reset(void)
+{
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune.clear();
+    return XPR_OK;
 }
 
 int xpr_q22_to_float(xpr_stream s, const void *sums, int acc_bits, float *out, uint64_t n)
