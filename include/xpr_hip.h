@@ -245,6 +245,15 @@ int xpr_synth_phantom(xpr_stream s, double vx, int64_t vwidth, int64_t vheight,
  * -2 parameters, -3 reconstruction.  If record is non-NULL it receives the
  * JSON run record (truncated to record_cap). */
 int xpr_reconstruct(const char *params, char *record, size_t record_cap);
+/* xpr_reconstruct with the projections and the volume in host memory instead
+ * of TIFF files (recon.input / recon.output are not used; reprojections still
+ * go to recon.proj_output): projections [count][height][width] raw (as read
+ * from the TIFFs, sorted by name), volume_out [vheight][vwidth][vwidth]
+ * float32, de-normalised like the written layers.  Any schedule of
+ * recon.method. */
+int xpr_reconstruct_mem(const char *params, const float *projections, int64_t width,
+                        int64_t height, int64_t count, float *volume_out, char *record,
+                        size_t record_cap);
 /* The default parameters as JSON (XPRecon -template).  0 on success, -1 if
  * cap is too small. */
 int xpr_template_json(char *out, size_t cap);

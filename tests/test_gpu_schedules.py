@@ -164,3 +164,31 @@ def test_bad_tiff_is_a_reconstruction_error(tmp_path, cuda):
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 256 - 3
     assert "incompatble format" in r.stderr
+
+
+@pytest.mark.parametrize("method,xpr", [("naive", {}),
+                                        ("method_2", {"device_budget_gb": 0.30 * (W * W * H * 4 + 6 * W * H * 16) / 1e9}),
+                                        ("method_1a", {"slab_height": 9, "layer_store": "ram"})])
+def test_reconstruct_mem_equals_cli(scan, method, xpr):
+    """xpr_reconstruct_mem (projections and volume in host memory) gives the
+    CLI's layers bit for bit, for the resident, chunked and slab schedules."""
+    import ctypes as C
+    from xpulse_recon_amd import _lib
+    t, spec, d = scan
+    ref, log = _run(t, f"cli_{method}", d, method=method, xpr=dict(xpr))
+    raw = np.stack([read_f32(d / f"proj_{a:04d}.tif") for a in range(N)])
+    params = {"recon": dict(it=2, sit=4, method=method),
+              "geometry": {"so": 100.0, "sd": 1000.0}, "detector": {"px": 0.1, "rx": 1.5},
+              "xpr": dict(xpr, jitter_seed=11)}
+    vol = np.empty_like(ref)
+    rec = C.create_string_buffer(1 << 16)
+    print(f"xpr_reconstruct_mem {method}", flush=True)
+    rc = _lib.load().xpr_reconstruct_mem(json.dumps(params).encode(),
+                                         raw.ctypes.data_as(C.c_void_p), W, H, N,
+                                         vol.ctypes.data_as(C.c_void_p), rec, len(rec))
+    assert rc == 0
+    record = json.loads(rec.value.decode())
+    assert record["jitter"] == log["jitter"]
+    if method == "method_2":
+        assert len(record["chunks"]) >= 2
+    np.testing.assert_array_equal(vol, ref)

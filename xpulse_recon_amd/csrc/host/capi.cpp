@@ -1,5 +1,6 @@
 // In-process entry to the XPRecon drop-in (the CLI's body, src/main.cpp:69-137),
 // for callers that load the library instead of spawning the executable.
+#include <cstdint>
 #include <cstring>
 #include <iostream>
 #include <string>
@@ -8,7 +9,10 @@
 #include "params.hpp"
 #include "recon.hpp"
 
-extern "C" int xpr_reconstruct(const char *params, char *record, size_t record_cap)
+namespace {
+
+int run(const char *params, const float *projections, int64_t width, int64_t height,
+        int64_t count, float *volume_out, char *record, size_t record_cap)
 {
     if (record && record_cap) record[0] = '\0';
     if (!params) return -1;                                  // RETURN_CODE_ERROR_ARGUMENTS
@@ -18,6 +22,13 @@ extern "C" int xpr_reconstruct(const char *params, char *record, size_t record_c
     } catch (const std::exception &e) {
         std::cerr << e.what() << std::endl;
         return -2;                                           // RETURN_CODE_ERROR_PARAMETERS
+    }
+    if (projections) {                                       // in-memory source and sink
+        p.xpr.mem_projections = projections;
+        p.xpr.mem_width = width;
+        p.xpr.mem_height = height;
+        p.xpr.mem_count = count;
+        p.xpr.mem_volume_out = volume_out;
     }
     try {
         auto recon = xpr::make_reconstruction(p);
@@ -33,6 +44,21 @@ extern "C" int xpr_reconstruct(const char *params, char *record, size_t record_c
         return -3;                                           // RETURN_CODE_ERROR_RECONSTRUCTION
     }
     return 0;
+}
+
+} // namespace
+
+extern "C" int xpr_reconstruct(const char *params, char *record, size_t record_cap)
+{
+    return run(params, nullptr, 0, 0, 0, nullptr, record, record_cap);
+}
+
+extern "C" int xpr_reconstruct_mem(const char *params, const float *projections, int64_t width,
+                                   int64_t height, int64_t count, float *volume_out, char *record,
+                                   size_t record_cap)
+{
+    if (!projections || !volume_out || width <= 0 || height <= 0 || count <= 0) return -1;
+    return run(params, projections, width, height, count, volume_out, record, record_cap);
 }
 
 extern "C" int xpr_template_json(char *out, size_t cap)

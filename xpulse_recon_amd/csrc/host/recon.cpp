@@ -203,7 +203,7 @@ Dataset::Dataset(Parameters &p) : p_(p)
     vwidth_ = grid.vwidth;
     vheight_ = grid.vheight;
     for (int i = 0; i < 3; ++i) orig_[i] = float(grid.orig[i]);
-    if (!p.xpr.mem_volume_out) {
+    if (!p.xpr.mem_volume_out || p.xpr.layer_store == "zstd") {   // layers or spill files
         fs::create_directories(p.recon.output);
         if (!p.recon.proj_output.empty()) fs::create_directories(p.recon.proj_output);
         proj_dir_ = (fs::path(p.recon.output) / "projs").string();
