@@ -267,7 +267,7 @@ __device__ __forceinline__ float load_or_zero<BF16>(__amdgpu_buffer_rsrc_t r, ui
 // ---------------------------------------------------------------------------
 constexpr int kWaves = 4;
 constexpr int kThreads = 64 * kWaves;
-constexpr int kBwdJ = 8;          // voxels per lane of the backward
+constexpr int kBwdJ = 4;          // voxels per lane of the backward (default shape)
 
 struct SlabShape {
     int32_t nx, nl, nk;
@@ -1189,8 +1189,9 @@ int time_best(hipStream_t st, int reps, F &&launch, float *best)
 
 constexpr int kBwdJs[] = {2, 4, 8, 16};
 
-// default voxels per lane: one 64*J segment should not leave most lanes idle
-int backward_default_j(int32_t nx) { return nx <= 128 ? 2 : nx <= 256 ? 4 : kBwdJ; }
+// default voxels per lane (J = 4 measured best at 768^3 / 36 views among 2, 4,
+// 8, 16); narrow slabs take J = 2 so one 64*J segment leaves few lanes idle
+int backward_default_j(int32_t nx) { return nx <= 128 ? 2 : kBwdJ; }
 
 template <typename VS, typename ES, int J>
 int backward_launch(hipStream_t st, void *vol, const void *err, const float *pd,
@@ -1226,6 +1227,7 @@ int backward_impl(hipStream_t st, void *vol, const void *err, const float *pd,
         switch (J) {
         case 2: return backward_launch<VS, ES, 2>(st, vol, err, pd, a, s, empty);
         case 4: return backward_launch<VS, ES, 4>(st, vol, err, pd, a, s, empty);
+        case 8: return backward_launch<VS, ES, 8>(st, vol, err, pd, a, s, empty);
         case 16: return backward_launch<VS, ES, 16>(st, vol, err, pd, a, s, empty);
         default: return backward_launch<VS, ES, kBwdJ>(st, vol, err, pd, a, s, empty);
         }
