@@ -1374,17 +1374,25 @@ int capture(Engine &e, int k, int cus)
 {
     XPR_HIP_CHECK(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
     const dim3 blk(kThreads);
-    // grids are multiples of kXcds, so every XCD residue class has workgroups
-    auto grid = [&](int per_cu) { return dim3((unsigned)(((cus * per_cu + 7) / 8) * 8)); };
+    // grids are multiples of kXcds, so every XCD residue class has workgroups.
+    // Workgroups per CU: the projectors fit 4 (forward: 128 VGPRs, 28 KB LDS)
+    // and 7 (backward) workgroups of 4 waves per CU; the defaults leave one
+    // slot per CU free so the small launches of jobs running beside the steps
+    // start at once (XPR_SERVE_SLOTS="forward,backward,elementwise" sets them).
+    int per_cu[3] = {3, 6, 2};
+    if (const char *v = std::getenv("XPR_SERVE_SLOTS"))
+        std::sscanf(v, "%d,%d,%d", &per_cu[0], &per_cu[1], &per_cu[2]);
+    for (int &q : per_cu) q = std::max(1, std::min(q, 16));
+    auto grid = [&](int per) { return dim3((unsigned)(((cus * per + 7) / 8) * 8)); };
     (void)hipMemcpyAsync(e.dev[k], e.host[k], sizeof(Table), hipMemcpyHostToDevice, e.stream);
     hipLaunchKernelGGL(k_batch_reset, dim3(1), blk, 0, e.stream, e.counters[k]);
-    hipLaunchKernelGGL(k_batch_elementwise, grid(4), blk, 0, e.stream, e.dev[k], e.counters[k],
+    hipLaunchKernelGGL(k_batch_elementwise, grid(per_cu[2]), blk, 0, e.stream, e.dev[k], e.counters[k],
                        (uint32_t)kConvert);
-    hipLaunchKernelGGL(k_batch_forward, grid(4), blk, 0, e.stream, e.dev[k], e.counters[k]);
-    hipLaunchKernelGGL(k_batch_elementwise, grid(4), blk, 0, e.stream, e.dev[k], e.counters[k],
+    hipLaunchKernelGGL(k_batch_forward, grid(per_cu[0]), blk, 0, e.stream, e.dev[k], e.counters[k]);
+    hipLaunchKernelGGL(k_batch_elementwise, grid(per_cu[2]), blk, 0, e.stream, e.dev[k], e.counters[k],
                        (uint32_t)kError);
-    hipLaunchKernelGGL(k_batch_backward, grid(8), blk, 0, e.stream, e.dev[k], e.counters[k]);
-    hipLaunchKernelGGL(k_batch_elementwise, grid(4), blk, 0, e.stream, e.dev[k], e.counters[k],
+    hipLaunchKernelGGL(k_batch_backward, grid(per_cu[1]), blk, 0, e.stream, e.dev[k], e.counters[k]);
+    hipLaunchKernelGGL(k_batch_elementwise, grid(per_cu[2]), blk, 0, e.stream, e.dev[k], e.counters[k],
                        (uint32_t)kFill);
     const hipError_t le = hipGetLastError();
     const hipError_t ce = hipStreamEndCapture(e.stream, &e.graph[k]);
