@@ -372,3 +372,44 @@ def test_tuned_shapes_keep_results(cuda):
     np.testing.assert_array_equal(s1, s0)
     np.testing.assert_array_equal(v1, v0)
     X.tune_reset()
+
+
+@pytest.mark.parametrize("acc", [32, 64])
+def test_forward_exact_views_equal_checked(cuda, monkeypatch, acc):
+    """Views whose rounding bound lets the tile test decide the bounds (the
+    exact-tile splat) give the sums of the canonical bounds test bit for bit,
+    on a 160^2 x 96 slab with 24 views (most bricks exact, edge bricks not),
+    a cropped row band and the oracle."""
+    import torch
+    import xpulse_recon_amd as X
+    spec = small_spec(160, 96, 48, rx=1.5)
+    views = list(range(0, 48, 2))
+    pd, grid = projdata_for(spec, views)
+    d = geom_args(spec, grid, len(views), jitter=(0.003, 0.001, 0.006))
+    rng = np.random.default_rng(42)
+    W, H = grid["vwidth"], grid["vheight"]
+    vol = rng.uniform(0.0, 1.5, (H, W, W)).astype(np.float32)
+    shape = (d["n_views"], d["line_rows"], d["det_width"])
+    tdt = torch.int32 if acc == 32 else torch.int64
+    tv, tp = torch.from_numpy(vol).to(cuda), torch.from_numpy(pd).to(cuda)
+    out = {}
+    for ex in ("1", "0"):
+        monkeypatch.setenv("XPR_FORWARD_EXACT", ex)
+        s = torch.zeros(shape, dtype=tdt, device=cuda)
+        X.forward(tv, s, tp, _args(d))
+        out[ex] = s.cpu().numpy()
+    np.testing.assert_array_equal(out["1"], out["0"])
+    ref = np.zeros(shape, np.uint32 if acc == 32 else np.uint64)
+    O.forward_f32(vol, ref, pd, oracle_geom(d), 8)
+    np.testing.assert_array_equal(out["1"].view(ref.dtype), ref)
+    # a cropped band of detector rows
+    dc = geom_args(spec, grid, len(views), jitter=(0.003, 0.001, 0.006), line_offset=30,
+                   line_rows=40)
+    shape_c = (dc["n_views"], dc["line_rows"], dc["det_width"])
+    out = {}
+    for ex in ("1", "0"):
+        monkeypatch.setenv("XPR_FORWARD_EXACT", ex)
+        s = torch.zeros(shape_c, dtype=tdt, device=cuda)
+        X.forward(tv, s, tp, _args(dc))
+        out[ex] = s.cpu().numpy()
+    np.testing.assert_array_equal(out["1"], out["0"])

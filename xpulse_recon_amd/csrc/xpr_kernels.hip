@@ -417,6 +417,7 @@ struct BrickShape {
     int32_t nx, nl, nk;      // slab extent (x, layers, depths)
     int32_t nbx, nbk;        // bricks along x and depth
     int32_t cap;             // tile entries in use (<= kTileCap); 0 forces the direct path
+    int32_t exact;           // 1: views may take the exact-tile splat (kBinExact)
     int64_t brick_base;      // first brick of this launch
     int64_t nbricks;         // bricks in the slab
 };
@@ -432,6 +433,7 @@ enum : uint32_t {
     kBinTile = 2u,     // splat into an LDS tile (else direct global atomics)
     kBinFast = 4u,     // every voxel's W is inside the fast reciprocal's range
     kBinDepth = 8u,    // depth-lane order
+    kBinExact = 16u,   // the tile test alone decides the canonical bounds (see bin_view)
 };
 
 struct ViewBin {                   // one (view, module) record of the brick, in LDS
@@ -443,10 +445,25 @@ struct ViewBin {                   // one (view, module) record of the brick, in
 
 // Footprint of the brick for view record m; run by 8 consecutive lanes (one
 // per corner: bit 0 x, bit 1 layer, bit 2 depth), written by the first.
+//
+// The tile is the footprint with one pixel of margin, clipped to the columns
+// and rows the canonical bounds accept (integers n with ceil(lo) <= n <
+// ceil(hi)), so every tile entry is in bounds.  Exact views: the voxel
+// positions are monotone in their indices, so the exact projection of every
+// voxel of the brick lies between the corners' (W > 0 over the box); with
+//   du = 8 eps ((Tx + U Tw) / Wmin + U),  eps = 2^-24,
+// bounding the rounding of the three FMA dot products, the reciprocal and the
+// product for any voxel or corner (Tx, Tw: sums of |coefficient x coordinate|
+// over the box; U = max |u|; Wmin = min W), every computed u lies within 2 du
+// of [umin, umax].  With 2 du <= 0.5 every in-bounds voxel-view rounds into
+// the tile, and a voxel-view in the tile is in bounds: v_cvt_rpi equals roundf
+// for u > -0.5, which every tile column >= 1 implies.  So when du, dv <= 0.25
+// and the tile starts at column and row >= 1, the tile test alone is the
+// canonical bounds test.
 template <bool CROP>
 __device__ __forceinline__ void bin_view(const float *__restrict__ pd, uint32_t m, bool live,
                                          int corner, const int32_t (&lo)[3],
-                                         const int32_t (&hi)[3], int cap,
+                                         const int32_t (&hi)[3], int cap, bool exact_ok,
                                          const xpr_proj_args &a, ViewBin *__restrict__ out)
 {
     const float *__restrict__ p = pd + 20u * m;
@@ -464,6 +481,10 @@ __device__ __forceinline__ void bin_view(const float *__restrict__ pd, uint32_t 
     // every voxel W of the brick lies between the corner values (fast_rcp_box)
     const float tmag = fabsf(p[12] * cpx) + fabsf(p[13] * cpy) + fabsf(p[14] * cpz) + fabsf(p[15]);
     int fast = fast_rcp_box(Wc, tmag);
+    // magnitudes for the rounding bound (maxima over the corners)
+    float tx = fabsf(p[0] * cpx) + fabsf(p[1] * cpy) + fabsf(p[2] * cpz) + fabsf(p[3]);
+    float ty = fabsf(p[4] * cpx) + fabsf(p[5] * cpy) + fabsf(p[6] * cpz) + fabsf(p[7]);
+    float tw = tmag, wmin = Wc;
     // column spread per voxel step along x (corner 1) and along depth (corner 4)
     const int base = (int)(threadIdx.x & 63) & ~7;     // lane of corner 0
     const float du_x = fabsf(__shfl(u, base + 1) - __shfl(u, base)) / (float)max(hi[0] - lo[0], 1);
@@ -474,6 +495,10 @@ __device__ __forceinline__ void bin_view(const float *__restrict__ pd, uint32_t 
         umax = fmaxf(umax, __shfl_xor(umax, off));
         vmin = fminf(vmin, __shfl_xor(vmin, off));
         vmax = fmaxf(vmax, __shfl_xor(vmax, off));
+        tx = fmaxf(tx, __shfl_xor(tx, off));
+        ty = fmaxf(ty, __shfl_xor(ty, off));
+        tw = fmaxf(tw, __shfl_xor(tw, off));
+        wmin = fminf(wmin, __shfl_xor(wmin, off));
         ok &= __shfl_xor(ok, off);
         fast &= __shfl_xor(fast, off);
     }
@@ -488,16 +513,24 @@ __device__ __forceinline__ void bin_view(const float *__restrict__ pd, uint32_t 
     b.vlo = lo_threshold(vlo);
     b.vhi = hi_threshold(vhi);
     if (ok) {
-        b.u0 = (int32_t)fmaxf(floorf(umin) - 1.0f, vpl);
-        b.v0 = (int32_t)fmaxf(floorf(vmin) - 1.0f, vlo);
-        const int32_t u1 = (int32_t)fminf(ceilf(umax) + 1.0f, vpr - 1.0f);
-        const int32_t v1 = (int32_t)fminf(ceilf(vmax) + 1.0f, vhi - 1.0f);
+        b.u0 = (int32_t)fmaxf(floorf(umin) - 1.0f, ceilf(vpl));
+        b.v0 = (int32_t)fmaxf(floorf(vmin) - 1.0f, ceilf(vlo));
+        const int32_t u1 = (int32_t)fminf(ceilf(umax) + 1.0f, ceilf(vpr) - 1.0f);
+        const int32_t v1 = (int32_t)fminf(ceilf(vmax) + 1.0f, ceilf(vhi) - 1.0f);
         b.tw = u1 - b.u0 + 1;
         b.th = v1 - b.v0 + 1;
-        if (b.tw <= 0 || b.th <= 0)
+        if (b.tw <= 0 || b.th <= 0) {
             flags |= kBinSkip;
-        else if (b.tw * b.th <= cap)
+        } else if (b.tw * b.th <= cap) {
             flags |= kBinTile;
+            constexpr float eps8 = 8.0f / 16777216.0f;
+            const float U = fmaxf(fabsf(umin), fabsf(umax)) + 1.0f;
+            const float V = fmaxf(fabsf(vmin), fabsf(vmax)) + fabsf(line_off) + 1.0f;
+            const float du = eps8 * ((tx + U * tw) / wmin + U);
+            const float dv = eps8 * ((ty + V * tw) / wmin + V);
+            if (exact_ok && fast && du <= 0.25f && dv <= 0.25f && b.u0 >= 1 && b.v0 >= 1)
+                flags |= kBinExact;
+        }
     }
     b.flags = flags;
     *out = b;
@@ -570,6 +603,27 @@ __device__ __forceinline__ void splat_one(const ViewConsts &c, float px, uint32_
     atomicAdd(&tl[hit ? e : sink_e], (unsigned long long)(hit ? qq : 0u));
     if (inb & !hit & (qq != 0u))      // off the tile: direct global add
         global_add(vsum + (__umul24(iv, det_width) + iu), (AccT)qq);
+}
+
+// one voxel-view of an exact view (kBinExact): the tile test is the bounds test
+template <bool CROP>
+__device__ __forceinline__ void splat_exact(const ViewConsts &c, float px, uint32_t qq,
+                                            float line_off, int32_t u0, int32_t v0, int32_t tw,
+                                            int32_t th, unsigned long long *__restrict__ tl,
+                                            uint32_t sink_e)
+{
+    const float X = __builtin_fmaf(c.c0x, px, c.k0);
+    const float Y = __builtin_fmaf(c.c1x, px, c.k1);
+    const float W = __builtin_fmaf(c.c3x, px, c.k3);
+    const float r0 = __builtin_amdgcn_rcpf(W);
+    const float r = __builtin_fmaf(__builtin_fmaf(-W, r0, 1.0f), r0, r0);
+    const float u = X * r;
+    const float v = CROP ? Y * r - line_off : Y * r;
+    const uint32_t tu = (uint32_t)pixel_of(u) - (uint32_t)u0;
+    const uint32_t tv = (uint32_t)pixel_of(v) - (uint32_t)v0;
+    const bool hit = (tu < (uint32_t)tw) & (tv < (uint32_t)th);
+    const uint32_t e = __umul24(tv, (uint32_t)tw) + tu;
+    atomicAdd(&tl[hit ? e : sink_e], (unsigned long long)(hit ? qq : 0u));
 }
 
 // a uniform float copied to a VGPR, so that an FMA taking it and a value read
@@ -666,8 +720,8 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
         {
             const uint32_t vi = threadIdx.x >> 3;
             const bool live = vi < cnt;
-            bin_view<CROP>(pd, live ? c0 + vi : c0, live, threadIdx.x & 7, lo, hi, s.cap, a,
-                           &bins[vi]);
+            bin_view<CROP>(pd, live ? c0 + vi : c0, live, threadIdx.x & 7, lo, hi, s.cap,
+                           s.exact != 0, a, &bins[vi]);
         }
         __syncthreads();
         for (uint32_t vi = 0; vi < cnt; ++vi) {
@@ -691,7 +745,25 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
             c.vhi = bins[vi].vhi;
             row_terms(p, py, pz_d, c);
             const uint32_t dw = a.det_width;
-            if (flags & kBinDepth) {
+            if (flags & kBinExact) {
+                if (flags & kBinDepth) {
+#pragma unroll
+                    for (int j = 0; j < kBK; ++j)
+                        splat_exact<CROP>(c, px_d[j], qd[j], line_off, u0, v0, tw, th, tl, sk);
+                } else {
+                    ViewConsts ci = c;
+                    ci.c0x = to_vgpr(c.c0x);
+                    ci.c1x = to_vgpr(c.c1x);
+                    ci.c3x = to_vgpr(c.c3x);
+#pragma unroll
+                    for (int i = 0; i < kBK; ++i) {
+                        ci.k0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.k0), i));
+                        ci.k1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.k1), i));
+                        ci.k3 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.k3), i));
+                        splat_exact<CROP>(ci, px_x, qx[i], line_off, u0, v0, tw, th, tl, sk);
+                    }
+                }
+            } else if (flags & kBinDepth) {
                 if (flags & kBinFast) {
 #pragma unroll
                     for (int j = 0; j < kBK; ++j)
@@ -927,6 +999,7 @@ __device__ __forceinline__ BrickShape batch_brick_shape(const xpr_proj_args &a)
     s.nbx = (s.nx + 63) / 64;
     s.nbk = (s.nk + kBK - 1) / kBK;
     s.cap = kTileCap;
+    s.exact = 1;
     s.brick_base = 0;
     s.nbricks = (int64_t)s.nbx * s.nbk * ((s.nl + kWaves - 1) / kWaves);
     return s;
@@ -1263,6 +1336,9 @@ int forward_impl(hipStream_t st, const void *vol, void *sums, const float *pd,
     if (cap < 0 && !std::getenv("XPR_FORWARD_TILE_CAP"))
         cap = tune_lookup(tune_key(kTuneForward, VS::code, (int)sizeof(AccT), a, s));
     b.cap = cap < 0 ? forward_tile_cap() : std::min(cap, kTileCap);
+    // tuning knob: XPR_FORWARD_EXACT=0 keeps the canonical bounds test in every view
+    const char *ex = std::getenv("XPR_FORWARD_EXACT");
+    b.exact = (ex && std::atoi(ex) == 0) ? 0 : 1;
     const bool crop = a.line_offset != 0;
     const int64_t max_blocks = (int64_t(1) << 31) / kThreads;
     for (int64_t b0 = 0; b0 < b.nbricks; b0 += max_blocks) {
