@@ -413,3 +413,55 @@ def test_forward_exact_views_equal_checked(cuda, monkeypatch, acc):
         X.forward(tv, s, tp, _args(dc))
         out[ex] = s.cpu().numpy()
     np.testing.assert_array_equal(out["1"], out["0"])
+
+
+INSIDE_CASES = [
+    # (spec kwargs, views, line_offset/rows or None): 100 view records (two
+    # set-up chunks of 64 lanes), 2 modules x 40 views, a cropped row band
+    (dict(w=160, h=96, n=100, rx=1.5), list(range(100)), None),
+    (dict(w=96, h=40, n=40, module_number=2, module_angle_offset=6.0), list(range(40)), None),
+    (dict(w=160, h=96, n=48, rx=-2.0), list(range(0, 48, 2)), (30, 40)),
+]
+
+
+@pytest.mark.parametrize("case", range(len(INSIDE_CASES)))
+def test_backward_inside_views_equal_checked(cuda, monkeypatch, case):
+    """Row segments whose rounding bound puts every voxel in bounds skip the
+    per-voxel test: the volume equals the canonical test's bit for bit (FP32
+    at J = 4 and 16, and FP16 storage), and the oracle within the expf
+    tolerance."""
+    import torch
+    import xpulse_recon_amd as X
+    kw, views, crop = INSIDE_CASES[case]
+    kw = dict(kw)
+    spec = small_spec(kw.pop("w"), kw.pop("h"), kw.pop("n"), **kw)
+    pd, grid = projdata_for(spec, views)
+    lo, rows = crop if crop else (0, None)
+    d = geom_args(spec, grid, len(views), jitter=(0.002, 0.004, 0.001), line_offset=lo,
+                  line_rows=rows)
+    W, H = grid["vwidth"], grid["vheight"]
+    rng = np.random.default_rng(500 + case)
+    vol = rng.uniform(0.5, 2.0, (H, W, W)).astype(np.float32)
+    err = rng.normal(0.0, 0.4, (d["n_views"], d["line_rows"], d["det_width"])).astype(np.float32)
+    te, tp = torch.from_numpy(err).to(cuda), torch.from_numpy(pd).to(cuda)
+
+    def run(dtype, inside, j):
+        monkeypatch.setenv("XPR_BACKWARD_INSIDE", inside)
+        monkeypatch.setenv("XPR_BACKWARD_J", j)
+        tv = torch.from_numpy(vol).to(cuda).to(dtype)
+        empty = torch.zeros(1, dtype=torch.int64, device=cuda)
+        X.backward(tv, te.to(dtype), tp, _args(d), empty=empty)
+        return tv.float().cpu().numpy(), int(empty.item())
+
+    for j in ("4", "16"):
+        a, ea = run(torch.float32, "1", j)
+        b, eb = run(torch.float32, "0", j)
+        np.testing.assert_array_equal(a, b)
+        assert ea == eb
+    h1, _ = run(torch.float16, "1", "4")
+    h0, _ = run(torch.float16, "0", "4")
+    np.testing.assert_array_equal(h1, h0)
+    ref = vol.copy()
+    empty_ref = O.backward_f32(ref, err, pd, oracle_geom(d), 8)
+    assert ea == empty_ref
+    np.testing.assert_allclose(a, ref, rtol=4e-7, atol=0)

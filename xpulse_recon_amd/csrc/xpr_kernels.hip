@@ -190,21 +190,9 @@ __device__ __forceinline__ bool fast_rcp_box(float w, float t)
     return (w >= 1e-2f) & (w <= 1e5f) & (t <= 1e4f);
 }
 
-// row segment [px_first, px_last] at fixed (py, pz): the two ends are its corners
-__device__ __forceinline__ bool fast_rcp_range(const float *__restrict__ pg, const ViewConsts &c,
-                                               float px_first, float px_last, float py, float pz)
-{
-    ConstF *p = pd_const(pg);
-    const float w0 = __builtin_fmaf(c.c3x, px_first, c.k3);
-    const float w1 = __builtin_fmaf(c.c3x, px_last, c.k3);
-    const float t = fabsf(c.c3x) * fmaxf(fabsf(px_first), fabsf(px_last)) + fabsf(p[13] * py) +
-                    fabsf(p[14] * pz) + fabsf(p[15]);
-    return fast_rcp_box(w0, t) & fast_rcp_box(w1, t);
-}
-
 // Projects the voxel at x-position px: true when in bounds, with its pixel
 // column iu and row iv (meaningful when in bounds).  FAST: the caller has
-// checked with fast_rcp_range that W is in the fast reciprocal's range.
+// checked (row_setup) that W is in the fast reciprocal's range.
 template <bool CROP, bool FAST>
 __device__ __forceinline__ bool project(const ViewConsts &c, float px, float line_off,
                                         uint32_t &iu, uint32_t &iv)
@@ -274,9 +262,88 @@ struct SlabShape {
     int32_t nseg;          // x segments per row
     int64_t row_base;      // first row of this launch
     int64_t nrows;         // rows in the slab
+    int32_t inside = 1;    // 1: backward views wholly in bounds skip the per-voxel test
 };
 
+// Projection of a voxel known to be in bounds (fast reciprocal range checked).
+template <bool CROP>
+__device__ __forceinline__ void project_inside(const ViewConsts &c, float px, float line_off,
+                                               uint32_t &iu, uint32_t &iv)
+{
+    const float X = __builtin_fmaf(c.c0x, px, c.k0);
+    const float Y = __builtin_fmaf(c.c1x, px, c.k1);
+    const float W = __builtin_fmaf(c.c3x, px, c.k3);
+    const float r0 = __builtin_amdgcn_rcpf(W);
+    const float r = __builtin_fmaf(__builtin_fmaf(-W, r0, 1.0f), r0, r0);
+    const float u = X * r;
+    const float v = CROP ? Y * r - line_off : Y * r;
+    iu = (uint32_t)pixel_of(u);
+    iv = (uint32_t)pixel_of(v);
+}
+
+__device__ __forceinline__ float lane_float(float x, uint32_t lane)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), lane));
+}
+
+// Set-up of one (row segment, view record) pair, done by one lane per record:
+// the row terms, the thresholds, whether the fast reciprocal is exact over the
+// segment, and whether every voxel of the segment is in bounds.
+//
+// "Inside" rests on a rounding bound.  At fixed (py, pz) the exact X, Y, W of
+// the evaluated FMAs are affine in px, so with W > 0 the exact u = X/W and
+// v = Y/W are monotone along the segment and lie between their values at the
+// segment's ends.  Each computed u carries the relative error of one FMA (X),
+// one FMA (W), the correctly rounded reciprocal and one product: at most
+// 4 * 2^-24 (1 + 2^-22), and v one subtraction of the line offset more.  The
+// segment is inside when its computed end values, widened by 2^-20 (U + 1)
+// (U the largest end |u|, about four times the two ends' errors together),
+// pass the canonical half-open test.  Every voxel of the segment then passes
+// it too, so its test can be skipped without changing any result.
+struct RowSetup {
+    float k0, k1, k3, ulo, uhi, vlo, vhi;
+    bool fast, inside;
+};
+
+__device__ __forceinline__ RowSetup row_setup(const float *__restrict__ p, const xpr_proj_args &a,
+                                              float py, float pz, float px_first, float px_last,
+                                              float line_off, bool inside_ok)
+{
+    RowSetup r;
+    const float c0x = p[0], c1x = p[4], c3x = p[12];
+    r.k0 = __builtin_fmaf(p[1], py, __builtin_fmaf(p[2], pz, p[3]));
+    r.k1 = __builtin_fmaf(p[5], py, __builtin_fmaf(p[6], pz, p[7]));
+    r.k3 = __builtin_fmaf(p[13], py, __builtin_fmaf(p[14], pz, p[15]));
+    r.ulo = lo_threshold(fmaxf(p[16], 0.0f));
+    r.uhi = hi_threshold(fminf(p[17], (float)a.det_width));
+    r.vlo = lo_threshold(fmaxf(p[18], 0.0f));
+    r.vhi = hi_threshold(fminf(p[19], (float)a.line_rows));
+    const float w0 = __builtin_fmaf(c3x, px_first, r.k3);
+    const float w1 = __builtin_fmaf(c3x, px_last, r.k3);
+    const float t = fabsf(c3x) * fmaxf(fabsf(px_first), fabsf(px_last)) + fabsf(p[13] * py) +
+                    fabsf(p[14] * pz) + fabsf(p[15]);
+    r.fast = fast_rcp_box(w0, t) & fast_rcp_box(w1, t);
+    // end values, computed exactly like a voxel's (fast reciprocal)
+    const float q0 = __builtin_fmaf(__builtin_fmaf(-w0, __builtin_amdgcn_rcpf(w0), 1.0f),
+                                    __builtin_amdgcn_rcpf(w0), __builtin_amdgcn_rcpf(w0));
+    const float q1 = __builtin_fmaf(__builtin_fmaf(-w1, __builtin_amdgcn_rcpf(w1), 1.0f),
+                                    __builtin_amdgcn_rcpf(w1), __builtin_amdgcn_rcpf(w1));
+    const float u0 = __builtin_fmaf(c0x, px_first, r.k0) * q0;
+    const float u1 = __builtin_fmaf(c0x, px_last, r.k0) * q1;
+    const float y0 = __builtin_fmaf(c1x, px_first, r.k1) * q0;
+    const float y1 = __builtin_fmaf(c1x, px_last, r.k1) * q1;
+    const float v0 = y0 - line_off, v1 = y1 - line_off;
+    constexpr float kMargin = 1.0f / 1048576.0f;
+    const float mu = kMargin * (fmaxf(fabsf(u0), fabsf(u1)) + 1.0f);
+    const float mv = kMargin * (fmaxf(fabsf(y0), fabsf(y1)) + fabsf(line_off) + 1.0f);
+    r.inside = inside_ok & r.fast & (fminf(u0, u1) - mu >= r.ulo) & (fmaxf(u0, u1) + mu < r.uhi) &
+               (fminf(v0, v1) - mv >= r.vlo) & (fmaxf(v0, v1) + mv < r.vhi);
+    return r;
+}
+
 // Segment seg of one (layer, depth) row of the backward, run by one wave.
+// The per-view set-up of the row is spread over the lanes (one view record
+// each, 64 at a time) and read back with v_readlane inside the view loop.
 template <typename VS, typename ES, int J, bool CROP>
 __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
                                              const typename ES::T *__restrict__ err,
@@ -299,40 +366,70 @@ __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
         sum[j] = 0.0f;
         cnt[j] = 0;
     }
-    // first and last voxel positions of the row segment (they bound W over it)
+    // first and last voxel positions of the row segment (they bound u, v, W over it)
     const float px_first = __builtin_fmaf((float)(seg * 64 * J + a.vmin[0]), a.voxel[0], a.origin[0]);
     const float px_last = __builtin_fmaf((float)(min(seg * 64 * J + 64 * J, s.nx) - 1 + a.vmin[0]),
                                          a.voxel[0], a.origin[0]);
     const float line_off = (float)a.line_offset;
     const uint32_t view_elems = a.det_width * (uint32_t)a.line_rows;
     const uint32_t esz = (uint32_t)sizeof(typename ES::T);
-    for (uint32_t view = 0, m = 0; view < a.n_views; ++view) {
-        const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-            (void *)(err + (uint64_t)view_of(a, view) * view_elems), 0, (int)(view_elems * esz),
-            kBufferWord3);
-        for (uint32_t mm = 0; mm < a.modules; ++mm, ++m) {
-            const float *__restrict__ p = pd + 20u * m;
+    const uint32_t nrec = a.n_views * a.modules;
+    uint32_t n_inside = 0;                  // records in which every voxel is in bounds
+    uint32_t view = 0, mm = 0;
+    for (uint32_t m0 = 0; m0 < nrec; m0 += 64) {
+        const RowSetup su = row_setup(pd + 20u * min(m0 + (uint32_t)lane, nrec - 1), a, py, pz,
+                                      px_first, px_last, line_off, s.inside != 0);
+        const uint64_t fast_mask = __builtin_amdgcn_ballot_w64(su.fast);
+        const uint64_t inside_mask = __builtin_amdgcn_ballot_w64(su.inside);
+        const uint32_t n_in = min(64u, nrec - m0);
+        for (uint32_t i = 0; i < n_in; ++i) {
+            const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+                (void *)(err + (uint64_t)view_of(a, view) * view_elems), 0,
+                (int)(view_elems * esz), kBufferWord3);
+            ConstF *ps = pd_const(pd + 20u * (m0 + i));
             ViewConsts c;
-            view_thresholds(p, a, c);
-            row_terms(p, py, pz, c);
-            if (fast_rcp_range(p, c, px_first, px_last, py, pz)) {
+            c.c0x = ps[0];
+            c.c1x = ps[4];
+            c.c3x = ps[12];
+            c.k0 = lane_float(su.k0, i);
+            c.k1 = lane_float(su.k1, i);
+            c.k3 = lane_float(su.k3, i);
+            if ((inside_mask >> i) & 1u) {
+                ++n_inside;
 #pragma unroll
                 for (int j = 0; j < J; ++j) {
                     uint32_t iu, iv;
-                    const bool inb = project<CROP, true>(c, px[j], line_off, iu, iv);
-                    const uint32_t off = (__umul24(iv, a.det_width) + iu) * esz;
-                    sum[j] += load_or_zero<ES>(rsrc, inb ? off : kOutOfRange);
-                    cnt[j] += inb ? 1u : 0u;
+                    project_inside<CROP>(c, px[j], line_off, iu, iv);
+                    sum[j] += load_or_zero<ES>(rsrc, (__umul24(iv, a.det_width) + iu) * esz);
                 }
             } else {
+                c.ulo = lane_float(su.ulo, i);
+                c.uhi = lane_float(su.uhi, i);
+                c.vlo = lane_float(su.vlo, i);
+                c.vhi = lane_float(su.vhi, i);
+                if ((fast_mask >> i) & 1u) {
 #pragma unroll
-                for (int j = 0; j < J; ++j) {
-                    uint32_t iu, iv;
-                    const bool inb = project<CROP, false>(c, px[j], line_off, iu, iv);
-                    const uint32_t off = (__umul24(iv, a.det_width) + iu) * esz;
-                    sum[j] += load_or_zero<ES>(rsrc, inb ? off : kOutOfRange);
-                    cnt[j] += inb ? 1u : 0u;
+                    for (int j = 0; j < J; ++j) {
+                        uint32_t iu, iv;
+                        const bool inb = project<CROP, true>(c, px[j], line_off, iu, iv);
+                        const uint32_t off = (__umul24(iv, a.det_width) + iu) * esz;
+                        sum[j] += load_or_zero<ES>(rsrc, inb ? off : kOutOfRange);
+                        cnt[j] += inb ? 1u : 0u;
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < J; ++j) {
+                        uint32_t iu, iv;
+                        const bool inb = project<CROP, false>(c, px[j], line_off, iu, iv);
+                        const uint32_t off = (__umul24(iv, a.det_width) + iu) * esz;
+                        sum[j] += load_or_zero<ES>(rsrc, inb ? off : kOutOfRange);
+                        cnt[j] += inb ? 1u : 0u;
+                    }
                 }
+            }
+            if (++mm == a.modules) {
+                mm = 0;
+                ++view;
             }
         }
     }
@@ -341,12 +438,13 @@ __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
 #pragma unroll
     for (int j = 0; j < J; ++j) {
         const int32_t x = xb + 64 * j;
+        const uint32_t n = cnt[j] + n_inside;
         if (x < s.nx) {
-            if (cnt[j] == 0) {
+            if (n == 0) {
                 ++empty;              // the reference computes exp(0/0) here
             } else {
                 const float v = VS::load(vol, rowoff + x);
-                VS::store(vol, rowoff + x, v * expf(sum[j] / (float)cnt[j]));
+                VS::store(vol, rowoff + x, v * expf(sum[j] / (float)n));
             }
         }
     }
@@ -1290,6 +1388,8 @@ int backward_impl(hipStream_t st, void *vol, const void *err, const float *pd,
 {
     if ((uint64_t)a.det_width * (uint64_t)a.line_rows * sizeof(typename ES::T) >= 0x80000000ull)
         return fail(XPR_E_INVALID, "a detector view above 2 GiB");
+    // tuning knob: XPR_BACKWARD_INSIDE=0 keeps the canonical bounds test in every view
+    if (const char *e = std::getenv("XPR_BACKWARD_INSIDE")) s.inside = std::atoi(e) != 0;
     if constexpr (std::is_same<VS, ES>::value) {       // the schedules' dtype pairs: all J compiled
         if (J < 0) {
             // tuning knob: force the voxels per lane
