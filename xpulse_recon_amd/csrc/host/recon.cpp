@@ -284,6 +284,9 @@ void Dataset::load(xpr_stream s)
 {
     if (loaded_) return;
     loaded_ = true;
+    // the RAM layer store is sized here, once: save_layer runs on several
+    // threads and must not resize the vector under them
+    if (ram_layers()) ram_store_.resize(size_t(vheight_));
     if (!p_.xpr.mem_projections) {
         raw_.assign(size_t(nproj_), {});
         parallel_for(nproj_, io_threads(p_), [&](int64_t i) { raw_[size_t(i)] = raw_image(i); });
@@ -339,7 +342,8 @@ void Dataset::save_layer(const float *data, int64_t layer, bool finalize)
         return;
     }
     if (ram_layers()) {
-        if (ram_store_.size() < size_t(vheight_)) ram_store_.resize(size_t(vheight_));
+        if (size_t(layer) >= ram_store_.size())
+            throw std::logic_error("RAM layer store used before Dataset::load");
         ram_store_[size_t(layer)].assign(data, data + n);
     } else {
         spill_write(numbered_path(p_.recon.output, "layer", layer, 5, "zstd"), data, n);
