@@ -377,9 +377,10 @@ def test_tuned_shapes_keep_results(cuda):
 @pytest.mark.parametrize("acc", [32, 64])
 def test_forward_exact_views_equal_checked(cuda, monkeypatch, acc):
     """Views whose rounding bound lets the tile test decide the bounds (the
-    exact-tile splat) give the sums of the canonical bounds test bit for bit,
-    on a 160^2 x 96 slab with 24 views (most bricks exact, edge bricks not),
-    a cropped row band and the oracle."""
+    exact-tile splat), and those whose tile the viewport does not clip in a
+    full brick (the inside splat, no test), give the sums of the canonical
+    bounds test bit for bit, on a 160^2 x 96 slab with 24 views (partial
+    bricks along x, edge bricks clipped), a cropped row band and the oracle."""
     import torch
     import xpulse_recon_amd as X
     spec = small_spec(160, 96, 48, rx=1.5)

@@ -532,6 +532,7 @@ enum : uint32_t {
     kBinFast = 4u,     // every voxel's W is inside the fast reciprocal's range
     kBinDepth = 8u,    // depth-lane order
     kBinExact = 16u,   // the tile test alone decides the canonical bounds (see bin_view)
+    kBinInside = 32u,  // exact, and every voxel-view of the brick lands in the tile
 };
 
 struct ViewBin {                   // one (view, module) record of the brick, in LDS
@@ -557,12 +558,16 @@ struct ViewBin {                   // one (view, module) record of the brick, in
 // the tile, and a voxel-view in the tile is in bounds: v_cvt_rpi equals roundf
 // for u > -0.5, which every tile column >= 1 implies.  So when du, dv <= 0.25
 // and the tile starts at column and row >= 1, the tile test alone is the
-// canonical bounds test.
+// canonical bounds test.  When in addition the viewport clips no side of the
+// tile and the brick lies wholly in the slab (so the corners bound every
+// voxel of it), every voxel-view rounds into the tile and is in bounds: the
+// splat needs no test at all (kBinInside).
 template <bool CROP>
 __device__ __forceinline__ void bin_view(const float *__restrict__ pd, uint32_t m, bool live,
                                          int corner, const int32_t (&lo)[3],
                                          const int32_t (&hi)[3], int cap, bool exact_ok,
-                                         const xpr_proj_args &a, ViewBin *__restrict__ out)
+                                         bool full, const xpr_proj_args &a,
+                                         ViewBin *__restrict__ out)
 {
     const float *__restrict__ p = pd + 20u * m;
     const float cpx = __builtin_fmaf((float)(((corner & 1) ? hi[0] : lo[0]) + a.vmin[0]), a.voxel[0], a.origin[0]);
@@ -611,10 +616,14 @@ __device__ __forceinline__ void bin_view(const float *__restrict__ pd, uint32_t 
     b.vlo = lo_threshold(vlo);
     b.vhi = hi_threshold(vhi);
     if (ok) {
-        b.u0 = (int32_t)fmaxf(floorf(umin) - 1.0f, ceilf(vpl));
-        b.v0 = (int32_t)fmaxf(floorf(vmin) - 1.0f, ceilf(vlo));
-        const int32_t u1 = (int32_t)fminf(ceilf(umax) + 1.0f, ceilf(vpr) - 1.0f);
-        const int32_t v1 = (int32_t)fminf(ceilf(vmax) + 1.0f, ceilf(vhi) - 1.0f);
+        const float fu0 = floorf(umin) - 1.0f, fv0 = floorf(vmin) - 1.0f;
+        const float fu1 = ceilf(umax) + 1.0f, fv1 = ceilf(vmax) + 1.0f;
+        const bool unclipped = (fu0 >= ceilf(vpl)) & (fv0 >= ceilf(vlo)) &
+                               (fu1 <= ceilf(vpr) - 1.0f) & (fv1 <= ceilf(vhi) - 1.0f);
+        b.u0 = (int32_t)fmaxf(fu0, ceilf(vpl));
+        b.v0 = (int32_t)fmaxf(fv0, ceilf(vlo));
+        const int32_t u1 = (int32_t)fminf(fu1, ceilf(vpr) - 1.0f);
+        const int32_t v1 = (int32_t)fminf(fv1, ceilf(vhi) - 1.0f);
         b.tw = u1 - b.u0 + 1;
         b.th = v1 - b.v0 + 1;
         if (b.tw <= 0 || b.th <= 0) {
@@ -626,8 +635,10 @@ __device__ __forceinline__ void bin_view(const float *__restrict__ pd, uint32_t 
             const float V = fmaxf(fabsf(vmin), fabsf(vmax)) + fabsf(line_off) + 1.0f;
             const float du = eps8 * ((tx + U * tw) / wmin + U);
             const float dv = eps8 * ((ty + V * tw) / wmin + V);
-            if (exact_ok && fast && du <= 0.25f && dv <= 0.25f && b.u0 >= 1 && b.v0 >= 1)
+            if (exact_ok && fast && du <= 0.25f && dv <= 0.25f && b.u0 >= 1 && b.v0 >= 1) {
                 flags |= kBinExact;
+                if (full && unclipped) flags |= kBinInside;
+            }
         }
     }
     b.flags = flags;
@@ -724,6 +735,25 @@ __device__ __forceinline__ void splat_exact(const ViewConsts &c, float px, uint3
     atomicAdd(&tl[hit ? e : sink_e], (unsigned long long)(hit ? qq : 0u));
 }
 
+// one voxel-view of an inside view (kBinInside): it lands in the tile.  tb is
+// the tile base minus the entry of pixel (0, 0), so the entry needs no offset;
+// tw8 is the tile row in bytes.
+template <bool CROP>
+__device__ __forceinline__ void splat_inside(const ViewConsts &c, float px, uint32_t qq,
+                                             float line_off, uint32_t tw8,
+                                             unsigned long long *__restrict__ tb)
+{
+    const float X = __builtin_fmaf(c.c0x, px, c.k0);
+    const float Y = __builtin_fmaf(c.c1x, px, c.k1);
+    const float W = __builtin_fmaf(c.c3x, px, c.k3);
+    const float r0 = __builtin_amdgcn_rcpf(W);
+    const float r = __builtin_fmaf(__builtin_fmaf(-W, r0, 1.0f), r0, r0);
+    const float u = X * r;
+    const float v = CROP ? Y * r - line_off : Y * r;
+    const uint32_t off = __umul24((uint32_t)pixel_of(v), tw8) + ((uint32_t)pixel_of(u) << 3);
+    atomicAdd((unsigned long long *)((char *)tb + off), (unsigned long long)qq);
+}
+
 // a uniform float copied to a VGPR, so that an FMA taking it and a value read
 // from one lane (an SGPR) needs no further copy
 __device__ __forceinline__ float to_vgpr(float x)
@@ -807,6 +837,7 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
     const int32_t lo[3] = {x0, l0, k0};
     const int32_t hi[3] = {min(x0 + 63, s.nx - 1), min(l0 + kWaves - 1, s.nl - 1),
                            min(k0 + kBK - 1, s.nk - 1)};
+    const bool full = (x0 + 64 <= s.nx) & (l0 + kWaves <= s.nl) & (k0 + kBK <= s.nk);
     const float line_off = (float)a.line_offset;
     const uint32_t view_elems = a.det_width * (uint32_t)a.line_rows;
     const uint32_t nm = a.n_views * a.modules;
@@ -819,7 +850,7 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
             const uint32_t vi = threadIdx.x >> 3;
             const bool live = vi < cnt;
             bin_view<CROP>(pd, live ? c0 + vi : c0, live, threadIdx.x & 7, lo, hi, s.cap,
-                           s.exact != 0, a, &bins[vi]);
+                           s.exact != 0, full, a, &bins[vi]);
         }
         __syncthreads();
         for (uint32_t vi = 0; vi < cnt; ++vi) {
@@ -843,7 +874,27 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
             c.vhi = bins[vi].vhi;
             row_terms(p, py, pz_d, c);
             const uint32_t dw = a.det_width;
-            if (flags & kBinExact) {
+            if (flags & kBinInside) {
+                unsigned long long *__restrict__ tb =
+                    tl - (uint32_t)(__umul24((uint32_t)v0, (uint32_t)tw) + (uint32_t)u0);
+                if (flags & kBinDepth) {
+#pragma unroll
+                    for (int j = 0; j < kBK; ++j)
+                        splat_inside<CROP>(c, px_d[j], qd[j], line_off, 8u * (uint32_t)tw, tb);
+                } else {
+                    ViewConsts ci = c;
+                    ci.c0x = to_vgpr(c.c0x);
+                    ci.c1x = to_vgpr(c.c1x);
+                    ci.c3x = to_vgpr(c.c3x);
+#pragma unroll
+                    for (int i = 0; i < kBK; ++i) {
+                        ci.k0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.k0), i));
+                        ci.k1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.k1), i));
+                        ci.k3 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.k3), i));
+                        splat_inside<CROP>(ci, px_x, qx[i], line_off, 8u * (uint32_t)tw, tb);
+                    }
+                }
+            } else if (flags & kBinExact) {
                 if (flags & kBinDepth) {
 #pragma unroll
                     for (int j = 0; j < kBK; ++j)
