@@ -735,13 +735,22 @@ __device__ __forceinline__ void splat_exact(const ViewConsts &c, float px, uint3
     atomicAdd(&tl[hit ? e : sink_e], (unsigned long long)(hit ? qq : 0u));
 }
 
+using LdsU64 = __attribute__((address_space(3))) unsigned long long;
+
+// LDS byte address of a pointer into shared memory
+__device__ __forceinline__ uint32_t lds_address(const unsigned long long *p)
+{
+    return (uint32_t)(uintptr_t)(const LdsU64 *)p;
+}
+
 // one voxel-view of an inside view (kBinInside): it lands in the tile.  tb is
-// the tile base minus the entry of pixel (0, 0), so the entry needs no offset;
-// tw8 is the tile row in bytes.
+// the LDS address of the tile base minus the entry of pixel (0, 0), so the
+// entry needs no offset; tw8 is the tile row in bytes.  The address is one
+// shift-add and one 24-bit multiply-add (left to itself the compiler spends
+// a multiply, two shifts and a three-way add).
 template <bool CROP>
 __device__ __forceinline__ void splat_inside(const ViewConsts &c, float px, uint32_t qq,
-                                             float line_off, uint32_t tw8,
-                                             unsigned long long *__restrict__ tb)
+                                             float line_off, uint32_t tw8, uint32_t tb)
 {
     const float X = __builtin_fmaf(c.c0x, px, c.k0);
     const float Y = __builtin_fmaf(c.c1x, px, c.k1);
@@ -750,8 +759,11 @@ __device__ __forceinline__ void splat_inside(const ViewConsts &c, float px, uint
     const float r = __builtin_fmaf(__builtin_fmaf(-W, r0, 1.0f), r0, r0);
     const float u = X * r;
     const float v = CROP ? Y * r - line_off : Y * r;
-    const uint32_t off = __umul24((uint32_t)pixel_of(v), tw8) + ((uint32_t)pixel_of(u) << 3);
-    atomicAdd((unsigned long long *)((char *)tb + off), (unsigned long long)qq);
+    const uint32_t col = ((uint32_t)pixel_of(u) << 3) + tb;
+    uint32_t addr;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(addr) : "v"((uint32_t)pixel_of(v)), "s"(tw8), "v"(col));
+    __hip_atomic_fetch_add((LdsU64 *)(uintptr_t)addr, (unsigned long long)qq, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // a uniform float copied to a VGPR, so that an FMA taking it and a value read
@@ -875,8 +887,8 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
             row_terms(p, py, pz_d, c);
             const uint32_t dw = a.det_width;
             if (flags & kBinInside) {
-                unsigned long long *__restrict__ tb =
-                    tl - (uint32_t)(__umul24((uint32_t)v0, (uint32_t)tw) + (uint32_t)u0);
+                const uint32_t tb = lds_address(tl) -
+                                    8u * (__umul24((uint32_t)v0, (uint32_t)tw) + (uint32_t)u0);
                 if (flags & kBinDepth) {
 #pragma unroll
                     for (int j = 0; j < kBK; ++j)
