@@ -540,6 +540,9 @@ struct ViewBin {                   // one (view, module) record of the brick, in
     float ulo, uhi, vlo, vhi;      // in-bounds thresholds (view_thresholds)
     uint32_t flags;
     uint32_t pad[3];
+    // ProjData rows 0, 1 and 3 of the matrix (record floats 0-7, 12-15): the
+    // splat reads them from here with the rest of the record, not from memory
+    float m[12];
 };
 
 // Footprint of the brick for view record m; run by 8 consecutive lanes (one
@@ -642,6 +645,10 @@ __device__ __forceinline__ void bin_view(const float *__restrict__ pd, uint32_t 
         }
     }
     b.flags = flags;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) b.m[i] = p[i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) b.m[8 + i] = p[12 + i];
     *out = b;
 }
 
@@ -766,15 +773,6 @@ __device__ __forceinline__ void splat_inside(const ViewConsts &c, float px, uint
                            __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// a uniform float copied to a VGPR, so that an FMA taking it and a value read
-// from one lane (an SGPR) needs no further copy
-__device__ __forceinline__ float to_vgpr(float x)
-{
-    float v;
-    asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "s"(x));
-    return v;
-}
-
 // One brick of the forward (b < s.nbricks, uniform over the workgroup).  lds
 // holds two tiles and one sink slot per lane ([2*kTileCap + kThreads]); the
 // tiles are zero on entry and left zero.
@@ -874,17 +872,25 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
             const int32_t tw = __builtin_amdgcn_readfirstlane(bins[vi].tw);
             const int32_t th = __builtin_amdgcn_readfirstlane(bins[vi].th);
             const bool use_tile = (flags & kBinTile) != 0;
-            const float *__restrict__ p = pd + 20u * m;
-            AccT *__restrict__ vsum = sums + (uint64_t)view_of(a, m / a.modules) * view_elems;
+            // (a uniform branch spares the single-module case the integer division)
+            const uint32_t view = a.modules == 1 ? m : m / a.modules;
+            AccT *__restrict__ vsum = sums + (uint64_t)view_of(a, view) * view_elems;
             unsigned long long *__restrict__ tl = lds + buf * kTileCap;
             const uint32_t sk = (uint32_t)((2 - buf) * kTileCap) + threadIdx.x;
-            // this lane's (layer, depth) row terms: depth k0 + dk
+            // this lane's (layer, depth) row terms: depth k0 + dk (row_terms
+            // on the record's copy of the matrix)
+            const ViewBin &bv = bins[vi];
             ViewConsts c;
-            c.ulo = bins[vi].ulo;
-            c.uhi = bins[vi].uhi;
-            c.vlo = bins[vi].vlo;
-            c.vhi = bins[vi].vhi;
-            row_terms(p, py, pz_d, c);
+            c.ulo = bv.ulo;
+            c.uhi = bv.uhi;
+            c.vlo = bv.vlo;
+            c.vhi = bv.vhi;
+            c.c0x = bv.m[0];
+            c.c1x = bv.m[4];
+            c.c3x = bv.m[8];
+            c.k0 = __builtin_fmaf(bv.m[1], py, __builtin_fmaf(bv.m[2], pz_d, bv.m[3]));
+            c.k1 = __builtin_fmaf(bv.m[5], py, __builtin_fmaf(bv.m[6], pz_d, bv.m[7]));
+            c.k3 = __builtin_fmaf(bv.m[9], py, __builtin_fmaf(bv.m[10], pz_d, bv.m[11]));
             const uint32_t dw = a.det_width;
             if (flags & kBinInside) {
                 const uint32_t tb = lds_address(tl) -
@@ -895,9 +901,6 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
                         splat_inside<CROP>(c, px_d[j], qd[j], line_off, 8u * (uint32_t)tw, tb);
                 } else {
                     ViewConsts ci = c;
-                    ci.c0x = to_vgpr(c.c0x);
-                    ci.c1x = to_vgpr(c.c1x);
-                    ci.c3x = to_vgpr(c.c3x);
 #pragma unroll
                     for (int i = 0; i < kBK; ++i) {
                         ci.k0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.k0), i));
@@ -913,9 +916,6 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
                         splat_exact<CROP>(c, px_d[j], qd[j], line_off, u0, v0, tw, th, tl, sk);
                 } else {
                     ViewConsts ci = c;
-                    ci.c0x = to_vgpr(c.c0x);
-                    ci.c1x = to_vgpr(c.c1x);
-                    ci.c3x = to_vgpr(c.c3x);
 #pragma unroll
                     for (int i = 0; i < kBK; ++i) {
                         ci.k0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.k0), i));
@@ -939,9 +939,6 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
             } else {
                 // x-lane order: the row terms of depth k0 + i sit in lane i
                 ViewConsts ci = c;
-                ci.c0x = to_vgpr(c.c0x);
-                ci.c1x = to_vgpr(c.c1x);
-                ci.c3x = to_vgpr(c.c3x);
                 if (flags & kBinFast) {
 #pragma unroll
                     for (int i = 0; i < kBK; ++i) {
