@@ -245,12 +245,42 @@ __device__ __forceinline__ float load_or_zero<BF16>(__amdgpu_buffer_rsrc_t r, ui
     return __builtin_bit_cast(float, (uint32_t)h << 16);
 }
 
+// Indexed (structured) buffer loads: the buffer's stride is the sample size
+// and num_records its sample count, so the hardware scales the index, and an
+// index at or beyond num_records reads 0 (kNoSample is beyond any view).  The
+// compiler offers raw buffer builtins only, so the intrinsic is bound by name.
+constexpr uint32_t kNoSample = 0xffffffffu;
+__device__ float struct_buffer_load_f32(__amdgpu_buffer_rsrc_t rsrc, uint32_t index, uint32_t voffset,
+                                        uint32_t soffset, uint32_t aux)
+    __asm("llvm.amdgcn.struct.ptr.buffer.load.f32");
+__device__ uint16_t struct_buffer_load_u16(__amdgpu_buffer_rsrc_t rsrc, uint32_t index, uint32_t voffset,
+                                           uint32_t soffset, uint32_t aux)
+    __asm("llvm.amdgcn.struct.ptr.buffer.load.i16");
+
+template <typename T>
+__device__ __forceinline__ float sample_or_zero(__amdgpu_buffer_rsrc_t r, uint32_t index);
+template <>
+__device__ __forceinline__ float sample_or_zero<F32>(__amdgpu_buffer_rsrc_t r, uint32_t index)
+{
+    return struct_buffer_load_f32(r, index, 0, 0, 0);
+}
+template <>
+__device__ __forceinline__ float sample_or_zero<F16>(__amdgpu_buffer_rsrc_t r, uint32_t index)
+{
+    return (float)__builtin_bit_cast(_Float16, struct_buffer_load_u16(r, index, 0, 0, 0));
+}
+template <>
+__device__ __forceinline__ float sample_or_zero<BF16>(__amdgpu_buffer_rsrc_t r, uint32_t index)
+{
+    return __builtin_bit_cast(float, (uint32_t)struct_buffer_load_u16(r, index, 0, 0, 0) << 16);
+}
+
 // ---------------------------------------------------------------------------
 // Backward: one wave per (layer, depth) row, lane owns J voxels at
 // x = seg*64*J + lane + 64*j.  Per view the (y, z, w) part of the three dot
 // products is computed once per row; a voxel-view then costs 3 FMA, the
 // reciprocal, 2 MUL, 4 compares, 2 conversions, the index and one gather.
-// The gather reads through a raw buffer over the view, so out-of-bounds
+// The gather reads through an indexed buffer over the view, so out-of-bounds
 // voxels read 0 without a branch and all J loads stay in flight.
 // ---------------------------------------------------------------------------
 constexpr int kWaves = 4;
@@ -384,8 +414,8 @@ __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
         const uint32_t n_in = min(64u, nrec - m0);
         for (uint32_t i = 0; i < n_in; ++i) {
             const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-                (void *)(err + (uint64_t)view_of(a, view) * view_elems), 0,
-                (int)(view_elems * esz), kBufferWord3);
+                (void *)(err + (uint64_t)view_of(a, view) * view_elems), (short)esz,
+                (int)view_elems, kBufferWord3);
             ConstF *ps = pd_const(pd + 20u * (m0 + i));
             ViewConsts c;
             c.c0x = ps[0];
@@ -400,7 +430,7 @@ __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
                 for (int j = 0; j < J; ++j) {
                     uint32_t iu, iv;
                     project_inside<CROP>(c, px[j], line_off, iu, iv);
-                    sum[j] += load_or_zero<ES>(rsrc, (__umul24(iv, a.det_width) + iu) * esz);
+                    sum[j] += sample_or_zero<ES>(rsrc, __umul24(iv, a.det_width) + iu);
                 }
             } else {
                 c.ulo = lane_float(su.ulo, i);
@@ -412,8 +442,8 @@ __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
                     for (int j = 0; j < J; ++j) {
                         uint32_t iu, iv;
                         const bool inb = project<CROP, true>(c, px[j], line_off, iu, iv);
-                        const uint32_t off = (__umul24(iv, a.det_width) + iu) * esz;
-                        sum[j] += load_or_zero<ES>(rsrc, inb ? off : kOutOfRange);
+                        const uint32_t idx = __umul24(iv, a.det_width) + iu;
+                        sum[j] += sample_or_zero<ES>(rsrc, inb ? idx : kNoSample);
                         cnt[j] += inb ? 1u : 0u;
                     }
                 } else {
@@ -421,8 +451,8 @@ __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
                     for (int j = 0; j < J; ++j) {
                         uint32_t iu, iv;
                         const bool inb = project<CROP, false>(c, px[j], line_off, iu, iv);
-                        const uint32_t off = (__umul24(iv, a.det_width) + iu) * esz;
-                        sum[j] += load_or_zero<ES>(rsrc, inb ? off : kOutOfRange);
+                        const uint32_t idx = __umul24(iv, a.det_width) + iu;
+                        sum[j] += sample_or_zero<ES>(rsrc, inb ? idx : kNoSample);
                         cnt[j] += inb ? 1u : 0u;
                     }
                 }
