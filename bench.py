@@ -122,6 +122,23 @@ def measured_traffic(kernel, args):
     return None
 
 
+def measured_valu(kernel, args):
+    """VALU figures of `kernel` on this workload from the committed rocprofv3
+    summary: instructions per voxel-view and the VALU-busy fraction
+    (SQ_ACTIVE_INST_VALU * 4 / SIMDs / GRBM_GUI_ACTIVE), or None."""
+    if (args.det, args.views, args.sit, args.slab) != (768, 720, 20, 64):
+        return None
+    path = os.path.join(HERE, "profiles", "kernel_stats_768cube_f32.md")
+    try:
+        for line in open(path):
+            parts = line.split()
+            if len(parts) == 4 and parts[0] == "valu" and parts[1] == kernel:
+                return {"instr_per_voxel_view": float(parts[2]), "busy_frac": float(parts[3])}
+    except OSError:
+        return None
+    return None
+
+
 SERVE_CLASSES = [(128, 180, 128), (384, 360, 256), (768, 720, 512)]   # detector, views, phantom
 
 
@@ -407,18 +424,20 @@ def main():
     def args_for(l0, l1, origin):
         return X.proj_args(D, A_s, 1, (0, l0, 0), (W, l1, W), 0, D, origin, (vx, vx, vx))
 
-    def step(k, ev=None):
+    def step(k, ev=None, lev=None):
         s = k % S
         origin = [np.float32(np.float32(grid["orig"][i]) + np.float32(jit[k][i])) for i in range(3)]
         sl = [args_for(l0, l1, origin) for (l0, l1) in slabs]
         if ev: ev[0].record()
-        for (l0, l1), a in zip(slabs, sl):
+        for i, ((l0, l1), a) in enumerate(zip(slabs, sl)):
             X.forward(vol[l0:l1], sums, pds[s], a)
+            if lev: lev[0][i].record()
         if ev: ev[1].record()
         X.error(images[s], sums, err, weight)
         if ev: ev[2].record()
-        for (l0, l1), a in zip(slabs, sl):
+        for i, ((l0, l1), a) in enumerate(zip(slabs, sl)):
             X.backward(vol[l0:l1], err, pds[s], a)
+            if lev: lev[1][i].record()
         if ev: ev[3].record()
         X.fill_u64(sums, 0)
         if ev: ev[4].record()
@@ -440,11 +459,15 @@ def main():
     log("warmup done")
 
     evs = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(args.steps)]
+    # per-launch events: each slab launch ends at its own event (the launches
+    # are back to back on one stream, so a launch runs from the previous event)
+    levs = [[[torch.cuda.Event(enable_timing=True) for _ in slabs] for _ in range(2)]
+            for _ in range(args.steps)]
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     start.record()
     for k in range(args.steps):
-        step(args.warmup + k, evs[k])
+        step(args.warmup + k, evs[k], levs[k])
     stop.record()
     torch.cuda.synchronize()
     total_ms = start.elapsed_time(stop)
@@ -481,6 +504,39 @@ def main():
                       "backward_frac": round(bp_bytes / (bp_launch * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                       "error_ms": round(er_ms, 4)}}
 
+    # secondary figures: per-launch median / p99, compulsory HBM traffic
+    # (volume slab and detector stack only), VALU issue from the committed
+    # rocprofv3 counters, flops per voxel-view
+    launches = {"k_forward_bin": [], "k_backward": []}
+    for k in range(args.steps):
+        e, le = evs[k], levs[k]
+        prev = e[0]
+        for ev_ in le[0]:
+            launches["k_forward_bin"].append(prev.elapsed_time(ev_))
+            prev = ev_
+        prev = e[2]
+        for ev_ in le[1]:
+            launches["k_backward"].append(prev.elapsed_time(ev_))
+            prev = ev_
+    acc_bytes = 8
+    comp = {"k_forward_bin": v_slab * sbytes + 2 * A_s * D * D * acc_bytes,
+            "k_backward": 2 * v_slab * sbytes + A_s * D * D * sbytes}
+    secondary = {"launch_ms": {}, "compulsory_hbm": {}, "valu": {},
+                 "flops_per_voxel_view": {"algorithmic": 27, "executed": 13,
+                                          "note": "3 dot4 as 12 FMA + rcp + 2 mul + sub; the "
+                                                  "kernels hoist the (y,z,w) part of each dot "
+                                                  "per row: 3 FMA + rcp + Newton (2 FMA) + 2 mul"}}
+    for kname, xs in launches.items():
+        xs = np.array(xs)
+        med = float(np.median(xs))
+        secondary["launch_ms"][kname] = {"median": round(med, 4),
+                                         "p99": round(float(np.percentile(xs, 99)), 4),
+                                         "n": int(xs.size)}
+        secondary["compulsory_hbm"][kname] = {
+            "bytes_per_launch": int(comp[kname]),
+            "frac": round(comp[kname] / (med * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        secondary["valu"][kname] = measured_valu(kname, args) if args.dtype == "f32" else None
+
     if args.calibrate:
         n = 1 << 28                      # 1 GiB read + 1 GiB written
         a = torch.empty(n, dtype=torch.float32, device=dev)
@@ -513,6 +569,7 @@ def main():
                    "voxel_views_per_step": vv_step, "tile_shapes": tuned or "defaults"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "secondary": secondary,
     }
     print(json.dumps(out), flush=True)
 
