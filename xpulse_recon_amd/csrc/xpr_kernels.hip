@@ -373,14 +373,19 @@ __device__ __forceinline__ RowSetup row_setup(const float *__restrict__ p, const
 
 // Segment seg of one (layer, depth) row of the backward, run by one wave.
 // The per-view set-up of the row is spread over the lanes (one view record
-// each, 64 at a time) and read back with v_readlane inside the view loop.
+// each, 64 at a time).  The view loop reads each record's row terms back from
+// the wave's LDS table rec[64] as one broadcast load, fetched a view ahead,
+// straight into VGPRs (v_readlane would need three reads and three copies,
+// since an FMA takes one scalar operand); LDS operations of a wave run in
+// order, so the table needs no barrier.
 template <typename VS, typename ES, int J, bool CROP>
 __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
                                              const typename ES::T *__restrict__ err,
                                              const float *__restrict__ pd,
                                              const xpr_proj_args &a, const SlabShape &s,
                                              int64_t row, int seg,
-                                             unsigned long long *__restrict__ empty_count)
+                                             unsigned long long *__restrict__ empty_count,
+                                             float4 *__restrict__ rec)
 {
     const int lane = threadIdx.x & 63;
     if (row >= s.nrows) return;                       // whole wave leaves together
@@ -412,18 +417,28 @@ __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
         const uint64_t fast_mask = __builtin_amdgcn_ballot_w64(su.fast);
         const uint64_t inside_mask = __builtin_amdgcn_ballot_w64(su.inside);
         const uint32_t n_in = min(64u, nrec - m0);
+        {
+            const float *__restrict__ pl = pd + 20u * min(m0 + (uint32_t)lane, nrec - 1);
+            rec[2 * lane] = make_float4(su.k0, su.k1, su.k3, pl[0]);
+            rec[2 * lane + 1] = make_float4(pl[4], pl[12], 0.0f, 0.0f);
+        }
+        float4 rn = rec[0], rn2 = rec[1];
         for (uint32_t i = 0; i < n_in; ++i) {
+            const float4 r = rn, r2 = rn2;
+            if (i + 1 < n_in) {                           // next record's terms in flight
+                rn = rec[2 * i + 2];
+                rn2 = rec[2 * i + 3];
+            }
             const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
                 (void *)(err + (uint64_t)view_of(a, view) * view_elems), (short)esz,
                 (int)view_elems, kBufferWord3);
-            ConstF *ps = pd_const(pd + 20u * (m0 + i));
             ViewConsts c;
-            c.c0x = ps[0];
-            c.c1x = ps[4];
-            c.c3x = ps[12];
-            c.k0 = lane_float(su.k0, i);
-            c.k1 = lane_float(su.k1, i);
-            c.k3 = lane_float(su.k3, i);
+            c.c0x = r.w;
+            c.c1x = r2.x;
+            c.c3x = r2.y;
+            c.k0 = r.x;
+            c.k1 = r.y;
+            c.k3 = r.z;
             if ((inside_mask >> i) & 1u) {
                 ++n_inside;
 #pragma unroll
@@ -491,9 +506,11 @@ __global__ __launch_bounds__(kThreads) void k_backward(
     const float *__restrict__ pd, xpr_proj_args a, SlabShape s,
     unsigned long long *__restrict__ empty_count)
 {
+    __shared__ float4 rec[kWaves * 128];              // per-wave view-record tables
     // segment blockIdx % nseg of the kWaves rows of row block blockIdx / nseg
     const int64_t row = s.row_base + (int64_t)(blockIdx.x / s.nseg) * kWaves + (threadIdx.x >> 6);
-    backward_row<VS, ES, J, CROP>(vol, err, pd, a, s, row, (int)(blockIdx.x % s.nseg), empty_count);
+    backward_row<VS, ES, J, CROP>(vol, err, pd, a, s, row, (int)(blockIdx.x % s.nseg), empty_count,
+                                  rec + (threadIdx.x >> 6) * 128);
 }
 
 // FLOAT_TO_FIXED (constants.cl:22), saturating like the GPU conversion.
@@ -1257,6 +1274,8 @@ __global__ __launch_bounds__(kThreads, 7) void k_batch_backward(const Table *__r
                                                                 uint32_t *counters)
 {
     __shared__ uint32_t slot;
+    __shared__ float4 rec[kWaves * 128];              // per-wave view-record tables
+    float4 *wrec = rec + (threadIdx.x >> 6) * 128;
     const uint32_t ph = xpr_batch::kBackward;
     const uint32_t total = uniform_u32(&t->total[ph]);
     if (total == 0) return;
@@ -1271,15 +1290,15 @@ __global__ __launch_bounds__(kThreads, 7) void k_batch_backward(const Table *__r
         switch (it.dt_dst) {                                // volume and error share the dtype
         case XPR_F32:
             backward_row<F32, F32, kBwdJ, false>((float *)it.dst, (const float *)it.err, it.pd, it.a, s,
-                                                 row, seg, nullptr);
+                                                 row, seg, nullptr, wrec);
             break;
         case XPR_F16:
             backward_row<F16, F16, kBwdJ, false>((uint16_t *)it.dst, (const uint16_t *)it.err, it.pd,
-                                                 it.a, s, row, seg, nullptr);
+                                                 it.a, s, row, seg, nullptr, wrec);
             break;
         default:
             backward_row<BF16, BF16, kBwdJ, false>((uint16_t *)it.dst, (const uint16_t *)it.err, it.pd,
-                                                   it.a, s, row, seg, nullptr);
+                                                   it.a, s, row, seg, nullptr, wrec);
             break;
         }
     }
