@@ -839,7 +839,8 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
                                               ViewBin *__restrict__ bins)
 {
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    // wave-uniform: lets the compiler keep row and flush addressing scalar
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int32_t bx = (int32_t)(b % s.nbx);
     b /= s.nbx;
     const int32_t bk = (int32_t)(b % s.nbk);
@@ -927,7 +928,8 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
             const int32_t th = __builtin_amdgcn_readfirstlane(bins[vi].th);
             const bool use_tile = (flags & kBinTile) != 0;
             // (a uniform branch spares the single-module case the integer division)
-            const uint32_t view = a.modules == 1 ? m : m / a.modules;
+            const uint32_t view =
+                __builtin_amdgcn_readfirstlane(a.modules == 1 ? m : m / a.modules);
             AccT *__restrict__ vsum = sums + (uint64_t)view_of(a, view) * view_elems;
             unsigned long long *__restrict__ tl = lds + buf * kTileCap;
             const uint32_t sk = (uint32_t)((2 - buf) * kTileCap) + threadIdx.x;
@@ -1016,13 +1018,15 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
             if (use_tile) {
                 lds_barrier();                // splat of this view complete
                 // flush and clear; the next view splats into the other tile
+                // (row pointers are wave-uniform; a lane adds a 32-bit column offset)
                 for (int rr = wave; rr < th; rr += kWaves) {
                     AccT *dst = vsum + (uint32_t)(v0 + rr) * dw + (uint32_t)u0;
-                    for (int cc = lane; cc < tw; cc += 64) {
-                        const unsigned long long t = tl[rr * tw + cc];
+                    unsigned long long *src = tl + rr * tw;
+                    for (uint32_t cc = (uint32_t)lane; cc < (uint32_t)tw; cc += 64u) {
+                        const unsigned long long t = src[cc];
                         if (t != 0ull) {
                             global_add(dst + cc, (AccT)t);
-                            tl[rr * tw + cc] = 0ull;
+                            src[cc] = 0ull;
                         }
                     }
                 }
