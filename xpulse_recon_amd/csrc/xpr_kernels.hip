@@ -508,7 +508,8 @@ __global__ __launch_bounds__(kThreads) void k_backward(
 {
     __shared__ float4 rec[kWaves * 128];              // per-wave view-record tables
     // segment blockIdx % nseg of the kWaves rows of row block blockIdx / nseg
-    const int64_t row = s.row_base + (int64_t)(blockIdx.x / s.nseg) * kWaves + (threadIdx.x >> 6);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform
+    const int64_t row = s.row_base + (int64_t)(blockIdx.x / s.nseg) * kWaves + wave;
     backward_row<VS, ES, J, CROP>(vol, err, pd, a, s, row, (int)(blockIdx.x % s.nseg), empty_count,
                                   rec + (threadIdx.x >> 6) * 128);
 }
@@ -911,7 +912,7 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
     for (uint32_t c0 = 0; c0 < nm; c0 += kViewChunk) {
         const uint32_t cnt = min((uint32_t)kViewChunk, nm - c0);
         __syncthreads();                              // previous chunk's records read
-        {
+        if ((uint32_t)wave * 8u < cnt) {              // a wave with no live record skips
             const uint32_t vi = threadIdx.x >> 3;
             const bool live = vi < cnt;
             bin_view<CROP>(pd, live ? c0 + vi : c0, live, threadIdx.x & 7, lo, hi, s.cap,
@@ -1289,7 +1290,7 @@ __global__ __launch_bounds__(kThreads, 7) void k_batch_backward(const Table *__r
         const Item it = uniform_item(&t->items[ph][item_of(t, ph, u)]);
         const SlabShape s = batch_slab_shape(it.a, kBwdJ);
         const int64_t unit = (int64_t)(u - it.first);
-        const int64_t row = (unit / s.nseg) * kWaves + (threadIdx.x >> 6);
+        const int64_t row = (unit / s.nseg) * kWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
         const int seg = (int)(unit % s.nseg);
         switch (it.dt_dst) {                                // volume and error share the dtype
         case XPR_F32:
