@@ -466,3 +466,43 @@ def test_backward_inside_views_equal_checked(cuda, monkeypatch, case):
     empty_ref = O.backward_f32(ref, err, pd, oracle_geom(d), 8)
     assert ea == empty_ref
     np.testing.assert_allclose(a, ref, rtol=4e-7, atol=0)
+
+
+@pytest.mark.parametrize("fast", ["1", "0"])
+def test_indexed_views_two_modules(cuda, monkeypatch, fast):
+    """Indexed launches (a subset of the detector views through view_index,
+    the reference's ForwardH/BackwardH) with two modules per view on a 128^2 x
+    64 grid (full bricks, so the inside splat and inside segments run): the
+    forward writes exactly the indexed views with the oracle's sums, the
+    backward equals the oracle fed the indexed error images; with the exact /
+    inside fast paths on and off (XPR_FORWARD_EXACT, XPR_BACKWARD_INSIDE)."""
+    import torch
+    import xpulse_recon_amd as X
+    monkeypatch.setenv("XPR_FORWARD_EXACT", fast)
+    monkeypatch.setenv("XPR_BACKWARD_INSIDE", fast)
+    spec = small_spec(128, 64, 40, module_number=2, module_angle_offset=5.0, rx=1.0)
+    idx = [3, 7, 8, 20, 33, 39]
+    pd, grid = projdata_for(spec, idx)
+    W, H = grid["vwidth"], grid["vheight"]
+    d = geom_args(spec, grid, len(idx), jitter=(0.002, 0.003, 0.001))
+    rng = np.random.default_rng(77)
+    vol = rng.uniform(0.2, 1.8, (H, W, W)).astype(np.float32)
+    rows, width = d["line_rows"], d["det_width"]
+    tidx = torch.tensor(idx, dtype=torch.int32, device=cuda)
+    a = X.proj_args(width, len(idx), 2, d["vmin"], d["vmax"], 0, rows, d["origin"], d["voxel"],
+                    view_index=tidx.data_ptr())
+    tp = torch.from_numpy(pd).to(cuda)
+    sums = torch.zeros((40, rows, width), dtype=torch.int64, device=cuda)
+    X.forward(torch.from_numpy(vol).to(cuda), sums, tp, a)
+    ref = np.zeros((len(idx), rows, width), np.uint64)
+    O.forward_f32(vol, ref, pd, oracle_geom(d), 8)
+    got = sums.cpu().numpy().view(np.uint64)
+    np.testing.assert_array_equal(got[idx], ref)
+    others = [v for v in range(40) if v not in idx]
+    assert not got[others].any()
+    err = rng.normal(0.0, 0.3, (40, rows, width)).astype(np.float32)
+    tv = torch.from_numpy(vol).to(cuda)
+    X.backward(tv, torch.from_numpy(err).to(cuda), tp, a)
+    ref_v = vol.copy()
+    O.backward_f32(ref_v, np.ascontiguousarray(err[idx]), pd, oracle_geom(d), 8)
+    np.testing.assert_allclose(tv.cpu().numpy(), ref_v, rtol=4e-7, atol=0)
