@@ -373,11 +373,13 @@ __device__ __forceinline__ RowSetup row_setup(const float *__restrict__ p, const
 
 // Segment seg of one (layer, depth) row of the backward, run by one wave.
 // The per-view set-up of the row is spread over the lanes (one view record
-// each, 64 at a time).  The view loop reads each record's row terms back from
-// the wave's LDS table rec[64] as one broadcast load, fetched a view ahead,
-// straight into VGPRs (v_readlane would need three reads and three copies,
-// since an FMA takes one scalar operand); LDS operations of a wave run in
-// order, so the table needs no barrier.
+// each, 64 at a time).  The view loop reads each record's row terms and
+// matrix coefficients back from the wave's LDS table rec[128] as broadcast
+// loads straight into VGPRs (v_readlane would need three reads and three
+// copies, since an FMA takes one scalar operand); LDS operations of a wave
+// run in order, so the table needs no barrier.  Both branches of a view leave
+// their J samples in smp[] and one accumulation follows, in view order, so
+// the sums equal the canonical ones and no register copies join the branches.
 template <typename VS, typename ES, int J, bool CROP>
 __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
                                              const typename ES::T *__restrict__ err,
@@ -422,13 +424,8 @@ __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
             rec[2 * lane] = make_float4(su.k0, su.k1, su.k3, pl[0]);
             rec[2 * lane + 1] = make_float4(pl[4], pl[12], 0.0f, 0.0f);
         }
-        float4 rn = rec[0], rn2 = rec[1];
         for (uint32_t i = 0; i < n_in; ++i) {
-            const float4 r = rn, r2 = rn2;
-            if (i + 1 < n_in) {                           // next record's terms in flight
-                rn = rec[2 * i + 2];
-                rn2 = rec[2 * i + 3];
-            }
+            const float4 r = rec[2 * i], r2 = rec[2 * i + 1];
             const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
                 (void *)(err + (uint64_t)view_of(a, view) * view_elems), (short)esz,
                 (int)view_elems, kBufferWord3);
@@ -439,13 +436,14 @@ __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
             c.k0 = r.x;
             c.k1 = r.y;
             c.k3 = r.z;
+            float smp[J];
             if ((inside_mask >> i) & 1u) {
                 ++n_inside;
 #pragma unroll
                 for (int j = 0; j < J; ++j) {
                     uint32_t iu, iv;
                     project_inside<CROP>(c, px[j], line_off, iu, iv);
-                    sum[j] += sample_or_zero<ES>(rsrc, __umul24(iv, a.det_width) + iu);
+                    smp[j] = sample_or_zero<ES>(rsrc, __umul24(iv, a.det_width) + iu);
                 }
             } else {
                 c.ulo = lane_float(su.ulo, i);
@@ -458,7 +456,7 @@ __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
                         uint32_t iu, iv;
                         const bool inb = project<CROP, true>(c, px[j], line_off, iu, iv);
                         const uint32_t idx = __umul24(iv, a.det_width) + iu;
-                        sum[j] += sample_or_zero<ES>(rsrc, inb ? idx : kNoSample);
+                        smp[j] = sample_or_zero<ES>(rsrc, inb ? idx : kNoSample);
                         cnt[j] += inb ? 1u : 0u;
                     }
                 } else {
@@ -467,11 +465,13 @@ __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
                         uint32_t iu, iv;
                         const bool inb = project<CROP, false>(c, px[j], line_off, iu, iv);
                         const uint32_t idx = __umul24(iv, a.det_width) + iu;
-                        sum[j] += sample_or_zero<ES>(rsrc, inb ? idx : kNoSample);
+                        smp[j] = sample_or_zero<ES>(rsrc, inb ? idx : kNoSample);
                         cnt[j] += inb ? 1u : 0u;
                     }
                 }
             }
+#pragma unroll
+            for (int j = 0; j < J; ++j) sum[j] += smp[j];
             if (++mm == a.modules) {
                 mm = 0;
                 ++view;
