@@ -960,9 +960,9 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
                     ViewConsts ci = c;
 #pragma unroll
                     for (int i = 0; i < kBK; ++i) {
-                        ci.k0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.k0), i));
-                        ci.k1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.k1), i));
-                        ci.k3 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.k3), i));
+                        ci.k0 = lane_float(c.k0, i);
+                        ci.k1 = lane_float(c.k1, i);
+                        ci.k3 = lane_float(c.k3, i);
                         splat_inside<CROP>(ci, px_x, qx[i], line_off, 8u * (uint32_t)tw, tb);
                     }
                 }
@@ -975,9 +975,9 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
                     ViewConsts ci = c;
 #pragma unroll
                     for (int i = 0; i < kBK; ++i) {
-                        ci.k0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.k0), i));
-                        ci.k1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.k1), i));
-                        ci.k3 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.k3), i));
+                        ci.k0 = lane_float(c.k0, i);
+                        ci.k1 = lane_float(c.k1, i);
+                        ci.k3 = lane_float(c.k3, i);
                         splat_exact<CROP>(ci, px_x, qx[i], line_off, u0, v0, tw, th, tl, sk);
                     }
                 }
@@ -999,18 +999,18 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
                 if (flags & kBinFast) {
 #pragma unroll
                     for (int i = 0; i < kBK; ++i) {
-                        ci.k0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.k0), i));
-                        ci.k1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.k1), i));
-                        ci.k3 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.k3), i));
+                        ci.k0 = lane_float(c.k0, i);
+                        ci.k1 = lane_float(c.k1, i);
+                        ci.k3 = lane_float(c.k3, i);
                         splat_one<CROP, true, AccT>(ci, px_x, qx[i], line_off, u0, v0, tw, th,
                                                     use_tile, tl, sk, vsum, dw);
                     }
                 } else {
 #pragma unroll
                     for (int i = 0; i < kBK; ++i) {
-                        ci.k0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.k0), i));
-                        ci.k1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.k1), i));
-                        ci.k3 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c.k3), i));
+                        ci.k0 = lane_float(c.k0, i);
+                        ci.k1 = lane_float(c.k1, i);
+                        ci.k3 = lane_float(c.k3, i);
                         splat_one<CROP, false, AccT>(ci, px_x, qx[i], line_off, u0, v0, tw, th,
                                                      use_tile, tl, sk, vsum, dw);
                     }
