@@ -603,9 +603,10 @@ struct ViewBin {                   // one (view, module) record of the brick, in
 // Footprint of the brick for view record m; run by 8 consecutive lanes (one
 // per corner: bit 0 x, bit 1 layer, bit 2 depth), written by the first.
 //
-// The tile is the footprint with one pixel of margin, clipped to the columns
-// and rows the canonical bounds accept (integers n with ceil(lo) <= n <
-// ceil(hi)), so every tile entry is in bounds.  Exact views: the voxel
+// The tile is the footprint (with one pixel of margin unless the rounding
+// bound below holds), clipped to the columns and rows the canonical bounds
+// accept (integers n with ceil(lo) <= n < ceil(hi)), so every tile entry is in
+// bounds.  Exact views: the voxel
 // positions are monotone in their indices, so the exact projection of every
 // voxel of the brick lies between the corners' (W > 0 over the box); with
 //   du = 8 eps ((Tx + U Tw) / Wmin + U),  eps = 2^-24,
@@ -674,8 +675,20 @@ __device__ __forceinline__ void bin_view(const float *__restrict__ pd, uint32_t 
     b.vlo = lo_threshold(vlo);
     b.vhi = hi_threshold(vhi);
     if (ok) {
-        const float fu0 = floorf(umin) - 1.0f, fv0 = floorf(vmin) - 1.0f;
-        const float fu1 = ceilf(umax) + 1.0f, fv1 = ceilf(vmax) + 1.0f;
+        constexpr float eps8 = 8.0f / 16777216.0f;
+        const float U = fmaxf(fabsf(umin), fabsf(umax)) + 1.0f;
+        const float V = fmaxf(fabsf(vmin), fabsf(vmax)) + fabsf(line_off) + 1.0f;
+        const float du = eps8 * ((tx + U * tw) / wmin + U);
+        const float dv = eps8 * ((ty + V * tw) / wmin + V);
+        // With du, dv <= 0.25 every computed u lies in [umin - 0.5, umax + 0.5],
+        // so its column floor(u + 0.5) lies in [floor(umin), floor(umax) + 1]
+        // (likewise rows): the tile needs no further margin.  Otherwise one
+        // pixel of margin each side.
+        const bool tight = exact_ok & (fast != 0) & (du <= 0.25f) & (dv <= 0.25f);
+        const float fu0 = tight ? floorf(umin) : floorf(umin) - 1.0f;
+        const float fv0 = tight ? floorf(vmin) : floorf(vmin) - 1.0f;
+        const float fu1 = tight ? floorf(umax) + 1.0f : ceilf(umax) + 1.0f;
+        const float fv1 = tight ? floorf(vmax) + 1.0f : ceilf(vmax) + 1.0f;
         const bool unclipped = (fu0 >= ceilf(vpl)) & (fv0 >= ceilf(vlo)) &
                                (fu1 <= ceilf(vpr) - 1.0f) & (fv1 <= ceilf(vhi) - 1.0f);
         b.u0 = (int32_t)fmaxf(fu0, ceilf(vpl));
@@ -688,12 +701,7 @@ __device__ __forceinline__ void bin_view(const float *__restrict__ pd, uint32_t 
             flags |= kBinSkip;
         } else if (b.tw * b.th <= cap) {
             flags |= kBinTile;
-            constexpr float eps8 = 8.0f / 16777216.0f;
-            const float U = fmaxf(fabsf(umin), fabsf(umax)) + 1.0f;
-            const float V = fmaxf(fabsf(vmin), fabsf(vmax)) + fabsf(line_off) + 1.0f;
-            const float du = eps8 * ((tx + U * tw) / wmin + U);
-            const float dv = eps8 * ((ty + V * tw) / wmin + V);
-            if (exact_ok && fast && du <= 0.25f && dv <= 0.25f && b.u0 >= 1 && b.v0 >= 1) {
+            if (tight && b.u0 >= 1 && b.v0 >= 1) {
                 flags |= kBinExact;
                 if (full && unclipped) flags |= kBinInside;
             }
