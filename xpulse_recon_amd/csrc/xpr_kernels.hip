@@ -680,15 +680,18 @@ __device__ __forceinline__ void bin_view(const float *__restrict__ pd, uint32_t 
         const float V = fmaxf(fabsf(vmin), fabsf(vmax)) + fabsf(line_off) + 1.0f;
         const float du = eps8 * ((tx + U * tw) / wmin + U);
         const float dv = eps8 * ((ty + V * tw) / wmin + V);
-        // With du, dv <= 0.25 every computed u lies in [umin - 0.5, umax + 0.5],
-        // so its column floor(u + 0.5) lies in [floor(umin), floor(umax) + 1]
-        // (likewise rows): the tile needs no further margin.  Otherwise one
-        // pixel of margin each side.
+        // With du, dv <= 0.25 every computed u lies in [umin - 2du, umax + 2du],
+        // so its column floor(u + 0.5) lies in [floor(umin - mu + 0.5),
+        // floor(umax + mu + 0.5)] with mu = 2du plus a term that covers the
+        // float evaluation of these bounds (likewise rows): the tile is the
+        // rounded footprint itself.  Otherwise one pixel of margin each side.
         const bool tight = exact_ok & (fast != 0) & (du <= 0.25f) & (dv <= 0.25f);
-        const float fu0 = tight ? floorf(umin) : floorf(umin) - 1.0f;
-        const float fv0 = tight ? floorf(vmin) : floorf(vmin) - 1.0f;
-        const float fu1 = tight ? floorf(umax) + 1.0f : ceilf(umax) + 1.0f;
-        const float fv1 = tight ? floorf(vmax) + 1.0f : ceilf(vmax) + 1.0f;
+        const float mu = 2.0f * du + 1e-6f * U + (1.0f / 1024.0f);
+        const float mv = 2.0f * dv + 1e-6f * V + (1.0f / 1024.0f);
+        const float fu0 = tight ? floorf(umin - mu + 0.5f) : floorf(umin) - 1.0f;
+        const float fv0 = tight ? floorf(vmin - mv + 0.5f) : floorf(vmin) - 1.0f;
+        const float fu1 = tight ? floorf(umax + mu + 0.5f) : ceilf(umax) + 1.0f;
+        const float fv1 = tight ? floorf(vmax + mv + 0.5f) : ceilf(vmax) + 1.0f;
         const bool unclipped = (fu0 >= ceilf(vpl)) & (fv0 >= ceilf(vlo)) &
                                (fu1 <= ceilf(vpr) - 1.0f) & (fv1 <= ceilf(vhi) - 1.0f);
         b.u0 = (int32_t)fmaxf(fu0, ceilf(vpl));
