@@ -322,7 +322,8 @@ def ooc_main(args):
                         "weight_factor": 0.9},
               "geometry": {"so": 100.0, "sd": 1000.0},
               "detector": {"px": 0.1},
-              "xpr": {"device_budget_gb": 16.0, "jitter_seed": 7, "layer_store": "ram"}}
+              "xpr": {"device_budget_gb": 16.0, "jitter_seed": 7, "layer_store": "ram",
+                      "quiet": True}}
     rec = C.create_string_buffer(1 << 20)
     log("reconstruction (method_2, 16 GB budget) ...")
     t0 = time.perf_counter()

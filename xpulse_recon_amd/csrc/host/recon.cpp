@@ -649,8 +649,9 @@ public:
         ops.setAngleNumber(As, p_.detector.module_number);
         check(xpr_fill(stream_, vol.get(), st, 1.0f, V), "fill");
         {
-            // tile shapes timed on 32 central layers of the all-1.0 volume
-            const int64_t l0 = std::max<int64_t>(0, H / 2 - 16), l1 = std::min<int64_t>(H, l0 + 32);
+            // tile shapes timed on 128 central layers of the all-1.0 volume (on
+            // 32 the launch tail can favour a shape that is slower at scale)
+            const int64_t l0 = std::max<int64_t>(0, H / 2 - 64), l1 = std::min<int64_t>(H, l0 + 128);
             xpr_proj_args ta = ops.args();
             ta.vmin[1] = int32_t(l0);
             ta.vmax[1] = int32_t(l1);

@@ -785,7 +785,7 @@ int xpr_tune_forward(xpr_stream s, const void *volume, int vol_dtype, void *sums
             }
         };
         float ms = 0.0f;
-        if (int rc = time_best(st, 2, launch, &ms)) return rc;
+        if (int rc = time_best(st, 3, launch, &ms)) return rc;
         if (ms < best) {
             best = ms;
             pick = cap;
@@ -822,7 +822,7 @@ int xpr_tune_backward(xpr_stream s, void *volume_scratch, int vol_dtype, const v
             }
         };
         float ms = 0.0f;
-        if (int rc = time_best(st, 2, launch, &ms)) return rc;
+        if (int rc = time_best(st, 3, launch, &ms)) return rc;
         if (ms < best) {
             best = ms;
             pick = J;
