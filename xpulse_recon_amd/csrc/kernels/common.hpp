@@ -1,4 +1,5 @@
-// Device building blocks: storage types, the canonical projection and its exact fast forms, buffer loads, launch shapes, Q.22 conversion.
+// Device building blocks: storage types, the canonical projection and its
+// exact fast forms, buffer loads, launch shapes, Q.22 conversion.
 // (library-internal; every header here is included once, by xpr_kernels.hip,
 // into one translation unit)
 #pragma once

@@ -1,4 +1,5 @@
-// Elementwise kernels: Error (KernelError.cl:8-24), preprocessing, fills, conversions.
+// Elementwise kernels: Error (KernelError.cl:8-24), preprocessing, fills,
+// conversions.
 // (library-internal; every header here is included once, by xpr_kernels.hip,
 // into one translation unit)
 #pragma once
