@@ -176,6 +176,16 @@ int xpr_tune_backward(xpr_stream s, void *volume_scratch, int vol_dtype, const v
 /* Forget every tuned shape (defaults apply again). */
 int xpr_tune_reset(void);
 
+/* No counterpart in the reference: a self-check of the two hardware
+ * identities the projectors' exact fast paths rely on, over every float32 bit
+ * pattern (one pass of a few milliseconds).  mismatches[0] counts w with
+ * normal |w| in [1e-6, 1e6] where v_rcp_f32 plus one Newton FMA differs from
+ * the IEEE quotient 1/w; mismatches[1] counts u in (-0.5, 2^23) where
+ * v_cvt_rpi_i32_f32 differs from (int)roundf(u).  Both are 0 on a correct
+ * device; mismatches[2] counts the reciprocal's mismatches outside that range
+ * (where the kernels use the IEEE division).  Synchronous. */
+int xpr_check_fast_paths(xpr_stream s, uint64_t mismatches[3]);
+
 /* ---- geometry (Geometry, dataset/geometry.hpp:35-76, 182-328) ----------- */
 
 /* JSON "geometry" and "detector" sections (global_parameters.hpp:23-54).

@@ -15,7 +15,8 @@ from ._lib import ProjArgs, XprError, proj_args  # noqa: F401
 
 __all__ = ["forward", "error", "backward", "fill", "fill_u32", "fill_u64", "convert",
            "preprocess", "minmax_normal", "proj_args", "ProjArgs", "XprError",
-           "dtype_code", "acc_bits", "tune_forward", "tune_backward", "tune_reset"]
+           "dtype_code", "acc_bits", "tune_forward", "tune_backward", "tune_reset",
+           "check_fast_paths"]
 
 
 def _stream(stream):
@@ -165,3 +166,13 @@ def tune_backward(volume_scratch, err, projdata, args: ProjArgs, stream=None) ->
 
 def tune_reset() -> None:
     _lib.call("xpr_tune_reset")
+
+
+def check_fast_paths(stream=None):
+    """Counts of float32 inputs where the exact fast forms differ from the
+    canonical ones: (reciprocal in [1e-6, 1e6], v_cvt_rpi vs roundf on
+    (-0.5, 2^23), reciprocal outside the range).  The first two are 0 on a
+    correct device."""
+    out = (C.c_uint64 * 3)()
+    _lib.call("xpr_check_fast_paths", _stream(stream), out)
+    return int(out[0]), int(out[1]), int(out[2])

@@ -108,4 +108,37 @@ __global__ __launch_bounds__(256) void k_preprocess(const float *__restrict__ in
     }
 }
 
+// Self-check of the exact fast forms (recip's fast branch and pixel_of)
+// over every float32 bit pattern; bad[0..2] as xpr_check_fast_paths counts.
+__global__ __launch_bounds__(256) void k_check_fast_paths(unsigned long long *__restrict__ bad)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint32_t n0 = 0, n1 = 0, n2 = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (1ull << 32); i += stride) {
+        const float w = __builtin_bit_cast(float, (uint32_t)i);
+        const float aw = fabsf(w);
+        if (aw > 0.0f && aw <= 3.4e38f) {                           // finite, non-zero
+            const float r0 = __builtin_amdgcn_rcpf(w);
+            const float r = __builtin_fmaf(__builtin_fmaf(-w, r0, 1.0f), r0, r0);
+            const float ref = 1.0f / w;                               // IEEE quotient
+            if (__builtin_bit_cast(uint32_t, r) != __builtin_bit_cast(uint32_t, ref)) {
+                if (aw >= 1e-6f && aw <= 1e6f) ++n0;
+                else ++n2;
+            }
+        }
+        if (w > -0.5f && w < 8388608.0f && pixel_of(w) != (int32_t)roundf(w)) ++n1;
+    }
+    // one atomic per wave and counter
+    for (int off = 32; off > 0; off >>= 1) {
+        n0 += __shfl_xor(n0, off);
+        n1 += __shfl_xor(n1, off);
+        n2 += __shfl_xor(n2, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (n0) atomicAdd(&bad[0], (unsigned long long)n0);
+        if (n1) atomicAdd(&bad[1], (unsigned long long)n1);
+        if (n2) atomicAdd(&bad[2], (unsigned long long)n2);
+    }
+}
+
 } // namespace

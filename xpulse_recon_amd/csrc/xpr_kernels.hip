@@ -840,6 +840,28 @@ int xpr_tune_reset(void)
     return XPR_OK;
 }
 
+int xpr_check_fast_paths(xpr_stream s, uint64_t mismatches[3])
+{
+    if (!mismatches) return fail(XPR_E_INVALID, "null mismatches");
+    hipStream_t st = (hipStream_t)s;
+    unsigned long long *d = nullptr;
+    XPR_HIP_CHECK(hipMallocAsync((void **)&d, 3 * sizeof(unsigned long long), st));
+    int rc = XPR_OK;
+    if (hipMemsetAsync(d, 0, 3 * sizeof(unsigned long long), st) != hipSuccess) {
+        rc = fail(XPR_E_HIP, "memset");
+    } else {
+        hipLaunchKernelGGL(k_check_fast_paths, dim3(8192), dim3(256), 0, st, d);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(mismatches, d, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, st) !=
+                hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            rc = fail(XPR_E_HIP, "fast-path check");
+    }
+    (void)hipFreeAsync(d, st);
+    (void)hipStreamSynchronize(st);
+    return rc;
+}
+
 int xpr_q22_to_float(xpr_stream s, const void *sums, int acc_bits, float *out, uint64_t n)
 {
     if (acc_bits != XPR_ACC_U32 && acc_bits != XPR_ACC_U64)
