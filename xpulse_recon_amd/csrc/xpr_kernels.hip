@@ -187,6 +187,24 @@ int time_best(hipStream_t st, int reps, F &&launch, float *best)
 
 constexpr int kBwdJs[] = {2, 4, 8, 16};
 
+// The tuned shape: the default unless another is more than 3 % faster, so
+// timing noise on the tuning launch cannot lock in a shape that is no better.
+template <size_t N>
+int pick_shape(const int (&cands)[N], const float (&ms)[N], int fallback)
+{
+    float t_def = 3.4e38f;
+    for (size_t i = 0; i < N; ++i)
+        if (cands[i] == fallback) t_def = ms[i];
+    int pick = fallback;
+    float best = 0.97f * t_def;
+    for (size_t i = 0; i < N; ++i)
+        if (ms[i] < best) {
+            best = ms[i];
+            pick = cands[i];
+        }
+    return pick;
+}
+
 // default voxels per lane (J = 4 measured best at 768^3 / 36 views among 2, 4,
 // 8, 16); narrow slabs take J = 2 so one 64*J segment leaves few lanes idle
 int backward_default_j(int32_t nx) { return nx <= 128 ? 2 : kBwdJ; }
@@ -767,8 +785,8 @@ int xpr_tune_forward(xpr_stream s, const void *volume, int vol_dtype, void *sums
     if (!volume || !sums_scratch || !projdata || !choice) return fail(XPR_E_INVALID, "null buffer");
     if (a->n_views == 0 || sh.nx == 0 || sh.nrows == 0) return fail(XPR_E_INVALID, "empty launch");
     hipStream_t st = (hipStream_t)s;
-    float best = 3.4e38f;
-    int pick = kTileCap;
+    float times[sizeof(kTileCaps) / sizeof(kTileCaps[0])];
+    int ti = 0;
     for (int cap : kTileCaps) {
         auto launch = [&]() {
             if (acc_bits == XPR_ACC_U32) {
@@ -784,13 +802,9 @@ int xpr_tune_forward(xpr_stream s, const void *volume, int vol_dtype, void *sums
             default: return forward_impl<BF16, uint64_t>(st, volume, sums_scratch, projdata, *a, sh, cap);
             }
         };
-        float ms = 0.0f;
-        if (int rc = time_best(st, 3, launch, &ms)) return rc;
-        if (ms < best) {
-            best = ms;
-            pick = cap;
-        }
+        if (int rc = time_best(st, 3, launch, &times[ti++])) return rc;
     }
+    const int pick = pick_shape(kTileCaps, times, kTileCap);
     const int acc_bytes = acc_bits / 8;
     tune_store(tune_key(kTuneForward, vol_dtype, acc_bytes, *a, sh), pick);
     *choice = pick;
@@ -811,8 +825,8 @@ int xpr_tune_backward(xpr_stream s, void *volume_scratch, int vol_dtype, const v
         return XPR_OK;
     }
     hipStream_t st = (hipStream_t)s;
-    float best = 3.4e38f;
-    int pick = kBwdJ;
+    float times[sizeof(kBwdJs) / sizeof(kBwdJs[0])];
+    int ti = 0;
     for (int J : kBwdJs) {
         auto launch = [&]() {
             switch (vol_dtype) {
@@ -821,13 +835,9 @@ int xpr_tune_backward(xpr_stream s, void *volume_scratch, int vol_dtype, const v
             default: return backward_impl<BF16, BF16>(st, volume_scratch, err, projdata, *a, sh, nullptr, J);
             }
         };
-        float ms = 0.0f;
-        if (int rc = time_best(st, 3, launch, &ms)) return rc;
-        if (ms < best) {
-            best = ms;
-            pick = J;
-        }
+        if (int rc = time_best(st, 3, launch, &times[ti++])) return rc;
     }
+    const int pick = pick_shape(kBwdJs, times, backward_default_j(sh.nx));
     tune_store(tune_key(kTuneBackward, vol_dtype, err_dtype, *a, sh), pick);
     *choice = pick;
     return XPR_OK;
