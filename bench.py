@@ -519,10 +519,15 @@ def main():
         for ev_ in le[1]:
             launches["k_backward"].append(prev.elapsed_time(ev_))
             prev = ev_
+    steps_ms = np.array([e[0].elapsed_time(e[4]) for e in evs])
+    secondary_steps = {"median": round(float(np.median(steps_ms)), 4),
+                       "p99": round(float(np.percentile(steps_ms, 99)), 4),
+                       "n": int(steps_ms.size)}
     acc_bytes = 8
     comp = {"k_forward_bin": v_slab * sbytes + 2 * A_s * D * D * acc_bytes,
             "k_backward": 2 * v_slab * sbytes + A_s * D * D * sbytes}
-    secondary = {"launch_ms": {}, "compulsory_hbm": {}, "valu": {},
+    secondary = {"sub_iteration_ms": secondary_steps, "launch_ms": {}, "compulsory_hbm": {},
+                 "valu": {},
                  "flops_per_voxel_view": {"algorithmic": 27, "executed": 13,
                                           "note": "3 dot4 as 12 FMA + rcp + 2 mul + sub; the "
                                                   "kernels hoist the (y,z,w) part of each dot "
