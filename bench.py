@@ -64,6 +64,12 @@ def parse():
                     help="out-of-core run: 2048^3 grid, 2048 views of 2048^2, method_2 under a "
                          "16 GB device budget, it=2, sit=16, through xpr_reconstruct_mem")
     ap.add_argument("--ooc-det", type=int, default=2048)
+    ap.add_argument("--ooc-views", type=int, default=0, help="views (default: detector width)")
+    ap.add_argument("--ooc-method", default="method_2",
+                    choices=["method_1a", "method_1b", "method_1c", "method_2"])
+    ap.add_argument("--ooc-it", type=int, default=2)
+    ap.add_argument("--ooc-sit", type=int, default=16)
+    ap.add_argument("--ooc-slab", type=int, default=64, help="slab height of method_1a/1b/1c")
     ap.add_argument("--jobs", type=int, default=24, help="jobs of the serving run")
     ap.add_argument("--rate", type=float, default=4.0, help="mean arrivals per second")
     ap.add_argument("--sizes", default="128,384,768",
@@ -300,7 +306,8 @@ def ooc_main(args):
 
     dev = torch.device("cuda:0")
     D = args.ooc_det
-    N = D
+    N = args.ooc_views or D
+    IT, S, method = args.ooc_it, args.ooc_sit, args.ooc_method
     geom = Geometry(dwidth=D, dheight=D, projections=N, so=100.0, sd=1000.0, px=0.1)
     _, grid = geom.projdata()
     W, H, vx = grid["vwidth"], grid["vheight"], grid["vx"]
@@ -318,14 +325,14 @@ def ooc_main(args):
             log(f"projections {v0}/{N}")
     log(f"{N} projections of {D}x{D} synthesised in {time.perf_counter() - t0:.1f} s")
     vol = np.empty((H, W, W), np.float32)
-    params = {"recon": {"method": "method_2", "it": 2, "sit": 16, "mlog": True, "weight": 1.0,
+    params = {"recon": {"method": method, "it": IT, "sit": S, "mlog": True, "weight": 1.0,
                         "weight_factor": 0.9},
               "geometry": {"so": 100.0, "sd": 1000.0},
               "detector": {"px": 0.1},
               "xpr": {"device_budget_gb": 16.0, "jitter_seed": 7, "layer_store": "ram",
-                      "quiet": True}}
+                      "slab_height": args.ooc_slab, "quiet": True}}
     rec = C.create_string_buffer(1 << 20)
-    log("reconstruction (method_2, 16 GB budget) ...")
+    log(f"reconstruction ({method}, 16 GB budget) ...")
     t0 = time.perf_counter()
     rc = _lib.load().xpr_reconstruct_mem(json.dumps(params).encode(), proj.ctypes.data_as(C.c_void_p),
                                          D, D, N, vol.ctypes.data_as(C.c_void_p), rec, len(rec))
@@ -335,8 +342,8 @@ def ooc_main(args):
     record = json.loads(rec.value.decode() or "{}")
     del proj
     dev_ms = sum(record.get("subiteration_ms", []))
-    A_s = N // 16
-    vv = 2.0 * W * W * H * A_s * (2 * 16) - W * W * H * A_s      # first pass skips the forward
+    A_s = N // S
+    vv = 2.0 * W * W * H * A_s * (IT * S) - W * W * H * A_s      # first pass skips the forward
     # error against the phantom, layer blocks on the device
     sq, mx, n = 0.0, 0.0, 0
     for l0 in range(0, H, 128):
@@ -358,8 +365,10 @@ def ooc_main(args):
            "maxabs_vs_phantom": mx, "finite": finite,
            "data": "synthetic: 40 random ellipsoids + 200 spheres r 2-6 voxels mu 0.05/voxel, "
                    "FP64 line integrals, Poisson I0=5e4",
-           "config": {"workload": f"{W}^2x{H} grid, {N} views of {D}x{D}, it=2 sit=16, "
-                                  "method_2, device budget 16 GB, volume and projections in "
+           "config": {"workload": f"{W}^2x{H} grid, {N} views of {D}x{D}, it={IT} sit={S}, "
+                                  f"{method}" + (f" slab {args.ooc_slab}" if method != "method_2"
+                                                 else "") +
+                                  ", device budget 16 GB, volume and projections in "
                                   "host memory (value = nominal voxel-views / wall time)"}}
     print(json.dumps(out), flush=True)
 
