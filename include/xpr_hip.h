@@ -167,8 +167,8 @@ int xpr_preprocess(xpr_stream s, const float *in, void *out, int out_dtype, uint
  * xpr_backward calls of that shape use it.  The shapes change only how work is
  * spread over lanes and paths: results are identical whatever is chosen.
  * Forward: LDS tile capacity (entries); the scratch sums receive several
- * forward passes.  Backward: voxels per lane (2, 4, 8, 16); the scratch volume
- * receives several updates.  *choice receives the shape kept. */
+ * forward passes.  Backward: voxels per lane (4, 8, 16: 1, 2 or 4 x positions
+ * in each of 4 layers); the scratch volume receives several updates.  *choice receives the shape kept. */
 int xpr_tune_forward(xpr_stream s, const void *volume, int vol_dtype, void *sums_scratch,
                      int acc_bits, const float *projdata, const xpr_proj_args *args, int *choice);
 int xpr_tune_backward(xpr_stream s, void *volume_scratch, int vol_dtype, const void *err,

@@ -3,7 +3,7 @@
 The forward's exact/inside views and the backward's inside segments skip the
 per-voxel bounds test when a rounding bound proves it redundant.  Over seeded
 random scans (source and detector distances, pixel size, detector offsets,
-one or two modules, slabs, cropped row bands, origin jitter) the sums and the
+detector roll, one or two modules, slabs, cropped row bands, origin jitter) the sums and the
 updated volume must be bit-identical with those paths switched off, and the
 forward sums must equal the oracle's."""
 import numpy as np
@@ -13,6 +13,21 @@ from oracle import oracle as O
 from tests.helpers import geom_args, oracle_geom, projdata_for, small_spec
 
 pytestmark = pytest.mark.gpu
+
+
+def _roll(pd, deg, w, h):
+    """The matrices of a detector rolled by deg about its centre pixel: the u
+    and v rows turn in the detector plane (float64, rounded to float32), so
+    the u row gains a layer coefficient and the w row keeps its own."""
+    t = np.deg2rad(deg)
+    c, s = np.cos(t), np.sin(t)
+    cu, cv = w / 2.0, h / 2.0
+    m = pd.astype(np.float64)
+    u, v, wr = m[:, 0:4], m[:, 4:8], m[:, 12:16]
+    out = pd.copy()
+    out[:, 0:4] = c * u - s * v + (cu - c * cu + s * cv) * wr
+    out[:, 4:8] = s * u + c * v + (cv - s * cu - c * cv) * wr
+    return out
 
 
 def _case(seed):
@@ -31,6 +46,8 @@ def _case(seed):
     nv = int(rng.integers(4, min(n, 24) + 1))
     views = sorted(rng.choice(n, nv, replace=False).tolist())
     pd, grid = projdata_for(spec, views)
+    if rng.random() < 0.5:
+        pd = _roll(pd, float(rng.uniform(-2.0, 2.0)), w, h)
     W, H = grid["vwidth"], grid["vheight"]
     l0 = int(rng.integers(0, H // 2))
     l1 = int(rng.integers(l0 + 1, H + 1))

@@ -98,7 +98,7 @@ __device__ __forceinline__ SlabShape batch_slab_shape(const xpr_proj_args &a, in
     s.nk = a.vmax[2] - a.vmin[2];
     s.nseg = (s.nx + 64 * J - 1) / (64 * J);
     s.row_base = 0;
-    s.nrows = (int64_t)s.nl * s.nk;
+    s.nrows = backward_rows(s.nl, s.nk, kBwdL);
     return s;
 }
 
@@ -140,15 +140,16 @@ __global__ __launch_bounds__(kThreads, 4) void k_batch_forward(const Table *__re
 }
 
 // Backward units are the single-job launch's workgroups: segment unit % nseg
-// of the kWaves adjacent rows of row block unit / nseg, one row per wave, run
-// together so the four rows (which gather nearly the same detector pixels)
-// share the CU's L1.
+// of the kWaves adjacent wave rows (layer group, depth) of row block
+// unit / nseg, one row per wave, run together so the four rows (which gather
+// nearly the same detector pixels) share the CU's L1.
 __global__ __launch_bounds__(kThreads, 7) void k_batch_backward(const Table *__restrict__ t,
                                                                 uint32_t *counters)
 {
     __shared__ uint32_t slot;
-    __shared__ float4 rec[kWaves * 128];              // per-wave view-record tables
-    float4 *wrec = rec + (threadIdx.x >> 6) * 128;
+    constexpr int kRec = 64 * BwdRecord<kBwdL>::kQuads;
+    __shared__ float4 rec[kWaves * kRec];             // per-wave view-record tables
+    float4 *wrec = rec + (threadIdx.x >> 6) * kRec;
     const uint32_t ph = xpr_batch::kBackward;
     const uint32_t total = uniform_u32(&t->total[ph]);
     if (total == 0) return;
@@ -162,15 +163,15 @@ __global__ __launch_bounds__(kThreads, 7) void k_batch_backward(const Table *__r
         const int seg = (int)(unit % s.nseg);
         switch (it.dt_dst) {                                // volume and error share the dtype
         case XPR_F32:
-            backward_row<F32, F32, kBwdJ, false>((float *)it.dst, (const float *)it.err, it.pd, it.a, s,
+            backward_row<F32, F32, kBwdJ, kBwdL, false>((float *)it.dst, (const float *)it.err, it.pd, it.a, s,
                                                  row, seg, nullptr, wrec);
             break;
         case XPR_F16:
-            backward_row<F16, F16, kBwdJ, false>((uint16_t *)it.dst, (const uint16_t *)it.err, it.pd,
+            backward_row<F16, F16, kBwdJ, kBwdL, false>((uint16_t *)it.dst, (const uint16_t *)it.err, it.pd,
                                                  it.a, s, row, seg, nullptr, wrec);
             break;
         default:
-            backward_row<BF16, BF16, kBwdJ, false>((uint16_t *)it.dst, (const uint16_t *)it.err, it.pd,
+            backward_row<BF16, BF16, kBwdJ, kBwdL, false>((uint16_t *)it.dst, (const uint16_t *)it.err, it.pd,
                                                    it.a, s, row, seg, nullptr, wrec);
             break;
         }

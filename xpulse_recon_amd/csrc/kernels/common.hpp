@@ -172,6 +172,12 @@ __device__ __forceinline__ bool project(const ViewConsts &c, float px, float lin
     return inb;
 }
 
+// lane `lane`'s x, uniform (v_readlane)
+__device__ __forceinline__ float lane_float(float x, uint32_t lane)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), lane));
+}
+
 // detector view of launch view `view` (uniform: every caller passes a uniform view)
 __device__ __forceinline__ uint32_t view_of(const xpr_proj_args &a, uint32_t view)
 {
@@ -237,13 +243,14 @@ __device__ __forceinline__ float sample_or_zero<BF16>(__amdgpu_buffer_rsrc_t r, 
 // ---------------------------------------------------------------------------
 constexpr int kWaves = 4;
 constexpr int kThreads = 64 * kWaves;
-constexpr int kBwdJ = 4;          // voxels per lane of the backward (default shape)
+constexpr int kBwdJ = 2;          // backward default shape: x positions per lane
+constexpr int kBwdL = 4;          // and layers per wave row
 
 struct SlabShape {
     int32_t nx, nl, nk;
     int32_t nseg;          // x segments per row
     int64_t row_base;      // first row of this launch
-    int64_t nrows;         // rows in the slab
+    int64_t nrows;         // wave rows in the slab (backward: layer groups x depths)
     int32_t inside = 1;    // 1: backward views wholly in bounds skip the per-voxel test
 };
 

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "../../../include/xpr_hip.h"
 #include "common.hpp"
@@ -13,26 +14,21 @@
 namespace {
 
 // ---------------------------------------------------------------------------
-// Forward with LDS binning.  A workgroup owns a brick of 64 (x) x 4 (layers,
-// one per wave) x 16 (depths).  Per view the brick's 8 corners bound its
-// detector footprint (a rounding bound, or one pixel of margin, covers float
-// rounding, see bin_view); voxel-views accumulate there with LDS u64 atomics and the non-zero footprint pixels go
-// to the detector sums with one global atomic each.  Two tiles alternate: the
-// flush of view v also clears its tile, so one barrier per view separates
-// splat and flush.  Footprints above the tile capacity and corners behind the
-// source take the direct path (one global atomic per voxel-view); bricks
-// holding Q.22 values above 2^32 take it with 64-bit values.  Every path adds
-// the same integers: the sums are bit-identical whatever the path, the lane
-// order or the launch order.
+// Forward with LDS binning.  A workgroup owns a brick of 64 (x) x 4 (layers)
+// x 16 (depths).  Per view the brick's 8 corners bound its detector footprint
+// (a rounding bound, or one pixel of margin, covers float rounding, see
+// bin_view); voxel-views accumulate there with LDS u64 atomics and the
+// non-zero footprint pixels go to the detector sums with one global atomic
+// each.  Two tiles alternate: the flush of view v also clears its tile, so
+// one barrier per view separates splat and flush.  Footprints above the tile
+// capacity and corners behind the source take the direct path (one global
+// atomic per voxel-view); bricks holding Q.22 values above 2^32 take it with
+// 64-bit values.  Every path adds the same integers: the sums are
+// bit-identical whatever the path, the lane order or the launch order.
 //
-// Lane order per view.  Voxels along x project to neighbouring detector
-// columns when the ray is across x, but onto one or two columns when the ray
-// runs along x (views near 90 and 270 degrees): 64 lanes of one x-row would
-// then hit the same LDS entries in one atomic.  Each wave therefore holds its
-// 1024 voxels twice, as 64 x-lanes by 16 depths and as 16 depth-lanes x 4
-// x-groups by 16 x-steps, and a view uses the order whose lanes spread more
-// across columns.  The per-(layer, depth) part of the dot products is
-// computed once per view by the depth-lanes; x-order reads it with readlane.
+// Each wave splats columns of the brick's 4 layers at one (x, depth), so a
+// view whose W does not change across the layers computes W and its
+// reciprocal once per column (see the coherence modes below).
 //
 // The footprint, bounds thresholds, matrix rows, order and fast-path flags
 // of a chunk of 32 views are computed once per workgroup (8 corner lanes per
@@ -248,150 +244,199 @@ __device__ __noinline__ void forward_direct_u64(const typename VS::T *__restrict
     }
 }
 
-// one voxel-view of a splat: its Q.22 value goes to its footprint tile entry
-// (voxel-views out of bounds add 0 to the lane's sink slot); one off the tile
-// goes straight to the detector sums.  Entries are counted from the tile
-// base; sink_e is the sink slot's entry relative to it.
-template <bool CROP, bool FAST, typename AccT>
-__device__ __forceinline__ void splat_one(const ViewConsts &c, float px, uint32_t qq,
-                                          float line_off, int32_t u0, int32_t v0, int32_t tw,
-                                          int32_t th, bool use_tile,
-                                          unsigned long long *__restrict__ tl, uint32_t sink_e,
-                                          AccT *__restrict__ vsum, uint32_t det_width)
-{
-    uint32_t iu, iv;
-    const bool inb = project<CROP, FAST>(c, px, line_off, iu, iv);
-    const uint32_t tu = iu - (uint32_t)u0;
-    const uint32_t tv = iv - (uint32_t)v0;
-    const bool hit = inb & use_tile & (tu < (uint32_t)tw) & (tv < (uint32_t)th);
-    const uint32_t e = __umul24(tv, (uint32_t)tw) + tu;
-    atomicAdd(&tl[hit ? e : sink_e], (unsigned long long)(hit ? qq : 0u));
-    if (inb & !hit & (qq != 0u))      // off the tile: direct global add
-        global_add(vsum + (__umul24(iv, det_width) + iu), (AccT)qq);
-}
-
-// one voxel-view of an exact view (kBinExact): the tile test is the bounds test
-template <bool CROP>
-__device__ __forceinline__ void splat_exact(const ViewConsts &c, float px, uint32_t qq,
-                                            float line_off, int32_t u0, int32_t v0, int32_t tw,
-                                            int32_t th, unsigned long long *__restrict__ tl,
-                                            uint32_t sink_e)
-{
-    const float X = __builtin_fmaf(c.c0x, px, c.k0);
-    const float Y = __builtin_fmaf(c.c1x, px, c.k1);
-    const float W = __builtin_fmaf(c.c3x, px, c.k3);
-    const float r0 = __builtin_amdgcn_rcpf(W);
-    const float r = __builtin_fmaf(__builtin_fmaf(-W, r0, 1.0f), r0, r0);
-    const float u = X * r;
-    const float v = CROP ? Y * r - line_off : Y * r;
-    const uint32_t tu = (uint32_t)pixel_of(u) - (uint32_t)u0;
-    const uint32_t tv = (uint32_t)pixel_of(v) - (uint32_t)v0;
-    const bool hit = (tu < (uint32_t)tw) & (tv < (uint32_t)th);
-    const uint32_t e = __umul24(tv, (uint32_t)tw) + tu;
-    atomicAdd(&tl[hit ? e : sink_e], (unsigned long long)(hit ? qq : 0u));
-}
-
-using LdsU64 = __attribute__((address_space(3))) unsigned long long;
-
 // LDS byte address of a pointer into shared memory
-__device__ __forceinline__ uint32_t lds_address(const unsigned long long *p)
+template <typename T>
+__device__ __forceinline__ uint32_t lds_address(const T *p)
 {
-    return (uint32_t)(uintptr_t)(const LdsU64 *)p;
+    return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) T *)p;
 }
 
-// one voxel-view of an inside view (kBinInside): it lands in the tile.  tb is
-// the LDS address of the tile base minus the entry of pixel (0, 0), so the
-// entry needs no offset; tw8 is the tile row in bytes.  The address is one
-// shift-add and one 24-bit multiply-add (left to itself the compiler spends
-// a multiply, two shifts and a three-way add).
-template <bool CROP>
-__device__ __forceinline__ void splat_inside(const ViewConsts &c, float px, uint32_t qq,
-                                             float line_off, uint32_t tw8, uint32_t tb)
+// How a view's per-voxel projection is evaluated for one wave (uniform).
+// The canonical X, Y, W of a voxel are fmaf(c, px, k) with the row terms k of
+// its (layer, depth).  When the w row terms of a depth are the same float in
+// the brick's 4 layers (a zero layer coefficient in ProjData float 13, as in
+// every detector the reference builds, or the rounding residue a roll about
+// the beam axis leaves there), W and its reciprocal are computed once per
+// (x, depth) column and serve the 4 layers (kColW); when the u row terms agree
+// too (an upright detector), so does u (kColU).  Equal floats may differ in
+// the sign of a zero: a zero X gives the same pixel and bounds result with
+// either sign, and the views that share W are fast views, whose W is not 0.
+enum : int { kColU = 0, kColW = 1, kColNone = 2 };
+// what a voxel-view does with its pixel
+enum : int {
+    kSplatInside = 0,    // kBinInside: lands in the tile, no test
+    kSplatExact = 1,     // kBinExact: the tile test is the bounds test
+    kSplatChecked = 2,   // bounds test; off the tile (or no tile): global atomic
+};
+
+template <typename AccT, typename TileT>
+struct SplatTarget {
+    int32_t u0, v0, tw, th;      // tile origin and extent
+    uint32_t twb, tb;            // inside: tile row in bytes, LDS address of pixel (0, 0)
+    TileT *tl;                   // tile
+    uint32_t sk;                 // this lane's sink entry, relative to tl
+    bool use_tile;
+    AccT *vsum;                  // the view's detector sums
+    uint32_t det_width;
+};
+
+template <typename TileT>
+__device__ __forceinline__ void lds_add(uint32_t addr, uint32_t q)
 {
-    const float X = __builtin_fmaf(c.c0x, px, c.k0);
-    const float Y = __builtin_fmaf(c.c1x, px, c.k1);
-    const float W = __builtin_fmaf(c.c3x, px, c.k3);
-    const float r0 = __builtin_amdgcn_rcpf(W);
-    const float r = __builtin_fmaf(__builtin_fmaf(-W, r0, 1.0f), r0, r0);
-    const float u = X * r;
-    const float v = CROP ? Y * r - line_off : Y * r;
-    const uint32_t col = ((uint32_t)pixel_of(u) << 3) + tb;
-    uint32_t addr;
-    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(addr) : "v"((uint32_t)pixel_of(v)), "s"(tw8), "v"(col));
-    __hip_atomic_fetch_add((LdsU64 *)(uintptr_t)addr, (unsigned long long)qq, __ATOMIC_RELAXED,
+    using L = __attribute__((address_space(3))) TileT;
+    __hip_atomic_fetch_add((L *)(uintptr_t)addr, (TileT)q, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// One brick of the forward (b < s.nbricks, uniform over the workgroup).  lds
-// holds two tiles and one sink slot per lane ([2*kTileCap + kThreads]); the
-// tiles are zero on entry and left zero.
-template <typename VS, typename AccT, bool CROP>
-__device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__ vol,
-                                              AccT *__restrict__ sums,
-                                              const float *__restrict__ pd,
-                                              const xpr_proj_args &a, const BrickShape &s,
-                                              int64_t b, unsigned long long *__restrict__ lds,
-                                              ViewBin *__restrict__ bins)
+// one voxel-view: its Q.22 value goes to its pixel's tile entry (inside: the
+// address is one shift-add and one 24-bit multiply-add); voxel-views out of
+// bounds add 0 to the lane's sink slot, so the splat needs no branch; a
+// checked voxel-view in bounds but off the tile goes straight to the sums
+template <int SPLAT, typename AccT, typename TileT>
+__device__ __forceinline__ void deposit(float u, float v, uint32_t q, const ViewConsts &c,
+                                        const SplatTarget<AccT, TileT> &t)
+{
+    constexpr uint32_t kShift = sizeof(TileT) == 8 ? 3u : 2u;
+    if (SPLAT == kSplatInside) {
+        const uint32_t col = ((uint32_t)pixel_of(u) << kShift) + t.tb;
+        uint32_t addr;
+        asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(addr) : "v"((uint32_t)pixel_of(v)), "s"(t.twb), "v"(col));
+        lds_add<TileT>(addr, q);
+    } else if (SPLAT == kSplatExact) {
+        const uint32_t tu = (uint32_t)pixel_of(u) - (uint32_t)t.u0;
+        const uint32_t tv = (uint32_t)pixel_of(v) - (uint32_t)t.v0;
+        const bool hit = (tu < (uint32_t)t.tw) & (tv < (uint32_t)t.th);
+        const uint32_t e = __umul24(tv, (uint32_t)t.tw) + tu;
+        atomicAdd(&t.tl[hit ? e : t.sk], (TileT)(hit ? q : 0u));
+    } else {
+        const bool inb = (u >= c.ulo) & (u < c.uhi) & (v >= c.vlo) & (v < c.vhi);
+        const uint32_t iu = (uint32_t)pixel_of(u), iv = (uint32_t)pixel_of(v);
+        const uint32_t tu = iu - (uint32_t)t.u0;
+        const uint32_t tv = iv - (uint32_t)t.v0;
+        const bool hit = inb & t.use_tile & (tu < (uint32_t)t.tw) & (tv < (uint32_t)t.th);
+        const uint32_t e = __umul24(tv, (uint32_t)t.tw) + tu;
+        atomicAdd(&t.tl[hit ? e : t.sk], (TileT)(hit ? q : 0u));
+        if (inb & !hit & (q != 0u))      // off the tile: direct global add
+            global_add(t.vsum + (__umul24(iv, t.det_width) + iu), (AccT)q);
+    }
+}
+
+// The 4 layers of one (x, depth) column of the brick for one view: px, the
+// Q.22 values q[l] and the row terms k0/k1/k3[l] of each layer.  FAST: every
+// W of the view is in the fast reciprocal's range.
+template <int MODE, int SPLAT, bool FAST, bool CROP, typename AccT, typename TileT>
+__device__ __forceinline__ void splat_column(float px, const uint32_t (&q)[kWaves],
+                                             const float (&k0)[kWaves], const float (&k1)[kWaves],
+                                             const float (&k3)[kWaves], const ViewConsts &c,
+                                             float line_off, const SplatTarget<AccT, TileT> &t)
+{
+    auto rcp = [](float w) {
+        if (FAST) {
+            const float r0 = __builtin_amdgcn_rcpf(w);
+            return __builtin_fmaf(__builtin_fmaf(-w, r0, 1.0f), r0, r0);
+        }
+        return recip(w);
+    };
+    float r = 0.0f, u = 0.0f;
+    if (MODE != kColNone) r = rcp(__builtin_fmaf(c.c3x, px, k3[0]));
+    if (MODE == kColU) u = __builtin_fmaf(c.c0x, px, k0[0]) * r;
+#pragma unroll
+    for (int l = 0; l < kWaves; ++l) {
+        if (MODE == kColNone) r = rcp(__builtin_fmaf(c.c3x, px, k3[l]));
+        if (MODE != kColU) u = __builtin_fmaf(c.c0x, px, k0[l]) * r;
+        const float y = __builtin_fmaf(c.c1x, px, k1[l]) * r;
+        deposit<SPLAT, AccT, TileT>(u, CROP ? y - line_off : y, q[l], c, t);
+    }
+}
+
+template <int V>
+using IntTag = std::integral_constant<int, V>;
+
+// Calls f(mode, splat, fast) as compile-time tags for the compiled
+// combinations: inside and exact views (fast) in every mode; checked views
+// with the per-layer projection, which is correct for every view.
+template <typename F>
+__device__ __forceinline__ void splat_dispatch(int splat, bool fast, int mode, F &&f)
+{
+    if (splat == kSplatChecked) {
+        if (fast) f(IntTag<kColNone>{}, IntTag<kSplatChecked>{}, std::true_type{});
+        else f(IntTag<kColNone>{}, IntTag<kSplatChecked>{}, std::false_type{});
+    } else if (splat == kSplatInside) {
+        if (mode == kColU) f(IntTag<kColU>{}, IntTag<kSplatInside>{}, std::true_type{});
+        else if (mode == kColW) f(IntTag<kColW>{}, IntTag<kSplatInside>{}, std::true_type{});
+        else f(IntTag<kColNone>{}, IntTag<kSplatInside>{}, std::true_type{});
+    } else {
+        if (mode == kColU) f(IntTag<kColU>{}, IntTag<kSplatExact>{}, std::true_type{});
+        else if (mode == kColW) f(IntTag<kColW>{}, IntTag<kSplatExact>{}, std::true_type{});
+        else f(IntTag<kColNone>{}, IntTag<kSplatExact>{}, std::true_type{});
+    }
+}
+
+constexpr int kD = kBK / kWaves;     // depths per wave in x-lane order, x steps in depth-lane order
+static_assert(kD == 4 && kWaves == 4, "a wave holds 4 x 4 columns of the brick's 4 layers");
+
+// x-lane order: depth 4 wave + d of the brick; lane 4 d + l holds the row
+// terms of (depth d, layer l)
+template <int MODE, int SPLAT, bool FAST, bool CROP, typename AccT, typename TileT>
+__device__ __forceinline__ void splat_x_order(float px, const uint32_t (&q)[kD][kWaves], float k0t,
+                                              float k1t, float k3t, const ViewConsts &c,
+                                              float line_off, const SplatTarget<AccT, TileT> &t)
+{
+#pragma unroll
+    for (int d = 0; d < kD; ++d) {
+        float K0[kWaves], K1[kWaves], K3[kWaves];
+#pragma unroll
+        for (int l = 0; l < kWaves; ++l) {
+            K0[l] = lane_float(k0t, 4 * d + l);
+            K1[l] = lane_float(k1t, 4 * d + l);
+            K3[l] = lane_float(k3t, 4 * d + l);
+        }
+        splat_column<MODE, SPLAT, FAST, CROP, AccT, TileT>(px, q[d], K0, K1, K3, c, line_off, t);
+    }
+}
+
+// depth-lane order: the lane's depth, x step xs
+template <int MODE, int SPLAT, bool FAST, bool CROP, typename AccT, typename TileT>
+__device__ __forceinline__ void splat_depth_order(const float (&px)[kD],
+                                                  const uint32_t (&q)[kD][kWaves],
+                                                  const float (&K0)[kWaves], const float (&K1)[kWaves],
+                                                  const float (&K3)[kWaves], const ViewConsts &c,
+                                                  float line_off, const SplatTarget<AccT, TileT> &t)
+{
+#pragma unroll
+    for (int xs = 0; xs < kD; ++xs)
+        splat_column<MODE, SPLAT, FAST, CROP, AccT, TileT>(px[xs], q[xs], K0, K1, K3, c, line_off, t);
+}
+
+// The views of one brick (see forward_brick), with TileT tile entries.
+template <typename VS, typename AccT, bool CROP, typename TileT>
+__device__ __forceinline__ void forward_views(AccT *__restrict__ sums, const float *__restrict__ pd,
+                                           const xpr_proj_args a, const BrickShape s, int32_t x0,
+                                           int32_t l0, int32_t k0, const uint32_t (&qx)[kD][kWaves],
+                                           const uint32_t (&qd)[kD][kWaves],
+                                           unsigned long long *__restrict__ lds,
+                                           ViewBin *__restrict__ bins)
 {
     const int lane = threadIdx.x & 63;
-    // wave-uniform: lets the compiler keep row and flush addressing scalar
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int32_t bx = (int32_t)(b % s.nbx);
-    b /= s.nbx;
-    const int32_t bk = (int32_t)(b % s.nbk);
-    const int32_t bl = (int32_t)(b / s.nbk);
-    const int32_t x0 = bx * 64, l0 = bl * kWaves, k0 = bk * kBK;
-    const int32_t l = l0 + wave;
     const int32_t dk = lane & 15, xg = lane >> 4;     // depth-lane order
-
-    // Q.22 voxel values in both lane orders (32 bits suffice unless some
-    // value is >= 1024).  Raw buffer loads from this wave's brick corner
-    // (x0, l, k0): out-of-slab voxels read 0 without a branch, so all 32
-    // loads are in flight together.
-    uint32_t qx[kBK], qd[kBK];
-    bool big = false;
-    {
-        constexpr uint32_t esz = (uint32_t)sizeof(typename VS::T);
-        const int64_t base = ((int64_t)l * s.nk + k0) * s.nx + x0;
-        const int64_t left = (int64_t)s.nl * s.nk * s.nx - base;      // voxels from the corner on
-        const uint32_t bytes = l < s.nl ? (uint32_t)min<int64_t>(left * esz, 0x7fffffff) : 0u;
-        const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-            (void *)(vol + (l < s.nl ? base : 0)), 0, (int)bytes, kBufferWord3);
-        auto q22 = [&](float v, bool track) {
-            const float sc = v * kQ22One;               // FLOAT_TO_FIXED (constants.cl:22)
-            const bool pos = sc > 0.0f, over = sc >= 4294967296.0f;
-            if (sizeof(AccT) == 8 && track) big |= pos & over;
-            return pos ? (over ? 0xffffffffu : (uint32_t)sc) : 0u;
-        };
-        float vx_[kBK], vd_[kBK];
-#pragma unroll
-        for (int i = 0; i < kBK; ++i) {
-            const bool in = (x0 + lane < s.nx) & (k0 + i < s.nk);
-            const uint32_t off = (uint32_t)(i * s.nx + lane) * esz;
-            vx_[i] = load_or_zero<VS>(rsrc, in ? off : kOutOfRange);
-        }
-#pragma unroll
-        for (int j = 0; j < kBK; ++j) {
-            const bool in = (x0 + xg * 16 + j < s.nx) & (k0 + dk < s.nk);
-            const uint32_t off = (uint32_t)(dk * s.nx + xg * 16 + j) * esz;
-            vd_[j] = load_or_zero<VS>(rsrc, in ? off : kOutOfRange);
-        }
-#pragma unroll
-        for (int i = 0; i < kBK; ++i) qx[i] = q22(vx_[i], true);
-#pragma unroll
-        for (int j = 0; j < kBK; ++j) qd[j] = q22(vd_[j], false);
-    }
-    if (__syncthreads_or(big)) {
-        forward_direct_u64<VS, AccT, CROP>(vol, sums, pd, a, s, x0, l0, k0);
-        return;
-    }
+    TileT *__restrict__ tiles = (TileT *)lds;
     const float px_x = __builtin_fmaf((float)(x0 + lane + a.vmin[0]), a.voxel[0], a.origin[0]);
-    float px_d[kBK];
+    float px_d[kD];
 #pragma unroll
-    for (int j = 0; j < kBK; ++j)
-        px_d[j] = __builtin_fmaf((float)(x0 + xg * 16 + j + a.vmin[0]), a.voxel[0], a.origin[0]);
-    const float py = __builtin_fmaf((float)(l + a.vmin[1]), a.voxel[1], a.origin[1]);
+    for (int xs = 0; xs < kD; ++xs)
+        px_d[xs] = __builtin_fmaf((float)(x0 + wave + 16 * xg + 4 * xs + a.vmin[0]), a.voxel[0],
+                                  a.origin[0]);
+    float py[kWaves];
+#pragma unroll
+    for (int l = 0; l < kWaves; ++l)
+        py[l] = __builtin_fmaf((float)(l0 + l + a.vmin[1]), a.voxel[1], a.origin[1]);
     const float pz_d = __builtin_fmaf((float)(k0 + dk + a.vmin[2]), a.voxel[2], a.origin[2]);
+    // x-lane order row terms: lane i computes (depth 4 wave + i % 16 / 4, layer i % 4)
+    const int32_t ti = lane & 15;
+    const float pz_t = __builtin_fmaf((float)(k0 + wave * kD + (ti >> 2) + a.vmin[2]), a.voxel[2],
+                                      a.origin[2]);
+    const float py_ti = __builtin_fmaf((float)(l0 + (ti & 3) + a.vmin[1]), a.voxel[1], a.origin[1]);
 
     // brick corners (voxel indices inside the slab)
     const int32_t lo[3] = {x0, l0, k0};
@@ -417,19 +462,22 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
             const uint32_t m = c0 + vi;
             const uint32_t flags = __builtin_amdgcn_readfirstlane(bins[vi].flags);
             if (flags & kBinSkip) continue;
-            const int32_t u0 = __builtin_amdgcn_readfirstlane(bins[vi].u0);
-            const int32_t v0 = __builtin_amdgcn_readfirstlane(bins[vi].v0);
-            const int32_t tw = __builtin_amdgcn_readfirstlane(bins[vi].tw);
-            const int32_t th = __builtin_amdgcn_readfirstlane(bins[vi].th);
-            const bool use_tile = (flags & kBinTile) != 0;
+            SplatTarget<AccT, TileT> t;
+            t.u0 = __builtin_amdgcn_readfirstlane(bins[vi].u0);
+            t.v0 = __builtin_amdgcn_readfirstlane(bins[vi].v0);
+            t.tw = __builtin_amdgcn_readfirstlane(bins[vi].tw);
+            t.th = __builtin_amdgcn_readfirstlane(bins[vi].th);
+            t.use_tile = (flags & kBinTile) != 0;
             // (a uniform branch spares the single-module case the integer division)
             const uint32_t view =
                 __builtin_amdgcn_readfirstlane(a.modules == 1 ? m : m / a.modules);
-            AccT *__restrict__ vsum = sums + (uint64_t)view_of(a, view) * view_elems;
-            unsigned long long *__restrict__ tl = lds + buf * kTileCap;
-            const uint32_t sk = (uint32_t)((2 - buf) * kTileCap) + threadIdx.x;
-            // this lane's (layer, depth) row terms: depth k0 + dk (row_terms
-            // on the record's copy of the matrix)
+            t.vsum = sums + (uint64_t)view_of(a, view) * view_elems;
+            t.det_width = a.det_width;
+            t.tl = tiles + buf * kTileCap;
+            t.sk = (uint32_t)((2 - buf) * kTileCap) + threadIdx.x;
+            t.twb = (uint32_t)sizeof(TileT) * (uint32_t)t.tw;
+            t.tb = lds_address(t.tl) - (uint32_t)sizeof(TileT) *
+                                           (__umul24((uint32_t)t.v0, (uint32_t)t.tw) + (uint32_t)t.u0);
             const ViewBin &bv = bins[vi];
             ViewConsts c;
             c.ulo = bv.ulo;
@@ -439,89 +487,60 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
             c.c0x = bv.m[0];
             c.c1x = bv.m[4];
             c.c3x = bv.m[8];
-            c.k0 = __builtin_fmaf(bv.m[1], py, __builtin_fmaf(bv.m[2], pz_d, bv.m[3]));
-            c.k1 = __builtin_fmaf(bv.m[5], py, __builtin_fmaf(bv.m[6], pz_d, bv.m[7]));
-            c.k3 = __builtin_fmaf(bv.m[9], py, __builtin_fmaf(bv.m[10], pz_d, bv.m[11]));
-            const uint32_t dw = a.det_width;
-            if (flags & kBinInside) {
-                const uint32_t tb = lds_address(tl) -
-                                    8u * (__umul24((uint32_t)v0, (uint32_t)tw) + (uint32_t)u0);
-                if (flags & kBinDepth) {
+            c.k0 = c.k1 = c.k3 = 0.0f;
+            const int splat = (flags & kBinInside) ? kSplatInside
+                              : (flags & kBinExact) ? kSplatExact : kSplatChecked;
+            const bool fast = (flags & kBinFast) != 0;
+            if (flags & kBinDepth) {
+                // depth-lane order: this lane's depth, the 4 layers' row terms
+                const float t0 = __builtin_fmaf(bv.m[2], pz_d, bv.m[3]);
+                const float t1 = __builtin_fmaf(bv.m[6], pz_d, bv.m[7]);
+                const float t3 = __builtin_fmaf(bv.m[10], pz_d, bv.m[11]);
+                float K0[kWaves], K1[kWaves], K3[kWaves];
+                bool wsame = true, usame = true;
 #pragma unroll
-                    for (int j = 0; j < kBK; ++j)
-                        splat_inside<CROP>(c, px_d[j], qd[j], line_off, 8u * (uint32_t)tw, tb);
-                } else {
-                    ViewConsts ci = c;
-#pragma unroll
-                    for (int i = 0; i < kBK; ++i) {
-                        ci.k0 = lane_float(c.k0, i);
-                        ci.k1 = lane_float(c.k1, i);
-                        ci.k3 = lane_float(c.k3, i);
-                        splat_inside<CROP>(ci, px_x, qx[i], line_off, 8u * (uint32_t)tw, tb);
-                    }
+                for (int l = 0; l < kWaves; ++l) {
+                    K0[l] = __builtin_fmaf(bv.m[1], py[l], t0);
+                    K1[l] = __builtin_fmaf(bv.m[5], py[l], t1);
+                    K3[l] = __builtin_fmaf(bv.m[9], py[l], t3);
+                    wsame &= K3[l] == K3[0];
+                    usame &= K0[l] == K0[0];
                 }
-            } else if (flags & kBinExact) {
-                if (flags & kBinDepth) {
-#pragma unroll
-                    for (int j = 0; j < kBK; ++j)
-                        splat_exact<CROP>(c, px_d[j], qd[j], line_off, u0, v0, tw, th, tl, sk);
-                } else {
-                    ViewConsts ci = c;
-#pragma unroll
-                    for (int i = 0; i < kBK; ++i) {
-                        ci.k0 = lane_float(c.k0, i);
-                        ci.k1 = lane_float(c.k1, i);
-                        ci.k3 = lane_float(c.k3, i);
-                        splat_exact<CROP>(ci, px_x, qx[i], line_off, u0, v0, tw, th, tl, sk);
-                    }
-                }
-            } else if (flags & kBinDepth) {
-                if (flags & kBinFast) {
-#pragma unroll
-                    for (int j = 0; j < kBK; ++j)
-                        splat_one<CROP, true, AccT>(c, px_d[j], qd[j], line_off, u0, v0, tw, th,
-                                                    use_tile, tl, sk, vsum, dw);
-                } else {
-#pragma unroll
-                    for (int j = 0; j < kBK; ++j)
-                        splat_one<CROP, false, AccT>(c, px_d[j], qd[j], line_off, u0, v0, tw, th,
-                                                     use_tile, tl, sk, vsum, dw);
-                }
+                const bool wcol = __builtin_amdgcn_ballot_w64(!wsame) == 0;
+                const bool ucol = wcol && __builtin_amdgcn_ballot_w64(!usame) == 0;
+                const int mode = ucol ? kColU : wcol ? kColW : kColNone;
+                splat_dispatch(splat, fast, mode, [&](auto M, auto S, auto F) __attribute__((always_inline)) {
+                    splat_depth_order<decltype(M)::value, decltype(S)::value, decltype(F)::value,
+                                      CROP, AccT, TileT>(px_d, qd, K0, K1, K3, c, line_off, t);
+                });
             } else {
-                // x-lane order: the row terms of depth k0 + i sit in lane i
-                ViewConsts ci = c;
-                if (flags & kBinFast) {
-#pragma unroll
-                    for (int i = 0; i < kBK; ++i) {
-                        ci.k0 = lane_float(c.k0, i);
-                        ci.k1 = lane_float(c.k1, i);
-                        ci.k3 = lane_float(c.k3, i);
-                        splat_one<CROP, true, AccT>(ci, px_x, qx[i], line_off, u0, v0, tw, th,
-                                                    use_tile, tl, sk, vsum, dw);
-                    }
-                } else {
-#pragma unroll
-                    for (int i = 0; i < kBK; ++i) {
-                        ci.k0 = lane_float(c.k0, i);
-                        ci.k1 = lane_float(c.k1, i);
-                        ci.k3 = lane_float(c.k3, i);
-                        splat_one<CROP, false, AccT>(ci, px_x, qx[i], line_off, u0, v0, tw, th,
-                                                     use_tile, tl, sk, vsum, dw);
-                    }
-                }
+                // x-lane order: lane 4 d + l computes the row terms of (depth
+                // 4 wave + d, layer l); the splat reads them with v_readlane
+                const float k0t = __builtin_fmaf(bv.m[1], py_ti, __builtin_fmaf(bv.m[2], pz_t, bv.m[3]));
+                const float k1t = __builtin_fmaf(bv.m[5], py_ti, __builtin_fmaf(bv.m[6], pz_t, bv.m[7]));
+                const float k3t = __builtin_fmaf(bv.m[9], py_ti, __builtin_fmaf(bv.m[10], pz_t, bv.m[11]));
+                const float k0f = __builtin_fmaf(bv.m[1], py[0], __builtin_fmaf(bv.m[2], pz_t, bv.m[3]));
+                const float k3f = __builtin_fmaf(bv.m[9], py[0], __builtin_fmaf(bv.m[10], pz_t, bv.m[11]));
+                const bool wcol = __builtin_amdgcn_ballot_w64(k3t != k3f) == 0;
+                const bool ucol = wcol && __builtin_amdgcn_ballot_w64(k0t != k0f) == 0;
+                const int mode = ucol ? kColU : wcol ? kColW : kColNone;
+                splat_dispatch(splat, fast, mode, [&](auto M, auto S, auto F) __attribute__((always_inline)) {
+                    splat_x_order<decltype(M)::value, decltype(S)::value, decltype(F)::value, CROP,
+                                  AccT, TileT>(px_x, qx, k0t, k1t, k3t, c, line_off, t);
+                });
             }
-            if (use_tile) {
+            if (t.use_tile) {
                 lds_barrier();                // splat of this view complete
                 // flush and clear; the next view splats into the other tile
                 // (row pointers are wave-uniform; a lane adds a 32-bit column offset)
-                for (int rr = wave; rr < th; rr += kWaves) {
-                    AccT *dst = vsum + (uint32_t)(v0 + rr) * dw + (uint32_t)u0;
-                    unsigned long long *src = tl + rr * tw;
-                    for (uint32_t cc = (uint32_t)lane; cc < (uint32_t)tw; cc += 64u) {
-                        const unsigned long long t = src[cc];
-                        if (t != 0ull) {
-                            global_add(dst + cc, (AccT)t);
-                            src[cc] = 0ull;
+                for (int rr = wave; rr < t.th; rr += kWaves) {
+                    AccT *dst = t.vsum + (uint32_t)(t.v0 + rr) * t.det_width + (uint32_t)t.u0;
+                    TileT *src = t.tl + rr * t.tw;
+                    for (uint32_t cc = (uint32_t)lane; cc < (uint32_t)t.tw; cc += 64u) {
+                        const TileT v = src[cc];
+                        if (v != 0) {
+                            global_add(dst + cc, (AccT)v);
+                            src[cc] = 0;
                         }
                     }
                 }
@@ -529,6 +548,104 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
             }
         }
     }
+}
+
+// One brick of the forward (b < s.nbricks, uniform over the workgroup).  lds
+// holds two tiles and one sink slot per lane ([2*kTileCap + kThreads]); the
+// tiles are zero on entry and left zero.
+//
+// Lane order per view.  Voxels along x project to neighbouring detector
+// columns when the ray is across x, but onto one or two columns when the ray
+// runs along x (views near 90 and 270 degrees): 64 lanes of one x-row would
+// then hit the same LDS entries in one atomic.  Each wave therefore holds 1024
+// voxels of the brick twice, each time as 256 columns of the 4 layers: in
+// x-lane order wave w takes depths 4w..4w+3 with x along the lanes; in
+// depth-lane order it takes x = w + 4m (m = 16 xg + 4 xs, xs = 0..3) with the
+// 16 depths along lanes and 4 x-groups xg.  A view uses the order whose lanes
+// spread more across columns.
+template <typename VS, typename AccT, bool CROP>
+__device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__ vol,
+                                              AccT *__restrict__ sums,
+                                              const float *__restrict__ pd,
+                                              const xpr_proj_args &a, const BrickShape &s,
+                                              int64_t b, unsigned long long *__restrict__ lds,
+                                              ViewBin *__restrict__ bins)
+{
+    const int lane = threadIdx.x & 63;
+    // wave-uniform: lets the compiler keep row and flush addressing scalar
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int32_t bx = (int32_t)(b % s.nbx);
+    b /= s.nbx;
+    const int32_t bk = (int32_t)(b % s.nbk);
+    const int32_t bl = (int32_t)(b / s.nbk);
+    const int32_t x0 = bx * 64, l0 = bl * kWaves, k0 = bk * kBK;
+    const int32_t dk = lane & 15, xg = lane >> 4;     // depth-lane order
+
+    // Q.22 voxel values in both lane orders (32 bits suffice unless some
+    // value is >= 1024), qx[d][l] and qd[xs][l].  Raw buffer loads from each
+    // layer's brick corner (x0, l0 + l, k0): out-of-slab voxels read 0 without
+    // a branch, so all 32 loads are in flight together.
+    uint32_t qx[kD][kWaves], qd[kD][kWaves];
+    bool big = false;
+    {
+        constexpr uint32_t esz = (uint32_t)sizeof(typename VS::T);
+        auto q22 = [&](float v, bool track) {
+            const float sc = v * kQ22One;               // FLOAT_TO_FIXED (constants.cl:22)
+            const bool pos = sc > 0.0f, over = sc >= 4294967296.0f;
+            if (sizeof(AccT) == 8 && track) big |= pos & over;
+            return pos ? (over ? 0xffffffffu : (uint32_t)sc) : 0u;
+        };
+        float vx_[kD][kWaves], vd_[kD][kWaves];
+#pragma unroll
+        for (int l = 0; l < kWaves; ++l) {
+            const int32_t ll = l0 + l;
+            const int64_t base = ((int64_t)ll * s.nk + k0) * s.nx + x0;
+            const int64_t left = (int64_t)s.nl * s.nk * s.nx - base;  // voxels from the corner on
+            const uint32_t bytes = ll < s.nl ? (uint32_t)min<int64_t>(left * esz, 0x7fffffff) : 0u;
+            const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+                (void *)(vol + (ll < s.nl ? base : 0)), 0, (int)bytes, kBufferWord3);
+#pragma unroll
+            for (int d = 0; d < kD; ++d) {
+                const int32_t kk = wave * kD + d;
+                const bool in = (x0 + lane < s.nx) & (k0 + kk < s.nk);
+                const uint32_t off = (uint32_t)(kk * s.nx + lane) * esz;
+                vx_[d][l] = load_or_zero<VS>(rsrc, in ? off : kOutOfRange);
+            }
+#pragma unroll
+            for (int xs = 0; xs < kD; ++xs) {
+                const int32_t xx = wave + 16 * xg + 4 * xs;
+                const bool in = (x0 + xx < s.nx) & (k0 + dk < s.nk);
+                const uint32_t off = (uint32_t)(dk * s.nx + xx) * esz;
+                vd_[xs][l] = load_or_zero<VS>(rsrc, in ? off : kOutOfRange);
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < kD; ++d)
+#pragma unroll
+            for (int l = 0; l < kWaves; ++l) {
+                qx[d][l] = q22(vx_[d][l], true);
+                qd[d][l] = q22(vd_[d][l], false);
+            }
+    }
+    if (__syncthreads_or(big)) {
+        forward_direct_u64<VS, AccT, CROP>(vol, sums, pd, a, s, x0, l0, k0);
+        return;
+    }
+    // A tile entry sums at most the brick's 4096 voxel-views of one view, so
+    // 32-bit entries hold every sum exactly when all Q.22 values are below
+    // 2^20, and always when the detector sums themselves are 32-bit (the same
+    // sums modulo 2^32): half the LDS atomic traffic.
+    bool wide = false;
+    if (sizeof(AccT) == 8) {
+#pragma unroll
+        for (int d = 0; d < kD; ++d)
+#pragma unroll
+            for (int l = 0; l < kWaves; ++l) wide |= qx[d][l] >= (1u << 20);
+    }
+    if (sizeof(AccT) == 8 && __syncthreads_or(wide))
+        forward_views<VS, AccT, CROP, unsigned long long>(sums, pd, a, s, x0, l0, k0, qx, qd, lds, bins);
+    else
+        forward_views<VS, AccT, CROP, uint32_t>(sums, pd, a, s, x0, l0, k0, qx, qd, lds, bins);
 }
 
 template <typename VS, typename AccT, bool CROP>

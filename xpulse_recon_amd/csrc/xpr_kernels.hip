@@ -105,11 +105,13 @@ int check_args(const xpr_proj_args *a, SlabShape &s)
     return XPR_OK;
 }
 
+// backward launches: wave rows of L layers, segments of 64*J voxels along x
 template <typename Launch>
-int launch_rows(SlabShape s, int J, Launch &&launch)
+int launch_rows(SlabShape s, int J, int L, Launch &&launch)
 {
     if (s.nx == 0 || s.nrows == 0) return XPR_OK;
     s.nseg = (s.nx + 64 * J - 1) / (64 * J);
+    s.nrows = backward_rows(s.nl, s.nk, L);
     // keep grid * block below 2^31 per launch
     const int64_t max_blocks = (int64_t(1) << 31) / kThreads;
     const int64_t rows_per_launch = (max_blocks / s.nseg) * kWaves;
@@ -185,7 +187,9 @@ int time_best(hipStream_t st, int reps, F &&launch, float *best)
     return rc;
 }
 
-constexpr int kBwdJs[] = {2, 4, 8, 16};
+// backward shapes by voxels per lane, J x positions times L layers:
+// 4 = 1 x 4, 8 = 2 x 4, 16 = 4 x 4
+constexpr int kBwdJs[] = {4, 8, 16};
 
 // The tuned shape: the default unless another is more than 3 % faster, so
 // timing noise on the tuning launch cannot lock in a shape that is no better.
@@ -205,22 +209,22 @@ int pick_shape(const int (&cands)[N], const float (&ms)[N], int fallback)
     return pick;
 }
 
-// default voxels per lane (J = 4 measured best at 768^3 / 36 views among 2, 4,
-// 8, 16); narrow slabs take J = 2 so one 64*J segment leaves few lanes idle
-int backward_default_j(int32_t nx) { return nx <= 128 ? 2 : kBwdJ; }
+// default voxels per lane (kBwdJ x kBwdL); narrow slabs take one x position
+// per lane so one 64*J segment leaves few lanes idle
+int backward_default_j(int32_t nx) { return nx <= 128 ? kBwdL : kBwdJ * kBwdL; }
 
-template <typename VS, typename ES, int J>
+template <typename VS, typename ES, int J, int L>
 int backward_launch(hipStream_t st, void *vol, const void *err, const float *pd,
                     const xpr_proj_args &a, SlabShape s, uint64_t *empty)
 {
     const bool crop = a.line_offset != 0;
-    return launch_rows(s, J, [&](unsigned blocks, const SlabShape &t) {
+    return launch_rows(s, J, L, [&](unsigned blocks, const SlabShape &t) {
         if (crop)
-            hipLaunchKernelGGL((k_backward<VS, ES, J, true>), dim3(blocks), dim3(kThreads), 0,
+            hipLaunchKernelGGL((k_backward<VS, ES, J, L, true>), dim3(blocks), dim3(kThreads), 0,
                                st, (typename VS::T *)vol, (const typename ES::T *)err, pd, a, t,
                                (unsigned long long *)empty);
         else
-            hipLaunchKernelGGL((k_backward<VS, ES, J, false>), dim3(blocks), dim3(kThreads), 0,
+            hipLaunchKernelGGL((k_backward<VS, ES, J, L, false>), dim3(blocks), dim3(kThreads), 0,
                                st, (typename VS::T *)vol, (const typename ES::T *)err, pd, a, t,
                                (unsigned long long *)empty);
     });
@@ -243,15 +247,14 @@ int backward_impl(hipStream_t st, void *vol, const void *err, const float *pd,
         }
         if (J < 0) J = backward_default_j(s.nx);
         switch (J) {
-        case 2: return backward_launch<VS, ES, 2>(st, vol, err, pd, a, s, empty);
-        case 4: return backward_launch<VS, ES, 4>(st, vol, err, pd, a, s, empty);
-        case 8: return backward_launch<VS, ES, 8>(st, vol, err, pd, a, s, empty);
-        case 16: return backward_launch<VS, ES, 16>(st, vol, err, pd, a, s, empty);
-        default: return backward_launch<VS, ES, kBwdJ>(st, vol, err, pd, a, s, empty);
+        case 4: return backward_launch<VS, ES, 1, 4>(st, vol, err, pd, a, s, empty);
+        case 8: return backward_launch<VS, ES, 2, 4>(st, vol, err, pd, a, s, empty);
+        case 16: return backward_launch<VS, ES, 4, 4>(st, vol, err, pd, a, s, empty);
+        default: return backward_launch<VS, ES, kBwdJ, kBwdL>(st, vol, err, pd, a, s, empty);
         }
     } else {
         (void)J;
-        return backward_launch<VS, ES, kBwdJ>(st, vol, err, pd, a, s, empty);
+        return backward_launch<VS, ES, kBwdJ, kBwdL>(st, vol, err, pd, a, s, empty);
     }
 }
 
@@ -375,7 +378,8 @@ uint32_t forward_units(const xpr_proj_args &a)
 uint32_t backward_units(const xpr_proj_args &a)
 {
     const int64_t nx = a.vmax[0] - a.vmin[0], nl = a.vmax[1] - a.vmin[1], nk = a.vmax[2] - a.vmin[2];
-    return (uint32_t)(((nl * nk + kWaves - 1) / kWaves) * ((nx + 64 * kBwdJ - 1) / (64 * kBwdJ)));
+    const int64_t rows = backward_rows((int32_t)nl, (int32_t)nk, kBwdL);
+    return (uint32_t)(((rows + kWaves - 1) / kWaves) * ((nx + 64 * kBwdJ - 1) / (64 * kBwdJ)));
 }
 
 struct Engine {
@@ -821,7 +825,7 @@ int xpr_tune_backward(xpr_stream s, void *volume_scratch, int vol_dtype, const v
     if (!volume_scratch || !err || !projdata || !choice) return fail(XPR_E_INVALID, "null buffer");
     if (sh.nx == 0 || sh.nrows == 0) return fail(XPR_E_INVALID, "empty launch");
     if (vol_dtype != err_dtype) {                // one shape compiled for mixed storage pairs
-        *choice = kBwdJ;
+        *choice = kBwdJ * kBwdL;
         return XPR_OK;
     }
     hipStream_t st = (hipStream_t)s;
