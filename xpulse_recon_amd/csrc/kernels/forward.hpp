@@ -60,6 +60,8 @@ enum : uint32_t {
     kBinDepth = 8u,    // depth-lane order
     kBinExact = 16u,   // the tile test alone decides the canonical bounds (see bin_view)
     kBinInside = 32u,  // exact, and every voxel-view of the brick lands in the tile
+    kBinColW = 64u,    // the w row terms of every (depth, layer) agree across the 4 layers
+    kBinColU = 128u,   // and so do the u row terms
 };
 
 // A view takes the depth-lane order when the brick's columns spread along
@@ -73,7 +75,8 @@ struct ViewBin {                   // one (view, module) record of the brick, in
     int32_t u0, v0, tw, th;        // tile origin and extent on the detector
     float ulo, uhi, vlo, vhi;      // in-bounds thresholds (view_thresholds)
     uint32_t flags;
-    uint32_t pad[3];
+    uint32_t magic;                // ceil(2^32 / tw) (0 when tw is 1): tile entry -> row
+    uint32_t pad[2];
     // ProjData rows 0, 1 and 3 of the matrix (record floats 0-7, 12-15): the
     // splat reads them from here with the rest of the record, not from memory
     float m[12];
@@ -143,6 +146,25 @@ __device__ __forceinline__ void bin_view(const float *__restrict__ pd, uint32_t 
         ok &= __shfl_xor(ok, off);
         fast &= __shfl_xor(fast, off);
     }
+    // Layer coherence of the brick's rows (see the coherence modes below):
+    // the 8 lanes of the view check 8 of the 64 (depth, layer) pairs each,
+    // with the positions and FMA chains the splat evaluates.
+    int wsame = 1, usame = 1;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int pr = corner * 8 + i, d = pr >> 2, l = pr & 3;
+        const float py = __builtin_fmaf((float)(lo[1] + l + a.vmin[1]), a.voxel[1], a.origin[1]);
+        const float py0 = __builtin_fmaf((float)(lo[1] + a.vmin[1]), a.voxel[1], a.origin[1]);
+        const float pz = __builtin_fmaf((float)(lo[2] + d + a.vmin[2]), a.voxel[2], a.origin[2]);
+        const float t0 = __builtin_fmaf(p[2], pz, p[3]), t3 = __builtin_fmaf(p[14], pz, p[15]);
+        wsame &= __builtin_fmaf(p[13], py, t3) == __builtin_fmaf(p[13], py0, t3);
+        usame &= __builtin_fmaf(p[1], py, t0) == __builtin_fmaf(p[1], py0, t0);
+    }
+#pragma unroll
+    for (int off = 1; off < 8; off <<= 1) {
+        wsame &= __shfl_xor(wsame, off);
+        usame &= __shfl_xor(usame, off);
+    }
     if (!live || corner != 0) return;
     ViewBin b{};
     uint32_t flags = fast ? kBinFast : 0u;
@@ -189,7 +211,10 @@ __device__ __forceinline__ void bin_view(const float *__restrict__ pd, uint32_t 
             }
         }
     }
+    if (wsame) flags |= kBinColW;
+    if (wsame & usame) flags |= kBinColU;
     b.flags = flags;
+    b.magic = b.tw > 1 ? 0xffffffffu / (uint32_t)b.tw + 1u : 0u;
 #pragma unroll
     for (int i = 0; i < 8; ++i) b.m[i] = p[i];
 #pragma unroll
@@ -491,24 +516,19 @@ __device__ __forceinline__ void forward_views(AccT *__restrict__ sums, const flo
             const int splat = (flags & kBinInside) ? kSplatInside
                               : (flags & kBinExact) ? kSplatExact : kSplatChecked;
             const bool fast = (flags & kBinFast) != 0;
+            const int mode = (flags & kBinColU) ? kColU : (flags & kBinColW) ? kColW : kColNone;
             if (flags & kBinDepth) {
                 // depth-lane order: this lane's depth, the 4 layers' row terms
                 const float t0 = __builtin_fmaf(bv.m[2], pz_d, bv.m[3]);
                 const float t1 = __builtin_fmaf(bv.m[6], pz_d, bv.m[7]);
                 const float t3 = __builtin_fmaf(bv.m[10], pz_d, bv.m[11]);
                 float K0[kWaves], K1[kWaves], K3[kWaves];
-                bool wsame = true, usame = true;
 #pragma unroll
                 for (int l = 0; l < kWaves; ++l) {
                     K0[l] = __builtin_fmaf(bv.m[1], py[l], t0);
                     K1[l] = __builtin_fmaf(bv.m[5], py[l], t1);
                     K3[l] = __builtin_fmaf(bv.m[9], py[l], t3);
-                    wsame &= K3[l] == K3[0];
-                    usame &= K0[l] == K0[0];
                 }
-                const bool wcol = __builtin_amdgcn_ballot_w64(!wsame) == 0;
-                const bool ucol = wcol && __builtin_amdgcn_ballot_w64(!usame) == 0;
-                const int mode = ucol ? kColU : wcol ? kColW : kColNone;
                 splat_dispatch(splat, fast, mode, [&](auto M, auto S, auto F) __attribute__((always_inline)) {
                     splat_depth_order<decltype(M)::value, decltype(S)::value, decltype(F)::value,
                                       CROP, AccT, TileT>(px_d, qd, K0, K1, K3, c, line_off, t);
@@ -519,11 +539,6 @@ __device__ __forceinline__ void forward_views(AccT *__restrict__ sums, const flo
                 const float k0t = __builtin_fmaf(bv.m[1], py_ti, __builtin_fmaf(bv.m[2], pz_t, bv.m[3]));
                 const float k1t = __builtin_fmaf(bv.m[5], py_ti, __builtin_fmaf(bv.m[6], pz_t, bv.m[7]));
                 const float k3t = __builtin_fmaf(bv.m[9], py_ti, __builtin_fmaf(bv.m[10], pz_t, bv.m[11]));
-                const float k0f = __builtin_fmaf(bv.m[1], py[0], __builtin_fmaf(bv.m[2], pz_t, bv.m[3]));
-                const float k3f = __builtin_fmaf(bv.m[9], py[0], __builtin_fmaf(bv.m[10], pz_t, bv.m[11]));
-                const bool wcol = __builtin_amdgcn_ballot_w64(k3t != k3f) == 0;
-                const bool ucol = wcol && __builtin_amdgcn_ballot_w64(k0t != k0f) == 0;
-                const int mode = ucol ? kColU : wcol ? kColW : kColNone;
                 splat_dispatch(splat, fast, mode, [&](auto M, auto S, auto F) __attribute__((always_inline)) {
                     splat_x_order<decltype(M)::value, decltype(S)::value, decltype(F)::value, CROP,
                                   AccT, TileT>(px_x, qx, k0t, k1t, k3t, c, line_off, t);
@@ -531,17 +546,20 @@ __device__ __forceinline__ void forward_views(AccT *__restrict__ sums, const flo
             }
             if (t.use_tile) {
                 lds_barrier();                // splat of this view complete
-                // flush and clear; the next view splats into the other tile
-                // (row pointers are wave-uniform; a lane adds a 32-bit column offset)
-                for (int rr = wave; rr < t.th; rr += kWaves) {
-                    AccT *dst = t.vsum + (uint32_t)(t.v0 + rr) * t.det_width + (uint32_t)t.u0;
-                    TileT *src = t.tl + rr * t.tw;
-                    for (uint32_t cc = (uint32_t)lane; cc < (uint32_t)t.tw; cc += 64u) {
-                        const TileT v = src[cc];
-                        if (v != 0) {
-                            global_add(dst + cc, (AccT)v);
-                            src[cc] = 0;
-                        }
+                // flush and clear, the tile's entries spread over the
+                // workgroup's threads; the next view splats into the other
+                // tile.  Entry e is (row, col) = (e / tw, e % tw), the row by
+                // a multiply-high with ceil(2^32 / tw) (exact for e * tw < 2^32).
+                const uint32_t n = (uint32_t)(t.tw * t.th);
+                const uint32_t magic = __builtin_amdgcn_readfirstlane(bv.magic);
+                AccT *base = t.vsum + (uint32_t)t.v0 * t.det_width + (uint32_t)t.u0;
+                for (uint32_t e = threadIdx.x; e < n; e += (uint32_t)kThreads) {
+                    const TileT v = t.tl[e];
+                    if (v != 0) {
+                        const uint32_t row = magic ? __umulhi(e, magic) : e;
+                        const uint32_t col = e - row * (uint32_t)t.tw;
+                        global_add(base + (row * t.det_width + col), (AccT)v);
+                        t.tl[e] = 0;
                     }
                 }
                 buf ^= 1;
