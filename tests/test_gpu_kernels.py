@@ -193,18 +193,22 @@ def test_convert_and_fill(cuda):
     assert (h.cpu() == 1.0).all()
 
 
+@pytest.mark.parametrize("hi", [2.0, 0.2])
+@pytest.mark.parametrize("tilt", ["none", "roll"])
 @pytest.mark.parametrize("vdt", ["f16", "bf16"])
-def test_mixed_precision_storage(cuda, vdt):
+def test_mixed_precision_storage(cuda, vdt, tilt, hi):
     """16-bit volume and error storage, FP32 arithmetic: forward is bit-exact
     on the widened values; backward equals the oracle's FP32 update rounded to
-    the storage type (one storage ulp where device expf differs)."""
+    the storage type (one storage ulp where device expf differs).  Upright and
+    rolled matrices; volumes up to 2.0 (64-bit tile entries) and up to 0.2
+    (32-bit entries)."""
     import torch
     import xpulse_recon_amd as X
-    spec, pd, d = _case(1)
+    spec, pd, d = _case(1, tilt)
     nx, nl, nk = (d["vmax"][i] - d["vmin"][i] for i in range(3))
     tdt = {"f16": torch.float16, "bf16": torch.bfloat16}[vdt]
     rng = np.random.default_rng(14)
-    vol32 = torch.from_numpy(rng.uniform(0.5, 2.0, (nl, nk, nx)).astype(np.float32))
+    vol32 = torch.from_numpy(rng.uniform(hi / 4.0, hi, (nl, nk, nx)).astype(np.float32))
     vol16 = vol32.to(tdt)
     err32 = torch.from_numpy(rng.normal(0, 0.3, (d["n_views"], d["line_rows"], d["det_width"]))
                              .astype(np.float32))
