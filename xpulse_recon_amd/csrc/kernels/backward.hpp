@@ -226,7 +226,7 @@ __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
                 (void *)(err + (uint64_t)view_of(a, view) * view_elems), (short)esz,
                 (int)view_elems, kBufferWord3);
             const uint32_t dw = a.det_width;
-            float smp[J][L];
+            typename SampleOf<ES>::type smp[J][L];
             if ((inside_mask >> i) & 1u) {
                 ++n_inside;
                 if ((ucol_mask >> i) & 1u) {
@@ -242,7 +242,7 @@ __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
                         for (int l = 0; l < L; ++l) {
                             const float Y = __builtin_fmaf(c1x, px[j], k1[l]);
                             const float v = CROP ? Y * r - line_off : Y * r;
-                            smp[j][l] = sample_or_zero<ES>(rsrc, __umul24((uint32_t)pixel_of(v), dw) + iu);
+                            smp[j][l] = sample_raw<ES>(rsrc, __umul24((uint32_t)pixel_of(v), dw) + iu);
                         }
                     }
                 } else {
@@ -256,7 +256,7 @@ __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
                             const float X = __builtin_fmaf(c0x, px[j], k0[l]);
                             const float Y = __builtin_fmaf(c1x, px[j], k1[l]);
                             const float v = CROP ? Y * r - line_off : Y * r;
-                            smp[j][l] = sample_or_zero<ES>(
+                            smp[j][l] = sample_raw<ES>(
                                 rsrc, __umul24((uint32_t)pixel_of(v), dw) + (uint32_t)pixel_of(X * r));
                         }
                     }
@@ -289,7 +289,7 @@ __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
                             const float v = CROP ? Y * r - line_off : Y * r;
                             const bool inb = (u >= c.ulo) & (u < c.uhi) & (v >= c.vlo) & (v < c.vhi);
                             const uint32_t idx = __umul24((uint32_t)pixel_of(v), dw) + (uint32_t)pixel_of(u);
-                            smp[j][l] = sample_or_zero<ES>(rsrc, inb ? idx : kNoSample);
+                            smp[j][l] = sample_raw<ES>(rsrc, inb ? idx : kNoSample);
                             cnt[j][l] += inb ? 1u : 0u;
                         }
                     }
@@ -307,7 +307,7 @@ __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
                             const bool inb = fast ? project<CROP, true>(c, px[j], line_off, iu, iv)
                                                   : project<CROP, false>(c, px[j], line_off, iu, iv);
                             const uint32_t idx = __umul24(iv, dw) + iu;
-                            smp[j][l] = sample_or_zero<ES>(rsrc, inb ? idx : kNoSample);
+                            smp[j][l] = sample_raw<ES>(rsrc, inb ? idx : kNoSample);
                             cnt[j][l] += inb ? 1u : 0u;
                         }
                     }
@@ -316,7 +316,7 @@ __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
 #pragma unroll
             for (int j = 0; j < J; ++j)
 #pragma unroll
-                for (int l = 0; l < L; ++l) sum[j][l] += smp[j][l];
+                for (int l = 0; l < L; ++l) sum[j][l] = add_sample<ES>(sum[j][l], smp[j][l]);
             if (++mm == a.modules) {
                 mm = 0;
                 ++view;

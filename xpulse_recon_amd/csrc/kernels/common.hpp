@@ -220,22 +220,53 @@ __device__ uint16_t struct_buffer_load_u16(__amdgpu_buffer_rsrc_t rsrc, uint32_t
                                            uint32_t soffset, uint32_t aux)
     __asm("llvm.amdgcn.struct.ptr.buffer.load.i16");
 
+// The backward keeps a 16-bit sample in its storage form until it is added:
+// a half sample joins the sum in one v_fma_mix_f32 (h * 1 + sum rounds once,
+// exactly like the float add of the widened sample), instead of a conversion
+// and an add.
+template <typename T> struct SampleOf { using type = float; };
+template <> struct SampleOf<F16> { using type = uint16_t; };
+template <> struct SampleOf<BF16> { using type = uint16_t; };
+
 template <typename T>
-__device__ __forceinline__ float sample_or_zero(__amdgpu_buffer_rsrc_t r, uint32_t index);
+__device__ __forceinline__ typename SampleOf<T>::type sample_raw(__amdgpu_buffer_rsrc_t r,
+                                                                 uint32_t index);
 template <>
-__device__ __forceinline__ float sample_or_zero<F32>(__amdgpu_buffer_rsrc_t r, uint32_t index)
+__device__ __forceinline__ float sample_raw<F32>(__amdgpu_buffer_rsrc_t r, uint32_t index)
 {
     return struct_buffer_load_f32(r, index, 0, 0, 0);
 }
 template <>
-__device__ __forceinline__ float sample_or_zero<F16>(__amdgpu_buffer_rsrc_t r, uint32_t index)
+__device__ __forceinline__ uint16_t sample_raw<F16>(__amdgpu_buffer_rsrc_t r, uint32_t index)
 {
-    return (float)__builtin_bit_cast(_Float16, struct_buffer_load_u16(r, index, 0, 0, 0));
+    return struct_buffer_load_u16(r, index, 0, 0, 0);
 }
 template <>
-__device__ __forceinline__ float sample_or_zero<BF16>(__amdgpu_buffer_rsrc_t r, uint32_t index)
+__device__ __forceinline__ uint16_t sample_raw<BF16>(__amdgpu_buffer_rsrc_t r, uint32_t index)
 {
-    return __builtin_bit_cast(float, (uint32_t)struct_buffer_load_u16(r, index, 0, 0, 0) << 16);
+    return struct_buffer_load_u16(r, index, 0, 0, 0);
+}
+
+template <typename T>
+__device__ __forceinline__ float add_sample(float sum, typename SampleOf<T>::type s);
+template <>
+__device__ __forceinline__ float add_sample<F32>(float sum, float s)
+{
+    return sum + s;
+}
+template <>
+__device__ __forceinline__ float add_sample<F16>(float sum, uint16_t s)
+{
+    // 1.0 through a scalar register the compiler cannot see into: a literal
+    // 1.0 lets it rewrite the FMA as an add of the converted sample
+    uint32_t one;
+    asm("s_mov_b32 %0, 0x3f800000" : "=s"(one));
+    return __builtin_fmaf((float)__builtin_bit_cast(_Float16, s), __builtin_bit_cast(float, one), sum);
+}
+template <>
+__device__ __forceinline__ float add_sample<BF16>(float sum, uint16_t s)
+{
+    return sum + __builtin_bit_cast(float, (uint32_t)s << 16);
 }
 
 // ---------------------------------------------------------------------------
