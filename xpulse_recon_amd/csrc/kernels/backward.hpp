@@ -49,8 +49,8 @@ struct BwdRecord {
 // between their corner values.  With
 //   du = 8 eps ((Tx + U Tw) / Wmin + U),  eps = 2^-24,
 // bounding the rounding of the three FMA dot products, the reciprocal and the
-// product for any voxel or corner (Tx, Tw: largest sums of |coefficient x
-// coordinate| over the corners; U = max |u| + 1; Wmin = min W), every computed
+// product for any voxel or corner (Tx, Tw: sums of |coefficient| x the largest
+// |coordinate| of the box; U = max |u| + 1; Wmin = min W), every computed
 // u lies within 2 du of the computed corners' range (likewise v with dv).  A
 // segment is inside when that range, widened by 2 du plus a term that covers
 // the float evaluation of the bound, passes the canonical half-open test:
@@ -91,9 +91,16 @@ __device__ __forceinline__ RowSetup<L> row_setup(const float *__restrict__ p,
     r.uhi = hi_threshold(fminf(p[17], (float)a.det_width));
     r.vlo = lo_threshold(fmaxf(p[18], 0.0f));
     r.vhi = hi_threshold(fminf(p[19], (float)a.line_rows));
+    // magnitudes for the rounding bound: sums of |coefficient x coordinate|,
+    // bounded over the box by the largest |coordinate| of each axis
+    const float mx = fmaxf(fabsf(px_first), fabsf(px_last));
+    const float my = fmaxf(fabsf(py[0]), fabsf(py[L - 1]));
+    const float tx = fabsf(c0x) * mx + fabsf(p[1]) * my + fabsf(p[2] * pz) + fabsf(p[3]);
+    const float ty = fabsf(c1x) * mx + fabsf(p[5]) * my + fabsf(p[6] * pz) + fabsf(p[7]);
+    const float tw = fabsf(c3x) * mx + fabsf(p[13]) * my + fabsf(p[14] * pz) + fabsf(p[15]);
     // the four corners of the segment's box, computed like a voxel's
     float umin = 3.4e38f, umax = -3.4e38f, vmin = 3.4e38f, vmax = -3.4e38f;
-    float tx = 0.0f, ty = 0.0f, tw = 0.0f, wmin = 3.4e38f, ymax = 0.0f;
+    float wmin = 3.4e38f, ymax = 0.0f;
     bool ok = true;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -113,12 +120,8 @@ __device__ __forceinline__ RowSetup<L> row_setup(const float *__restrict__ p,
         vmin = fminf(vmin, v);
         vmax = fmaxf(vmax, v);
         ymax = fmaxf(ymax, fabsf(y));
-        const float tmw = fabsf(c3x * cx) + fabsf(p[13] * cy) + fabsf(p[14] * pz) + fabsf(p[15]);
-        tx = fmaxf(tx, fabsf(c0x * cx) + fabsf(p[1] * cy) + fabsf(p[2] * pz) + fabsf(p[3]));
-        ty = fmaxf(ty, fabsf(c1x * cx) + fabsf(p[5] * cy) + fabsf(p[6] * pz) + fabsf(p[7]));
-        tw = fmaxf(tw, tmw);
         wmin = fminf(wmin, W);
-        ok &= fast_rcp_box(W, tmw);
+        ok &= fast_rcp_box(W, tw);
     }
     // every voxel W of the box lies between the corner values (fast_rcp_box)
     r.fast = ok;
@@ -126,8 +129,9 @@ __device__ __forceinline__ RowSetup<L> row_setup(const float *__restrict__ p,
     constexpr float eps8 = 8.0f / 16777216.0f;
     const float U = fmaxf(fabsf(umin), fabsf(umax)) + 1.0f;
     const float V = ymax + fabsf(line_off) + 1.0f;
-    const float du = eps8 * ((tx + U * tw) / wmin + U);
-    const float dv = eps8 * ((ty + V * tw) / wmin + V);
+    const float iw = 1.0f / wmin;
+    const float du = eps8 * ((tx + U * tw) * iw + U);
+    const float dv = eps8 * ((ty + V * tw) * iw + V);
     const float mu = 2.0f * du + 1e-6f * U + (1.0f / 1024.0f);
     const float mv = 2.0f * dv + 1e-6f * V + (1.0f / 1024.0f);
     r.inside = inside_ok & ok & r.wcol & (umin - mu >= r.ulo) & (umax + mu < r.uhi) &
