@@ -276,7 +276,8 @@ __device__ __forceinline__ uint32_t lds_address(const T *p)
     return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) T *)p;
 }
 
-// How a view's per-voxel projection is evaluated for one wave (uniform).
+// How a view's per-voxel projection is evaluated for one brick (uniform over
+// the workgroup; bin_view decides it from the brick's row terms).
 // The canonical X, Y, W of a voxel are fmaf(c, px, k) with the row terms k of
 // its (layer, depth).  When the w row terms of a depth are the same float in
 // the brick's 4 layers (a zero layer coefficient in ProjData float 13, as in
