@@ -45,7 +45,8 @@ def _tilt(pd, tilt):
     beam axis) and the w row a residue of 3e-7, as a float32 roll leaves it:
     the w row terms stay equal across layers, the u row terms do not.
     "pitch": a w row coefficient large enough that every layer has its own W:
-    the projectors take the canonical per-voxel projection there."""
+    the projectors take the canonical per-voxel projection there.
+    ("residue": see _residue.)"""
     pd = pd.copy()
     if tilt == "roll":
         pd[1::2, 1] = 0.02 * pd[1::2, 5]
@@ -56,23 +57,41 @@ def _tilt(pd, tilt):
     return pd
 
 
+def _residue(pd, d):
+    """Odd records with w row coefficients around half an ulp of the w row
+    term over the volume's layer positions: some bricks and row segments keep
+    equal w row terms in all their layers, some in none, some in part of their
+    rows (these must take the per-voxel projection)."""
+    pd = pd.copy()
+    rng = np.random.default_rng(77)
+    n = pd[1::2].shape[0]
+    y0 = d["origin"][1] + d["vmin"][1] * d["voxel"][1]
+    y1 = d["origin"][1] + (d["vmax"][1] - 1) * d["voxel"][1]
+    ymax = max(abs(y0), abs(y1), 1e-6)
+    ulp = np.spacing(np.abs(pd[1::2, 15]).astype(np.float32)).astype(np.float64)
+    pd[1::2, 13] = (rng.uniform(0.05, 3.0, n) * ulp / ymax * rng.choice([-1.0, 1.0], n)).astype(np.float32)
+    return pd
+
+
 def _case(i, tilt="none"):
     kw, views, slab, crop, jit = CASES[i]
     kw = dict(kw)
     w, h, n = kw.pop("w"), kw.pop("h"), kw.pop("n")
     spec = small_spec(w, h, n, **kw)
     pd, grid = projdata_for(spec, views)
-    if tilt != "none":
+    if tilt in ("roll", "pitch"):
         pd = _tilt(pd, tilt)
     vmin, vmax = slab if slab else (None, None)
     lo, rows = crop if crop else (0, None)
     d = geom_args(spec, grid, len(views), vmin=vmin, vmax=vmax, jitter=jit, line_offset=lo,
                   line_rows=rows)
+    if tilt == "residue":
+        pd = _residue(pd, d)
     return spec, pd, d
 
 
 @pytest.mark.parametrize("scale", [3.0, 0.2])
-@pytest.mark.parametrize("tilt", ["none", "roll", "pitch"])
+@pytest.mark.parametrize("tilt", ["none", "roll", "pitch", "residue"])
 @pytest.mark.parametrize("case", range(len(CASES)))
 @pytest.mark.parametrize("acc", [32, 64])
 def test_forward_bit_exact(cuda, case, acc, tilt, scale):
@@ -97,7 +116,7 @@ def test_forward_bit_exact(cuda, case, acc, tilt, scale):
     np.testing.assert_array_equal(got, ref)
 
 
-@pytest.mark.parametrize("tilt", ["none", "roll", "pitch"])
+@pytest.mark.parametrize("tilt", ["none", "roll", "pitch", "residue"])
 @pytest.mark.parametrize("case", range(len(CASES)))
 def test_backward_parity(cuda, case, tilt):
     import torch
@@ -345,7 +364,7 @@ def test_forward_bit_exact_across_tile_widths(cuda, scale):
     np.testing.assert_array_equal(sums.cpu().numpy().view(np.uint64), ref)
 
 
-@pytest.mark.parametrize("tilt", ["none", "roll", "pitch"])
+@pytest.mark.parametrize("tilt", ["none", "roll", "pitch", "residue"])
 @pytest.mark.parametrize("case", [1, 2, 4])
 def test_backward_lane_shapes_bit_identical(cuda, monkeypatch, case, tilt):
     """Every compiled voxels-per-lane shape of the backward (4, 8, 16: 1, 2 or
