@@ -139,6 +139,25 @@ int xpr_error(xpr_stream s, const void *images, int img_dtype, const void *sums,
 int xpr_backward(xpr_stream s, void *volume, int vol_dtype, const void *err, int err_dtype,
                  const float *projdata, const xpr_proj_args *a, uint64_t *empty_count);
 
+/* One sub-iteration of the slab (the body of ReconstructionN::exec's loop,
+ * reconstruction_n.hpp:66-76, with the values setOrigin / setWeight would
+ * set, src/reconstruction.hpp:255-272, in *a and weight):
+ *   sums <- 0, then xpr_forward of the volume into them; on the first pass
+ *           of a reconstruction (first_pass != 0) no forward and every sum
+ *           is 1.0, i.e. 1 << 22 (reconstruction_n.hpp:60,70-72);
+ *   err  <- xpr_error(images, sums, weight) over n_views*line_rows*det_width
+ *           pixels;
+ *   xpr_backward of err into the volume (update in place).
+ * images: [n_views][line_rows][det_width] of img_dtype.  sums (acc_bits
+ * Q.22) and err (err_dtype) are the caller's workspace, n_views*line_rows*
+ * det_width elements each; err may alias sums when acc_bits == 32 and
+ * err_dtype == XPR_F32.  Their contents on entry do not matter; on return
+ * err holds the error images.  a->view_index must be NULL. */
+int xpr_subiteration(xpr_stream s, void *volume, int vol_dtype, const void *images,
+                     int img_dtype, const float *projdata, const xpr_proj_args *a, float weight,
+                     int first_pass, void *sums, int acc_bits, void *err, int err_dtype,
+                     uint64_t *empty_count);
+
 /* Q.22 sums to float (FIXED_TO_FLOAT, constants.cl:21) for n pixels; used to
  * write reprojected views (Reconstruction::forward_exec,
  * src/reconstruction.hpp:139-153). */

@@ -13,7 +13,7 @@ import ctypes as C
 from . import _lib
 from ._lib import ProjArgs, XprError, proj_args  # noqa: F401
 
-__all__ = ["forward", "error", "backward", "fill", "fill_u32", "fill_u64", "convert",
+__all__ = ["forward", "error", "backward", "subiteration", "fill", "fill_u32", "fill_u64", "convert",
            "preprocess", "minmax_normal", "proj_args", "ProjArgs", "XprError",
            "dtype_code", "acc_bits", "tune_forward", "tune_backward", "tune_reset",
            "check_fast_paths"]
@@ -96,6 +96,22 @@ def backward(volume, err, projdata, args: ProjArgs, empty=None, stream=None) -> 
     _check_sizes(volume, err, projdata, args)
     _lib.call("xpr_backward", _stream(stream), _ptr(volume), dtype_code(volume), _ptr(err),
               dtype_code(err), _ptr(projdata), C.byref(args),
+              None if empty is None else _ptr(empty))
+
+
+def subiteration(volume, images, projdata, args: ProjArgs, weight: float, first_pass: bool,
+                 sums, err, empty=None, stream=None) -> None:
+    """One sub-iteration of the slab (reconstruction_n.hpp:66-76): sums <- 0
+    and Forward (none on the first pass: sums = 1.0), Error into err,
+    Backward + update.  sums and err are the workspace (their contents on
+    entry do not matter)."""
+    _need_cuda(volume, images, projdata, sums, err)
+    _check_sizes(volume, sums, projdata, args)
+    _check_sizes(volume, err, projdata, args)
+    _check_sizes(volume, images, projdata, args)
+    _lib.call("xpr_subiteration", _stream(stream), _ptr(volume), dtype_code(volume),
+              _ptr(images), dtype_code(images), _ptr(projdata), C.byref(args), float(weight),
+              int(bool(first_pass)), _ptr(sums), acc_bits(sums), _ptr(err), dtype_code(err),
               None if empty is None else _ptr(empty))
 
 

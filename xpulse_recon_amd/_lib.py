@@ -131,6 +131,8 @@ _SIGS = {
     "xpr_forward": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, _P, C.POINTER(ProjArgs)]),
     "xpr_error": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, _P, C.c_int, C.c_uint64, C.c_float]),
     "xpr_backward": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, _P, C.POINTER(ProjArgs), _P]),
+    "xpr_subiteration": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, _P, C.POINTER(ProjArgs),
+                                   C.c_float, C.c_int, _P, C.c_int, _P, C.c_int, _P]),
     "xpr_tune_forward": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, _P, C.POINTER(ProjArgs),
                                    C.POINTER(C.c_int)]),
     "xpr_tune_backward": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, _P, C.POINTER(ProjArgs),

@@ -777,6 +777,38 @@ int xpr_backward(xpr_stream s, void *volume, int vol_dtype, const void *err, int
 #undef XPR_BW
 }
 
+int xpr_subiteration(xpr_stream s, void *volume, int vol_dtype, const void *images,
+                     int img_dtype, const float *projdata, const xpr_proj_args *a, float weight,
+                     int first_pass, void *sums, int acc_bits, void *err, int err_dtype,
+                     uint64_t *empty_count)
+{
+    SlabShape sh;
+    if (int rc = check_args(a, sh)) return rc;
+    if (int rc = check_dtype(vol_dtype)) return rc;
+    if (int rc = check_dtype(img_dtype)) return rc;
+    if (int rc = check_dtype(err_dtype)) return rc;
+    if (acc_bits != XPR_ACC_U32 && acc_bits != XPR_ACC_U64)
+        return fail(XPR_E_INVALID, "acc_bits must be 32 or 64");
+    if (a->view_index)
+        return fail(XPR_E_INVALID, "xpr_subiteration: indexed views need xpr_forward / xpr_backward");
+    if (!volume || !images || !projdata || !sums || !err) return fail(XPR_E_INVALID, "null buffer");
+    if (err == sums && !(acc_bits == XPR_ACC_U32 && err_dtype == XPR_F32))
+        return fail(XPR_E_INVALID, "in-place error needs 32-bit sums and f32 error");
+    const uint64_t n = (uint64_t)a->n_views * (uint64_t)a->line_rows * (uint64_t)a->det_width;
+    // sums start at 0 before a forward, at 1.0 (Q.22) on the first pass
+    // (reconstruction_n.hpp:60,75): set here, so the workspace may hold anything
+    const uint64_t init = first_pass ? (1ull << 22) : 0ull;
+    int rc = acc_bits == XPR_ACC_U32 ? xpr_fill_u32(s, (uint32_t *)sums, (uint32_t)init, n)
+                                     : xpr_fill_u64(s, (uint64_t *)sums, init, n);
+    if (rc == XPR_OK && !first_pass)
+        rc = xpr_forward(s, volume, vol_dtype, sums, acc_bits, projdata, a);
+    if (rc == XPR_OK)
+        rc = xpr_error(s, images, img_dtype, sums, acc_bits, err, err_dtype, n, weight);
+    if (rc == XPR_OK)
+        rc = xpr_backward(s, volume, vol_dtype, err, err_dtype, projdata, a, empty_count);
+    return rc;
+}
+
 // ---- tile-shape tuning --------------------------------------------------------
 int xpr_tune_forward(xpr_stream s, const void *volume, int vol_dtype, void *sums_scratch,
                      int acc_bits, const float *projdata, const xpr_proj_args *a, int *choice)
