@@ -158,6 +158,23 @@ int xpr_subiteration(xpr_stream s, void *volume, int vol_dtype, const void *imag
                      int first_pass, void *sums, int acc_bits, void *err, int err_dtype,
                      uint64_t *empty_count);
 
+/* No counterpart in the reference.  xpr_subiteration for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the sums and error workspace live in a
+ * static device array of XPR_STATIC_SCRATCH_BYTES per GPU, held from the
+ * library's load until the process ends; the call allocates nothing.
+ * n_views*line_rows*det_width*(acc_bits/8 + error sample bytes) + 256 must
+ * not exceed XPR_STATIC_SCRATCH_BYTES (XPR_E_INVALID otherwise: e.g. 36 views
+ * of 384x384 with 64-bit sums and f32 error fit, 36 views of 768x768 do not).
+ * The sums are cleared inside the call.  All calls on one GPU share its
+ * array: calls that overlap on one GPU (concurrent streams or threads) race
+ * and must be serialised by the caller. */
+#define XPR_STATIC_SCRATCH_BYTES (64ull << 20)
+int xpr_subiteration_static(xpr_stream s, void *volume, int vol_dtype, const void *images,
+                            int img_dtype, const float *projdata, const xpr_proj_args *a,
+                            float weight, int first_pass, int acc_bits, int err_dtype,
+                            uint64_t *empty_count);
+
 /* Q.22 sums to float (FIXED_TO_FLOAT, constants.cl:21) for n pixels; used to
  * write reprojected views (Reconstruction::forward_exec,
  * src/reconstruction.hpp:139-153). */

@@ -115,6 +115,20 @@ def subiteration(volume, images, projdata, args: ProjArgs, weight: float, first_
               None if empty is None else _ptr(empty))
 
 
+def subiteration_static(volume, images, projdata, args: ProjArgs, weight: float,
+                        first_pass: bool, acc: int = 64, err_dtype: str = "f32", empty=None,
+                        stream=None) -> None:
+    """subiteration() with the sums and error workspace in the library's
+    static device scratch (no device allocation; calls that overlap on one
+    GPU race)."""
+    _need_cuda(volume, images, projdata)
+    _check_sizes(volume, images, projdata, args)
+    _lib.call("xpr_subiteration_static", _stream(stream), _ptr(volume), dtype_code(volume),
+              _ptr(images), dtype_code(images), _ptr(projdata), C.byref(args), float(weight),
+              int(bool(first_pass)), int(acc), _lib.DTYPES[err_dtype],
+              None if empty is None else _ptr(empty))
+
+
 def fill(t, value: float, stream=None) -> None:
     _need_cuda(t)
     _lib.call("xpr_fill", _stream(stream), _ptr(t), dtype_code(t), float(value), t.numel())

@@ -133,6 +133,9 @@ _SIGS = {
     "xpr_backward": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, _P, C.POINTER(ProjArgs), _P]),
     "xpr_subiteration": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, _P, C.POINTER(ProjArgs),
                                    C.c_float, C.c_int, _P, C.c_int, _P, C.c_int, _P]),
+    "xpr_subiteration_static": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, _P,
+                                          C.POINTER(ProjArgs), C.c_float, C.c_int, C.c_int,
+                                          C.c_int, _P]),
     "xpr_tune_forward": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, _P, C.POINTER(ProjArgs),
                                    C.POINTER(C.c_int)]),
     "xpr_tune_backward": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, _P, C.POINTER(ProjArgs),

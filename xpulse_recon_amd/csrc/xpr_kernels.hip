@@ -364,6 +364,40 @@ int convert_from(hipStream_t st, void *dst, const void *src, int src_dtype, uint
 
 int xprint_fail(int code, const char *msg) { return fail(code, msg); }
 
+// ---------------------------------------------------------------------------
+// Scratch of xpr_subiteration_static: one array per GPU, held from module
+// load until the process ends.  Host code never passes `scratch` itself (on
+// the host it is a stand-in symbol, not a device address); it uses the
+// device address hipGetSymbolAddress returns, looked up once per device.
+// ---------------------------------------------------------------------------
+namespace {
+
+static __device__ __align__(256) unsigned char scratch[XPR_STATIC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+int static_scratch(void **out)
+{
+    int dev = 0;
+    XPR_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(XPR_E_INVALID, "device index out of range");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr)
+            return fail(XPR_E_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                       hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return XPR_OK;
+}
+
+} // namespace
+
 // ===========================================================================
 // Batched step engine (library-internal, used by the job server)
 // ===========================================================================
@@ -807,6 +841,37 @@ int xpr_subiteration(xpr_stream s, void *volume, int vol_dtype, const void *imag
     if (rc == XPR_OK)
         rc = xpr_backward(s, volume, vol_dtype, err, err_dtype, projdata, a, empty_count);
     return rc;
+}
+
+int xpr_subiteration_static(xpr_stream s, void *volume, int vol_dtype, const void *images,
+                            int img_dtype, const float *projdata, const xpr_proj_args *a,
+                            float weight, int first_pass, int acc_bits, int err_dtype,
+                            uint64_t *empty_count)
+{
+    if (a == nullptr) return fail(XPR_E_INVALID, "null xpr_proj_args");
+    if (int rc = check_dtype(err_dtype)) return rc;
+    if (acc_bits != XPR_ACC_U32 && acc_bits != XPR_ACC_U64)
+        return fail(XPR_E_INVALID, "acc_bits must be 32 or 64");
+    if (a->line_rows < 0) return fail(XPR_E_INVALID, "line_rows < 0");
+    const uint64_t per_view = (uint64_t)a->line_rows * (uint64_t)a->det_width;
+    const uint64_t n = per_view * (uint64_t)a->n_views;
+    const uint64_t pixel_bytes = (uint64_t)(acc_bits / 8) + (err_dtype == XPR_F32 ? 4u : 2u);
+    // per_view bounded before the products, so they cannot wrap
+    const bool fits = a->n_views == 0 ||
+                      (per_view <= XPR_STATIC_SCRATCH_BYTES &&
+                       n * pixel_bytes + 256 <= XPR_STATIC_SCRATCH_BYTES);
+    if (!fits)
+        return fail(XPR_E_INVALID,
+                    "xpr_subiteration_static: " + std::to_string(a->n_views) + " views of " +
+                        std::to_string(per_view) + " pixels need more than the " +
+                        std::to_string((unsigned long long)XPR_STATIC_SCRATCH_BYTES) +
+                        " bytes of static scratch (use xpr_subiteration with a workspace)");
+    const uint64_t sums_bytes = (n * (uint64_t)(acc_bits / 8) + 255) & ~255ull;
+    void *base = nullptr;
+    if (int rc = static_scratch(&base)) return rc;
+    unsigned char *p = (unsigned char *)base;
+    return xpr_subiteration(s, volume, vol_dtype, images, img_dtype, projdata, a, weight,
+                            first_pass, p, acc_bits, p + sums_bytes, err_dtype, empty_count);
 }
 
 // ---- tile-shape tuning --------------------------------------------------------
