@@ -300,6 +300,17 @@ int xpr_reconstruct(const char *params, char *record, size_t record_cap);
 int xpr_reconstruct_mem(const char *params, const float *projections, int64_t width,
                         int64_t height, int64_t count, float *volume_out, char *record,
                         size_t record_cap);
+/* The method_2 chunk plan (GeometryChunk, dataset/geometry_chunk.hpp:92-197,
+ * made by ReconstructionChunk's constructor, reconstruction_chunk.hpp:13-25)
+ * for `count` projections of width x height under the parameters' geometry,
+ * storage and xpr.device_budget_gb (required).  Host function, no device
+ * work and no file access.  chunks_out receives 6 int64 per chunk: vOffset,
+ * vSize, iOffset, iSize, vPadTop, vPadBot (the order of the run record's
+ * "chunks"); *n_chunks the number of chunks (also when cap is too small,
+ * which returns XPR_E_INVALID).  XPR_E_NOMEM when a single layer does not fit
+ * the budget. */
+int xpr_plan_chunks(const char *params, int64_t width, int64_t height, int64_t count,
+                    int64_t *chunks_out, int64_t cap, int64_t *n_chunks);
 /* The default parameters as JSON (XPRecon -template).  0 on success, -1 if
  * cap is too small. */
 int xpr_template_json(char *out, size_t cap);

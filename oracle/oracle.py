@@ -360,3 +360,183 @@ def reconstruct_naive_f64(images: np.ndarray, projdata64: np.ndarray, grid: dict
             sums[...] = 0.0
         wgt = wgt * weight_factor
     return vol
+
+
+# --------------------------------------------------------------------------
+# method_2: chunk planner and chunked schedule (geometry_chunk.hpp:92-197,
+# reconstruction_chunk.hpp:36-131)
+# --------------------------------------------------------------------------
+def layer_row_bounds(projdata: np.ndarray, grid: dict):
+    """computeLayerYBounds (geometry_chunk.hpp:120-145): per layer, the lowest
+    and highest detector row coordinate of the four vertical edges of the
+    volume (x = +-orig.x, z = +-orig.z) at the layer's top and bottom, over
+    every ProjData record.  float32, evaluated as the sums
+    ((m4*x + m5*y) + m6*z) + m7 over ((m12*x + m13*y) + m14*z) + m15."""
+    f = np.float32
+    H, vx = int(grid["vheight"]), f(grid["vx"])
+    o = [f(grid["orig"][i]) for i in range(3)]
+    cx = np.array([o[0], -o[0], o[0], -o[0]], f)
+    cz = np.array([-o[2], -o[2], o[2], o[2]], f)
+    m = projdata.reshape(-1, 20).astype(f)
+    lmin = np.empty(H, f)
+    lmax = np.empty(H, f)
+    for l0 in range(0, H, 256):
+        l1 = min(H, l0 + 256)
+        # y of the top and bottom edge of every layer of the block: [L, 2]
+        y = (o[1] + (np.arange(l0, l1)[:, None] + np.arange(2)[None, :]).astype(f) * vx).astype(f)
+        y = y[:, :, None, None]                                     # [L, 2, 1, 1]
+        r = lambda k: m[:, k][None, None, :, None]                  # [1, 1, R, 1]
+        Y = ((r(4) * cx + r(5) * y) + r(6) * cz) + r(7)
+        Wh = ((r(12) * cx + r(13) * y) + r(14) * cz) + r(15)
+        v = (Y / Wh).reshape(l1 - l0, -1)
+        lmin[l0:l1] = v.min(axis=1)
+        lmax[l0:l1] = v.max(axis=1)
+    return lmin, lmax
+
+
+def plan_chunks(projdata: np.ndarray, grid: dict, det_width: int, det_height: int,
+                concurrent: int, storage_bytes: int, budget: float, overlap_fct: float = 1.0):
+    """GeometryChunk::computeChunks and its helpers (geometry_chunk.hpp:92-197)
+    under the build's intended semantics, as [vOffset, vSize, iOffset, iSize,
+    vPadTop, vPadBot] per chunk:
+
+    * every layer 0..H-1 belongs to one chunk (the reference's loop runs over
+      its first..last complete layers and drops the layer that does not fit,
+      :94-111); a layer that does not fit starts the next chunk;
+    * a chunk's detector rows are [floor(min)-1, ceil(max - iOffset)+2) of its
+      first and last layers' bounds, clipped to the detector (:164-168);
+    * padding: the layers above (below) whose row bounds reach the rows
+      (:175-188), the bottom one up to layer H-1 (the reference stops at
+      H-2), each count scaled by overlap_fct and rounded up;
+    * a chunk fits when its device buffers (padded volume in the storage
+      type; images, u64 sums and error over the chunk's rows) plus the
+      ProjData are below the budget (the reference's sizeInMemory, :62-72,
+      with this build's buffers)."""
+    f = np.float32
+    lmin, lmax = layer_row_bounds(projdata, grid)
+    H, W = int(grid["vheight"]), int(grid["vwidth"])
+    pd_bytes = projdata.size * 4
+
+    def rows_of(off, size):
+        mn = min(lmin[off], lmin[off + size - 1])
+        mx = max(lmax[off], lmax[off + size - 1])
+        i_off = int(max(np.floor(mn) - f(1), f(0)))
+        i_size = int(max(min(np.ceil(f(mx) - f(i_off)) + f(2), f(det_height - i_off)), f(0)))
+        return i_off, i_size
+
+    def pads(off, size, i_off, i_size):
+        top = 0
+        while off - top > 0 and lmax[off - top - 1] > f(i_off - 1):
+            top += 1
+        bot = 0
+        while off + size + bot < H and lmin[off + size + bot] < f(i_off + i_size + 1):
+            bot += 1
+        fct = f(overlap_fct)
+        return int(np.ceil(f(top) * fct)), int(np.ceil(f(bot) * fct))
+
+    def make(off, size):
+        i_off, i_size = rows_of(off, size)
+        top, bot = pads(off, size, i_off, i_size)
+        return [off, size, i_off, i_size, top, bot]
+
+    def nbytes(c):
+        vol = (c[1] + c[4] + c[5]) * W * W * storage_bytes
+        img = c[3] * det_width * concurrent
+        return vol + img * (storage_bytes + 8 + storage_bytes)
+
+    chunks = []
+    off, size = 0, 0
+    cur = None
+    for _ in range(H):
+        trial = make(off, size + 1)
+        if nbytes(trial) + pd_bytes < budget:
+            cur, size = trial, size + 1
+            continue
+        if size == 0:
+            raise RuntimeError("Not enough GPU memory for the chunk division")
+        chunks.append(cur)
+        off, size = off + size, 1
+        cur = make(off, 1)
+        if not nbytes(cur) + pd_bytes < budget:
+            raise RuntimeError("Not enough GPU memory for the chunk division")
+    chunks.append(cur)
+    return chunks
+
+
+def reconstruct_chunked(images: np.ndarray, projdata: np.ndarray, grid: dict, it: int, sit: int,
+                        weight: float, weight_factor: float, jitter, chunks, modules: int = 1,
+                        threads: int = 1):
+    """FP32 canonical method_2 schedule (reconstruction_chunk.hpp:36-131): each
+    chunk [vOffset, vSize, iOffset, iSize, vPadTop, vPadBot] is an independent
+    naive reconstruction of its padded layers against the detector rows
+    [iOffset, iOffset+iSize) (kernels bounded by those rows, the Error over
+    W*iSize*A_s pixels), with the next it*sit rows of jitter; only the chunk's
+    own layers are kept.  images: [N, H, W] preprocessed."""
+    n, h, w = images.shape
+    W = grid["vwidth"]
+    a_s = n // sit
+    vx = grid["vx"]
+    out = np.empty((grid["vheight"], W, W), np.float32)
+    k = 0
+    for (v_off, v_size, i_off, i_size, top, bot) in chunks:
+        l0, l1 = v_off - top, v_off + v_size + bot
+        vol = np.ones((l1 - l0, W, W), np.float32)
+        sums = np.full((a_s, i_size, w), 1 << 22, np.uint64)
+        wgt = np.float32(weight)
+        for mit in range(it):
+            for s in range(sit):
+                views = sit_views(n, sit, s)
+                pd = np.ascontiguousarray(
+                    np.concatenate([projdata[v * modules:(v + 1) * modules] for v in views]))
+                img = np.ascontiguousarray(images[views][:, i_off:i_off + i_size])
+                o = [np.float32(np.float32(grid["orig"][i]) + np.float32(jitter[k][i]))
+                     for i in range(3)]
+                k += 1
+                g = make_geom(w, a_s, modules, (0, l0, 0), (W, l1, W), i_off, i_size, o,
+                              (vx, vx, vx))
+                if s > 0 or mit > 0:
+                    forward_f32(vol, sums, pd, g, threads)
+                err = error_f32(img, sums, float(wgt))
+                backward_f32(vol, err, pd, g, threads)
+                sums[...] = 0
+            wgt = np.float32(wgt * np.float32(weight_factor))
+        out[v_off:v_off + v_size] = vol[top:top + v_size]
+    return out
+
+
+def reproject(volume: np.ndarray, projdata: np.ndarray, grid: dict, det_width: int,
+              det_height: int, n_proj: int, sit: int, jitter, modules: int = 1,
+              samples: int = 32, threads: int = 1):
+    """forward_exec (src/reconstruction.hpp:66-158): the volume divided by
+    `samples` and zeroed outside the inscribed circle of each layer
+    (sqrt((i-W/2)^2 + (j-W/2)^2) >= W/2 - 1, float32), forward projected
+    `samples` times per sub-iteration with the jitter rows in order
+    (sub-iteration-major) into Q.22 sums; returns {view: float32 image},
+    image = float(sum) / 2^22 (FIXED_TO_FLOAT)."""
+    f = np.float32
+    W, H = int(grid["vwidth"]), int(grid["vheight"])
+    vol = (volume.astype(f) / f(samples)).astype(f)
+    i = np.arange(W, dtype=f)
+    di = (i - f(W) * f(0.5)).astype(f)
+    r = np.sqrt((di[None, :] * di[None, :] + di[:, None] * di[:, None]).astype(f)).astype(f)
+    vol[:, r >= f(W) * f(0.5) - f(1.0)] = f(0.0)
+    vol = np.ascontiguousarray(vol)
+    a_s = n_proj // sit
+    vx = grid["vx"]
+    h, w = int(det_height), int(det_width)
+    out = {}
+    k = 0
+    for s in range(sit):
+        views = sit_views(n_proj, sit, s)
+        pd = np.ascontiguousarray(
+            np.concatenate([projdata[v * modules:(v + 1) * modules] for v in views]))
+        sums = np.zeros((a_s, h, w), np.uint64)
+        for _ in range(samples):
+            o = [f(f(grid["orig"][c]) + f(jitter[k][c])) for c in range(3)]
+            k += 1
+            g = make_geom(w, a_s, modules, (0, 0, 0), (W, H, W), 0, h, o, (vx, vx, vx))
+            forward_f32(vol, sums, pd, g, threads)
+        img = (sums.astype(np.float64).astype(f) / f(1 << 22)).astype(f)
+        for j, v in enumerate(views):
+            out[v] = img[j]
+    return out

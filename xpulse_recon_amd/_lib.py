@@ -152,6 +152,8 @@ _SIGS = {
     "xpr_geometry_frames": (C.c_int, [C.POINTER(GeometryParams), _P]),
     "xpr_reconstruct": (C.c_int, [C.c_char_p, C.c_char_p, C.c_size_t]),
     "xpr_template_json": (C.c_int, [C.c_char_p, C.c_size_t]),
+    "xpr_plan_chunks": (C.c_int, [C.c_char_p, C.c_int64, C.c_int64, C.c_int64, _P, C.c_int64,
+                                  C.POINTER(C.c_int64)]),
     "xpr_reconstruct_mem": (C.c_int, [C.c_char_p, _P, C.c_int64, C.c_int64, C.c_int64, _P,
                                       C.c_char_p, C.c_size_t]),
     "xpr_server_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint64, C.c_int, C.c_int]),

@@ -80,6 +80,8 @@ struct XprSection {
     const float *mem_projections = nullptr;   // [n][h][w] raw images
     int64_t mem_width = 0, mem_height = 0, mem_count = 0;
     float *mem_volume_out = nullptr;          // [vheight][vwidth][vwidth]
+    bool shape_only = false;                  // geometry of mem_width x mem_height x mem_count
+                                              // projections only: no data, no output files
     bool quiet = false;                       // no progress output
 };
 

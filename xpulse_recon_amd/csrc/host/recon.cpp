@@ -138,7 +138,7 @@ struct Event {
 // ---------------------------------------------------------------------------
 Dataset::Dataset(Parameters &p) : p_(p)
 {
-    if (p.xpr.mem_projections) {                  // in-process job
+    if (p.xpr.mem_projections || p.xpr.shape_only) {   // in-process job or planning
         dwidth_ = p.xpr.mem_width;
         dheight_ = p.xpr.mem_height;
         nproj_ = p.xpr.mem_count;
@@ -146,9 +146,10 @@ Dataset::Dataset(Parameters &p) : p_(p)
             throw std::runtime_error("empty projection stack");
     }
     // collectFromDirectory + checkTiffFileInfos (dataset.hpp:214-269)
-    if (!p.xpr.mem_projections && !fs::is_directory(p.recon.input))
+    const bool files = !p.xpr.mem_projections && !p.xpr.shape_only;
+    if (files && !fs::is_directory(p.recon.input))
         throw std::runtime_error("input directory not found: " + p.recon.input);
-    if (!p.xpr.mem_projections)
+    if (files)
     for (const std::string &f : list_tiffs(p.recon.input)) {
         TiffInfo i = tiff_info(f);
         if (i.sample_format != 3 || i.bits != 32)
@@ -162,7 +163,7 @@ Dataset::Dataset(Parameters &p) : p_(p)
         }
         if (i.width == dwidth_ && i.height == dheight_) files_.push_back(f);
     }
-    if (!p.xpr.mem_projections) {
+    if (files) {
         if (files_.empty()) throw std::runtime_error("No valid TIFF file found in the directory.");
         nproj_ = int64_t(files_.size());
     }
@@ -203,12 +204,12 @@ Dataset::Dataset(Parameters &p) : p_(p)
     vwidth_ = grid.vwidth;
     vheight_ = grid.vheight;
     for (int i = 0; i < 3; ++i) orig_[i] = float(grid.orig[i]);
-    if (!p.xpr.mem_volume_out || p.xpr.layer_store == "zstd") {   // layers or spill files
+    if (!p.xpr.shape_only && (!p.xpr.mem_volume_out || p.xpr.layer_store == "zstd")) {
         fs::create_directories(p.recon.output);
         if (!p.recon.proj_output.empty()) fs::create_directories(p.recon.proj_output);
         proj_dir_ = (fs::path(p.recon.output) / "projs").string();
     }
-    if (!p.xpr.quiet)
+    if (!p.xpr.quiet && !p.xpr.shape_only)
         std::cout << "Volume dimension : " << vwidth_ << "x" << vheight_ << "x" << vwidth_ << "\n"
                   << "Pixel size : " << p.detector.px << ", Voxel size : " << vx_ << std::endl;
 }
