@@ -180,6 +180,10 @@ int xpr_subiteration_static(xpr_stream s, void *volume, int vol_dtype, const voi
  * src/reconstruction.hpp:139-153). */
 int xpr_q22_to_float(xpr_stream s, const void *sums, int acc_bits, float *out, uint64_t n);
 
+/* x <- x * factor for n floats: the de-normalisation of the final layers
+ * (DataLoader::saveLayer with finalize, data_loader.hpp:135-139). */
+int xpr_scale(xpr_stream s, float *x, uint64_t n, float factor);
+
 /* ---- preprocessing (DataLoader::initializeTempImages,
  *      dataset/data_loader.hpp:48-101) -------------------------------------- */
 
@@ -298,6 +302,14 @@ int xpr_reconstruct(const char *params, char *record, size_t record_cap);
  * float32, de-normalised like the written layers.  Any schedule of
  * recon.method. */
 int xpr_reconstruct_mem(const char *params, const float *projections, int64_t width,
+                        int64_t height, int64_t count, float *volume_out, char *record,
+                        size_t record_cap);
+/* xpr_reconstruct_mem with both buffers in device memory (HBM) of the
+ * current device: the projections are preprocessed straight from
+ * `projections` (device float32, [count][height][width], raw) and the
+ * de-normalised volume is written to `volume_out` (device float32,
+ * [vheight][vwidth][vwidth]); no host copy of either is made. */
+int xpr_reconstruct_dev(const char *params, const float *projections, int64_t width,
                         int64_t height, int64_t count, float *volume_out, char *record,
                         size_t record_cap);
 /* The method_2 chunk plan (GeometryChunk, dataset/geometry_chunk.hpp:92-197,

@@ -218,17 +218,17 @@ def raw_projections(scan, cuda):
     return proj
 
 
-def _reconstruct(proj, method, storage):
+def _reconstruct(proj, method, storage, store="ram"):
     from xpulse_recon_amd import _lib
     params = {"recon": {"method": method, "it": 1, "sit": SIT, "mlog": True, "weight": 1.0,
                         "weight_factor": 0.9},
               "geometry": {"so": 100.0, "sd": 1000.0},
               "detector": {"px": 0.1, "rx": 2.5, "roll": 0.3},
-              "xpr": {"jitter_seed": 7, "layer_store": "ram", "slab_height": SLAB,
+              "xpr": {"jitter_seed": 7, "layer_store": store, "slab_height": SLAB,
                       "storage": storage, "final_fp32": False, "quiet": True}}
     vol = np.empty((D, D, D), np.float32)
     rec = C.create_string_buffer(1 << 16)
-    print(f"xpr_reconstruct_mem {method} {storage} 768^3", flush=True)
+    print(f"xpr_reconstruct_mem {method} {storage} {store} 768^3", flush=True)
     rc = _lib.load().xpr_reconstruct_mem(json.dumps(params).encode(),
                                          proj.ctypes.data_as(C.c_void_p), D, D, NV,
                                          vol.ctypes.data_as(C.c_void_p), rec, len(rec))
@@ -236,10 +236,14 @@ def _reconstruct(proj, method, storage):
     return vol, json.loads(rec.value.decode())
 
 
-@pytest.mark.parametrize("storage", ["f32", "f16"])
-def test_engine_method_1a_equals_naive(scan, raw_projections, storage):
+@pytest.mark.parametrize("storage,store", [("f32", "ram"), ("f32", "device"), ("f16", "ram")])
+def test_engine_method_1a_equals_naive(scan, raw_projections, storage, store):
+    """The engine's method_1a with the layer store in pinned host memory (12
+    slabs streamed in and out per pass) or in HBM (slabs launched in place)
+    equals its naive schedule bit for bit."""
     naive, rn = _reconstruct(raw_projections, "naive", storage)
-    slab, rs = _reconstruct(raw_projections, "method_1a", storage)
+    slab, rs = _reconstruct(raw_projections, "method_1a", storage, store)
     assert rn["jitter"] == rs["jitter"] and len(rs["subiteration_ms"]) == SIT
+    assert rs["layer_store"] == store and rs["launches_per_pass"] == D // SLAB
     assert np.isfinite(naive).all()
     np.testing.assert_array_equal(slab, naive)

@@ -94,16 +94,48 @@ def test_naive_matches_oracle(scan):
 
 
 @pytest.mark.parametrize("method,xpr", [
-    ("method_1a", {"slab_height": 7}),
+    ("method_1a", {"slab_height": 7, "layer_store": "zstd"}),
+    ("method_1a", {"slab_height": 7, "layer_store": "ram"}),
+    ("method_1a", {"slab_height": 7, "layer_store": "device"}),
+    ("method_1a", {"slab_height": 7, "layer_store": "ram", "storage": "f16"}),
     ("method_1b", {"slab_height": 16, "layer_store": "ram", "threads": 4}),
-    ("method_1c", {"slab_height": 5}),
+    ("method_1b", {"slab_height": 6, "layer_store": "zstd", "threads": 4, "storage": "bf16"}),
+    ("method_1c", {"slab_height": 5, "layer_store": "zstd"}),
+    ("method_1c", {"slab_height": 5, "layer_store": "device"}),
+    # slab height from the budget: room for two 13-layer slab buffers beside
+    # the sums, error and images of one sub-iteration, ProjData and the two
+    # preprocessing upload slots
+    ("method_1a", {"layer_store": "ram",
+                   "device_budget_gb": (2 * 13 * W * W * 4 + 6 * W * H * 16 + 24 * 80 +
+                                        2 * 6 * W * H * 4 + 100) / 1e9}),
 ])
 def test_streaming_schedules_equal_naive(scan, method, xpr):
+    """method_1a/1b/1c over every layer store (zstd spill files, pinned RAM,
+    HBM) and storage type give the naive schedule's volume bit for bit."""
     t, spec, d = scan
-    naive, _ = _run(t, "naive_ref", d, method="naive")
-    vol, log = _run(t, method, d, method=method, xpr=dict(xpr))
+    storage = xpr.get("storage", "f32")
+    naive, _ = _run(t, f"naive_ref_{storage}", d, method="naive",
+                    xpr={"storage": storage, "final_fp32": False})
+    name = f"{method}_{xpr['layer_store']}_{storage}_{xpr.get('slab_height', 0)}"
+    vol, log = _run(t, name, d, method=method, xpr=dict(xpr, final_fp32=False))
     np.testing.assert_array_equal(vol, naive)
-    assert not list((t / method).glob("*.zstd"))          # spill removed at the end
+    assert log["layer_store"] == xpr["layer_store"]
+    if "slab_height" not in xpr:
+        assert log["slab_height"] == 13
+    assert log["launches_per_pass"] == -(-H // log["slab_height"])
+    assert len(log["forward_pass_ms"]) == 7 and len(log["backward_pass_ms"]) == 8
+    assert not list((t / name).glob("*.zstd"))          # spill removed at the end
+
+
+def test_auto_layer_store(scan):
+    """layer_store auto: HBM when the whole volume fits the budget (one launch
+    per pass when no slab height is given), else zstd spill files."""
+    t, spec, d = scan
+    _, log = _run(t, "auto_dev", d, method="method_1a", xpr={})
+    assert log["layer_store"] == "device" and log["slab_height"] == H
+    assert log["images_resident"]
+    _, log = _run(t, "auto_zstd", d, method="method_1a", xpr={"device_budget_gb": 0.0003})
+    assert log["layer_store"] == "zstd" and 1 <= log["slab_height"] < H
 
 
 def test_method_2_single_chunk_equals_naive(scan):
@@ -214,3 +246,45 @@ def test_reconstruct_mem_equals_cli(scan, method, xpr):
     if method == "method_2":
         assert len(record["chunks"]) >= 2
     np.testing.assert_array_equal(vol, ref)
+
+
+@pytest.mark.parametrize("method,xpr,normalize", [
+    ("naive", {}, False),
+    ("naive", {"storage": "f16"}, True),
+    ("method_2", {"device_budget_gb": 0.30 * (W * W * H * 4 + 6 * W * H * 16) / 1e9}, True),
+    ("method_1a", {"slab_height": 9, "layer_store": "device"}, False),
+    ("method_1a", {"slab_height": 9, "layer_store": "ram"}, True),
+    ("method_1c", {"slab_height": 5, "layer_store": "zstd"}, False),
+])
+def test_reconstruct_dev_equals_cli(scan, method, xpr, normalize):
+    """xpr_reconstruct_dev (projections read from and the volume written to
+    device memory, no host copy) gives the CLI's layers bit for bit, with and
+    without normalisation, for every kind of schedule; the reprojection of a
+    device job reads the device volume."""
+    import ctypes as C
+    import torch
+    from xpulse_recon_amd import _lib
+    t, spec, d = scan
+    pout = t / f"reproj_dev_{method}"
+    ref, log = _run(t, f"clid_{method}_{normalize}", d, method=method, xpr=dict(xpr),
+                    normalize=normalize, proj_output=str(t / f"reproj_cli_{method}"))
+    raw = np.stack([read_f32(d / f"proj_{a:04d}.tif") for a in range(N)])
+    params = {"recon": dict(it=2, sit=4, method=method, normalize=normalize,
+                            output=str(t / f"devout_{method}"), proj_output=str(pout)),
+              "geometry": {"so": 100.0, "sd": 1000.0}, "detector": {"px": 0.1, "rx": 1.5},
+              "xpr": dict(xpr, jitter_seed=11)}
+    dproj = torch.from_numpy(raw).cuda()
+    dvol = torch.full(ref.shape, -1.0, device="cuda")
+    torch.cuda.synchronize()
+    rec = C.create_string_buffer(1 << 16)
+    print(f"xpr_reconstruct_dev {method}", flush=True)
+    rc = _lib.load().xpr_reconstruct_dev(json.dumps(params).encode(),
+                                         C.c_void_p(dproj.data_ptr()), W, H, N,
+                                         C.c_void_p(dvol.data_ptr()), rec, len(rec))
+    assert rc == 0
+    record = json.loads(rec.value.decode())
+    assert record["jitter"] == log["jitter"]
+    np.testing.assert_array_equal(dvol.cpu().numpy(), ref)
+    for a in (0, 7, N - 1):
+        np.testing.assert_array_equal(read_f32(pout / f"p_{a:010d}.tif"),
+                                      read_f32(t / f"reproj_cli_{method}" / f"p_{a:010d}.tif"))

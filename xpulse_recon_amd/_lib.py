@@ -143,6 +143,7 @@ _SIGS = {
     "xpr_tune_reset": (C.c_int, []),
     "xpr_check_fast_paths": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "xpr_q22_to_float": (C.c_int, [_P, _P, C.c_int, _P, C.c_uint64]),
+    "xpr_scale": (C.c_int, [_P, _P, C.c_uint64, C.c_float]),
     "xpr_minmax_init": (C.c_int, [_P, _P]),
     "xpr_minmax_normal": (C.c_int, [_P, _P, C.c_uint64, _P]),
     "xpr_minmax_read": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
@@ -152,6 +153,8 @@ _SIGS = {
     "xpr_geometry_frames": (C.c_int, [C.POINTER(GeometryParams), _P]),
     "xpr_reconstruct": (C.c_int, [C.c_char_p, C.c_char_p, C.c_size_t]),
     "xpr_template_json": (C.c_int, [C.c_char_p, C.c_size_t]),
+    "xpr_reconstruct_dev": (C.c_int, [C.c_char_p, _P, C.c_int64, C.c_int64, C.c_int64, _P,
+                                      C.c_char_p, C.c_size_t]),
     "xpr_plan_chunks": (C.c_int, [C.c_char_p, C.c_int64, C.c_int64, C.c_int64, _P, C.c_int64,
                                   C.POINTER(C.c_int64)]),
     "xpr_reconstruct_mem": (C.c_int, [C.c_char_p, _P, C.c_int64, C.c_int64, C.c_int64, _P,

@@ -12,7 +12,7 @@
 namespace {
 
 int run(const char *params, const float *projections, int64_t width, int64_t height,
-        int64_t count, float *volume_out, char *record, size_t record_cap)
+        int64_t count, float *volume_out, char *record, size_t record_cap, bool on_device = false)
 {
     if (record && record_cap) record[0] = '\0';
     if (!params) return -1;                                  // RETURN_CODE_ERROR_ARGUMENTS
@@ -29,6 +29,7 @@ int run(const char *params, const float *projections, int64_t width, int64_t hei
         p.xpr.mem_height = height;
         p.xpr.mem_count = count;
         p.xpr.mem_volume_out = volume_out;
+        p.xpr.mem_on_device = on_device;
     }
     try {
         auto recon = xpr::make_reconstruction(p);
@@ -97,6 +98,14 @@ extern "C" int xpr_plan_chunks(const char *params, int64_t width, int64_t height
         return xprint_fail(XPR_E_NOMEM, e.what());
     }
     return XPR_OK;
+}
+
+extern "C" int xpr_reconstruct_dev(const char *params, const float *projections, int64_t width,
+                                   int64_t height, int64_t count, float *volume_out, char *record,
+                                   size_t record_cap)
+{
+    if (!projections || !volume_out || width <= 0 || height <= 0 || count <= 0) return -1;
+    return run(params, projections, width, height, count, volume_out, record, record_cap, true);
 }
 
 extern "C" int xpr_template_json(char *out, size_t cap)

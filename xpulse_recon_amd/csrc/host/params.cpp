@@ -172,10 +172,12 @@ Parameters parameters_from_json(const Json &root)
         rdn(x, "jitter_seed", q.jitter_seed, "xpr");
         rd(x, "geometry_fp64", q.geometry_fp64, "xpr");
         rdn(x, "slab_height", q.slab_height, "xpr");
-        if (q.slab_height < 1) bad("xpr.slab_height", "must be >= 1");
+        if (q.slab_height < 0) bad("xpr.slab_height", "must be >= 0 (0: from the device budget)");
         rdn(x, "device_budget_gb", q.device_budget_gb, "xpr");
         rd(x, "layer_store", q.layer_store, "xpr");
-        if (q.layer_store != "zstd" && q.layer_store != "ram") bad("xpr.layer_store", "zstd or ram");
+        if (q.layer_store != "zstd" && q.layer_store != "ram" && q.layer_store != "device" &&
+            q.layer_store != "auto")
+            bad("xpr.layer_store", "device, ram, zstd or auto");
         rdn(x, "threads", q.threads, "xpr");
         rd(x, "run_log", q.run_log, "xpr");
         rd(x, "autotune", q.autotune, "xpr");

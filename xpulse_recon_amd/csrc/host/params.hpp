@@ -69,9 +69,11 @@ struct XprSection {
     bool jitter = true;              // per-sub-iteration origin jitter U[0, vx)^3
     int64_t jitter_seed = -1;        // -1: std::random_device like the reference
     bool geometry_fp64 = false;      // FP64 ProjData construction (default: float32 as glm)
-    int64_t slab_height = 64;        // layers per slab for method_1a/1b/1c
+    int64_t slab_height = 0;         // layers per slab for method_1a/1b/1c; 0: from the budget
     double device_budget_gb = 0.0;   // 0: free device memory
-    std::string layer_store = "zstd";// method_1a/1b/1c layer spill: "zstd" or "ram"
+    std::string layer_store = "auto";// method_1a/1b/1c layer store between passes: "device"
+                                     // (HBM), "ram" (pinned host memory), "zstd" (host spill
+                                     // files) or "auto" (device when it fits the budget, else zstd)
     int threads = 0;                 // host IO threads (0: hardware concurrency, max 16)
     std::string run_log = "";        // JSON run record (jitter, timings, memory)
     bool autotune = true;            // time the projectors' tile shapes once per launch shape
@@ -80,6 +82,8 @@ struct XprSection {
     const float *mem_projections = nullptr;   // [n][h][w] raw images
     int64_t mem_width = 0, mem_height = 0, mem_count = 0;
     float *mem_volume_out = nullptr;          // [vheight][vwidth][vwidth]
+    bool mem_on_device = false;               // mem_projections / mem_volume_out are device
+                                              // addresses (inputs resident in HBM)
     bool shape_only = false;                  // geometry of mem_width x mem_height x mem_count
                                               // projections only: no data, no output files
     bool quiet = false;                       // no progress output

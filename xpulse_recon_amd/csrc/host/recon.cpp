@@ -206,9 +206,10 @@ Dataset::Dataset(Parameters &p) : p_(p)
     for (int i = 0; i < 3; ++i) orig_[i] = float(grid.orig[i]);
     if (!p.xpr.shape_only && (!p.xpr.mem_volume_out || p.xpr.layer_store == "zstd")) {
         fs::create_directories(p.recon.output);
-        if (!p.recon.proj_output.empty()) fs::create_directories(p.recon.proj_output);
         proj_dir_ = (fs::path(p.recon.output) / "projs").string();
     }
+    if (!p.xpr.shape_only && !p.recon.proj_output.empty())
+        fs::create_directories(p.recon.proj_output);
     if (!p.xpr.quiet && !p.xpr.shape_only)
         std::cout << "Volume dimension : " << vwidth_ << "x" << vheight_ << "x" << vwidth_ << "\n"
                   << "Pixel size : " << p.detector.px << ", Voxel size : " << vx_ << std::endl;
@@ -285,9 +286,10 @@ void Dataset::load(xpr_stream s)
 {
     if (loaded_) return;
     loaded_ = true;
-    // the RAM layer store is sized here, once: save_layer runs on several
-    // threads and must not resize the vector under them
-    if (ram_layers()) ram_store_.resize(size_t(vheight_));
+    // the RAM layer store is one pinned block, allocated here once (slabs are
+    // copied to and from it by DMA, and save_layer runs on several threads)
+    if (ram_layers() && !ram_block_.get())
+        ram_block_.alloc(size_t(vwidth_ * vwidth_ * vheight_) * 4);
     if (!p_.xpr.mem_projections) {
         raw_.assign(size_t(nproj_), {});
         parallel_for(nproj_, io_threads(p_), [&](int64_t i) { raw_[size_t(i)] = raw_image(i); });
@@ -297,11 +299,17 @@ void Dataset::load(xpr_stream s)
     AllocStreamScope scope(s);
     DevBuf d_mm(8);
     check(xpr_minmax_init(s, d_mm.as<int32_t>()), "minmax init");
-    std::vector<int64_t> all(static_cast<size_t>(nproj_));
-    for (int64_t i = 0; i < nproj_; ++i) all[size_t(i)] = i;
-    stream_views(s, all, 0, dheight_, [&](const float *d, size_t, size_t c) {
-        check(xpr_minmax_normal(s, d, uint64_t(c * n), d_mm.as<int32_t>()), "minmax");
-    });
+    if (device_io()) {                             // projections already in HBM
+        for (int64_t i = 0; i < nproj_; ++i)
+            check(xpr_minmax_normal(s, p_.xpr.mem_projections + size_t(i) * n, uint64_t(n),
+                                    d_mm.as<int32_t>()), "minmax");
+    } else {
+        std::vector<int64_t> all(static_cast<size_t>(nproj_));
+        for (int64_t i = 0; i < nproj_; ++i) all[size_t(i)] = i;
+        stream_views(s, all, 0, dheight_, [&](const float *d, size_t, size_t c) {
+            check(xpr_minmax_normal(s, d, uint64_t(c * n), d_mm.as<int32_t>()), "minmax");
+        });
+    }
     int32_t keys[2];
     check(xpr_download(s, keys, d_mm.get(), 8), "download");
     check(xpr_stream_sync(s), "sync");
@@ -314,19 +322,21 @@ void Dataset::upload_sit_images(xpr_stream s, int64_t sit, int64_t row0, int64_t
     load(s);
     const size_t n = size_t(rows * dwidth_);
     char *out = static_cast<char *>(dst);
+    if (device_io()) {                              // preprocess straight from HBM
+        if (row0 < 0 || rows <= 0 || row0 + rows > dheight_)
+            throw std::runtime_error("detector row window outside the projections");
+        const auto views = sit_views(sit);
+        for (size_t j = 0; j < views.size(); ++j)
+            check(xpr_preprocess(s, p_.xpr.mem_projections + size_t(views[j] * dheight_ + row0) *
+                                        size_t(dwidth_),
+                                 out + j * n * dtype_bytes(dt), dt, uint64_t(n),
+                                 p_.recon.normalize, p_.recon.mlog, mn_, mx_), "preprocess");
+        return;
+    }
     stream_views(s, sit_views(sit), row0, rows, [&](const float *d, size_t j0, size_t c) {
         check(xpr_preprocess(s, d, out + j0 * n * dtype_bytes(dt), dt, uint64_t(c * n),
                              p_.recon.normalize, p_.recon.mlog, mn_, mx_), "preprocess");
     });
-}
-
-void Dataset::finish_volume_sink()
-{
-    if (!p_.xpr.mem_volume_out || !p_.recon.normalize) return;
-    const float scale = mx_ - mn_;
-    float *v = p_.xpr.mem_volume_out;
-    const size_t n = size_t(vwidth_ * vwidth_ * vheight_);
-    for (size_t k = 0; k < n; ++k) v[k] = v[k] * scale;
 }
 
 void Dataset::save_layer(const float *data, int64_t layer, bool finalize)
@@ -334,18 +344,14 @@ void Dataset::save_layer(const float *data, int64_t layer, bool finalize)
     const size_t n = size_t(vwidth_ * vwidth_);
     if (finalize) {
         save_final_layers(data, layer, layer + 1);
-        if (ram_layers()) {
-            if (size_t(layer) < ram_store_.size()) std::vector<float>().swap(ram_store_[size_t(layer)]);
-        } else {
+        if (!ram_layers()) {
             std::error_code ec;
             fs::remove(numbered_path(p_.recon.output, "layer", layer, 5, "zstd"), ec);
         }
         return;
     }
     if (ram_layers()) {
-        if (size_t(layer) >= ram_store_.size())
-            throw std::logic_error("RAM layer store used before Dataset::load");
-        ram_store_[size_t(layer)].assign(data, data + n);
+        std::memcpy(ram_layer(layer), data, n * 4);
     } else {
         spill_write(numbered_path(p_.recon.output, "layer", layer, 5, "zstd"), data, n);
     }
@@ -354,13 +360,45 @@ void Dataset::save_layer(const float *data, int64_t layer, bool finalize)
 void Dataset::load_layer(int64_t layer, float *out) const
 {
     const size_t n = size_t(vwidth_ * vwidth_);
-    if (ram_layers() && size_t(layer) < ram_store_.size() && !ram_store_[size_t(layer)].empty()) {
-        std::memcpy(out, ram_store_[size_t(layer)].data(), n * 4);
+    if (ram_layers()) {
+        std::memcpy(out, ram_layer(layer), n * 4);
         return;
     }
     const std::string z = numbered_path(p_.recon.output, "layer", layer, 5, "zstd");
     if (fs::exists(z)) {
         spill_read(z, out, n);
+        return;
+    }
+    int64_t w = 0, h = 0;
+    auto v = tiff_read_f32(numbered_path(p_.recon.output, "layer", layer, 5, "tif"), &w, &h);
+    if (size_t(w * h) != n) throw std::runtime_error("layer file has the wrong size");
+    std::memcpy(out, v.data(), n * 4);
+}
+
+void Dataset::ensure_output_dir()
+{
+    if (!proj_dir_.empty()) return;
+    fs::create_directories(p_.recon.output);
+    proj_dir_ = (fs::path(p_.recon.output) / "projs").string();
+}
+
+float *Dataset::ram_layer(int64_t layer) const
+{
+    if (!ram_block_.get() || layer < 0 || layer >= vheight_)
+        throw std::logic_error("RAM layer store used before Dataset::load");
+    return ram_block_.as<float>() + size_t(layer) * size_t(vwidth_ * vwidth_);
+}
+
+void Dataset::load_final_layer(int64_t layer, float *out) const
+{
+    const size_t n = size_t(vwidth_ * vwidth_);
+    if (const float *sink = p_.xpr.mem_volume_out) {
+        if (device_io()) {
+            check(xpr_download(nullptr, out, sink + size_t(layer) * n, n * 4), "download");
+            check(xpr_stream_sync(nullptr), "sync");
+        } else {
+            std::memcpy(out, sink + size_t(layer) * n, n * 4);
+        }
         return;
     }
     int64_t w = 0, h = 0;
@@ -443,6 +481,7 @@ Reconstruction::Reconstruction(Parameters &p) : p_(p)
     ds_ = std::make_unique<Dataset>(p);
     check(xpr_stream_create(&stream_), "stream");
     check(xpr_stream_create(&copy_), "stream");
+    check(xpr_stream_create(&copy_back_), "stream");
     size_t fr = 0, tot = 0;
     check(xpr_mem_info(&fr, &tot), "memory info");
     const uint64_t reserve = 512ull << 20;                 // gpu/context.hpp:104-108
@@ -467,6 +506,7 @@ Reconstruction::~Reconstruction()
 {
     if (stream_) xpr_stream_destroy(stream_);
     if (copy_) xpr_stream_destroy(copy_);
+    if (copy_back_) xpr_stream_destroy(copy_back_);
 }
 
 std::array<float, 3> Reconstruction::next_origin()
@@ -499,15 +539,20 @@ void Reconstruction::check_memory(uint64_t need, const char *what)
                                  std::to_string(budget_ >> 20) + " MiB)");
 }
 
-void Reconstruction::log_sit(int64_t mit, int64_t sit, double ms)
+void Reconstruction::log_sit(int64_t mit, int64_t sit, double ms, double fwd_ms, double bwd_ms)
 {
     (void)mit;
     (void)sit;
     sit_ms_.arr.push_back(Json::number(ms));
-    record_.obj.erase(std::remove_if(record_.obj.begin(), record_.obj.end(),
-                                     [](const auto &kv) { return kv.first == "subiteration_ms"; }),
-                      record_.obj.end());
+    if (fwd_ms >= 0.0) fwd_ms_.arr.push_back(Json::number(fwd_ms));
+    if (bwd_ms >= 0.0) bwd_ms_.arr.push_back(Json::number(bwd_ms));
+    for (const char *k : {"subiteration_ms", "forward_pass_ms", "backward_pass_ms"})
+        record_.obj.erase(std::remove_if(record_.obj.begin(), record_.obj.end(),
+                                         [k](const auto &kv) { return kv.first == k; }),
+                          record_.obj.end());
     note("subiteration_ms", sit_ms_);
+    note("forward_pass_ms", fwd_ms_);
+    note("backward_pass_ms", bwd_ms_);
 }
 
 void Reconstruction::tune_shapes(void *vol_slab, int vdt, void *sums, void *err, int edt,
@@ -552,16 +597,45 @@ void upload_as(xpr_stream s, DevBuf &dst, int dt, const float *src, size_t n, De
     check(xpr_stream_sync(s), "sync");
 }
 
-void download_f32(xpr_stream s, float *dst, const DevBuf &src, int dt, size_t n, DevBuf &stage)
+void download_f32(xpr_stream s, float *dst, const void *src, int dt, size_t n, DevBuf &stage)
 {
     if (dt == XPR_F32) {
-        check(xpr_download(s, dst, src.get(), n * 4), "download");
+        check(xpr_download(s, dst, src, n * 4), "download");
     } else {
         if (stage.bytes() < n * 4) stage.alloc(n * 4);
-        check(xpr_convert(s, stage.get(), XPR_F32, src.get(), dt, n), "convert");
+        check(xpr_convert(s, stage.get(), XPR_F32, src, dt, n), "convert");
         check(xpr_download(s, dst, stage.get(), n * 4), "download");
     }
     check(xpr_stream_sync(s), "sync");
+}
+
+// Final layers [l0, l1) of the volume, from device memory of dtype dt, to
+// where the reconstruction writes them: the caller's device or host volume
+// buffer, or layer_%05d.tif files; de-normalised when normalising
+// (saveLayer(finalize), data_loader.hpp:133-146).  Returns with the data
+// written (synchronises s).
+void write_final(Dataset &ds, xpr_stream s, const void *dev, int dt, int64_t l0, int64_t l1,
+                 DevBuf &stage)
+{
+    const size_t layer = size_t(ds.vwidth() * ds.vwidth());
+    const size_t n = size_t(l1 - l0) * layer;
+    const float scale = ds.norm_max() - ds.norm_min();
+    float *sink = ds.volume_sink();
+    if (sink && ds.device_io()) {
+        float *dst = sink + size_t(l0) * layer;
+        check(xpr_convert(s, dst, XPR_F32, dev, dt, n), "convert");
+        if (ds.normalize()) check(xpr_scale(s, dst, n, scale), "scale");
+        check(xpr_stream_sync(s), "sync");
+    } else if (sink) {
+        float *dst = sink + size_t(l0) * layer;
+        download_f32(s, dst, dev, dt, n, stage);
+        if (ds.normalize())
+            for (size_t k = 0; k < n; ++k) dst[k] = dst[k] * scale;
+    } else {
+        std::vector<float> tmp(n);
+        download_f32(s, tmp.data(), dev, dt, n, stage);
+        ds.save_final_layers(tmp.data(), l0, l1);
+    }
 }
 
 void fill_sums(xpr_stream s, DevBuf &sums, int acc, uint64_t value, uint64_t n)
@@ -663,7 +737,8 @@ public:
         fill_sums(stream_, sums, p_.xpr.accumulator, 1ull << 22, uint64_t(As * P));
         upload(0);
         int vdt = st;
-        Event e0, e1;
+        Event e0, e1, fb, fe, bb, be;
+        note("launches_per_pass", Json::number(1));
         for (int64_t mit = 0; mit < p_.recon.it; ++mit) {
             ops.setWeight(next_weight(mit));
             for (int64_t s = 0; s < S; ++s) {
@@ -682,27 +757,29 @@ public:
                     ops.setErrorBuffer(err32.get(), XPR_F32);
                     vdt = XPR_F32;
                 }
-                if (s > 0 || mit > 0) ops.forward();
+                const bool fwd = s > 0 || mit > 0;
+                if (fwd) {
+                    check(xpr_event_record(fb.e, stream_), "event");
+                    ops.forward();
+                    check(xpr_event_record(fe.e, stream_), "event");
+                }
                 ops.error();
+                check(xpr_event_record(bb.e, stream_), "event");
                 ops.backward();
+                check(xpr_event_record(be.e, stream_), "event");
                 fill_sums(stream_, sums, p_.xpr.accumulator, 0, uint64_t(As * P));
                 check(xpr_event_record(e1.e, stream_), "event");
                 if (mit == 0 && s + 1 < S) upload(s + 1);
                 check(xpr_event_sync(e1.e), "sync");
-                float ms = 0;
+                float ms = 0, fms = -1.0f, bms = 0;
                 check(xpr_event_elapsed_ms(e0.e, e1.e, &ms), "elapsed");
-                log_sit(mit, s, ms);
+                if (fwd) check(xpr_event_elapsed_ms(fb.e, fe.e, &fms), "elapsed");
+                check(xpr_event_elapsed_ms(bb.e, be.e, &bms), "elapsed");
+                log_sit(mit, s, ms, fms, bms);
             }
         }
         const DevBuf &result = vdt == XPR_F32 && final32 ? vol32 : vol;
-        if (float *sink = ds.volume_sink()) {
-            download_f32(stream_, sink, result, vdt, V, stage);
-            ds.finish_volume_sink();
-        } else {
-            std::vector<float> out(V);
-            download_f32(stream_, out.data(), result, vdt, V, stage);
-            ds.save_final_layers(out.data(), 0, H);
-        }
+        write_final(ds, stream_, result.get(), vdt, 0, H, stage);
     }
 };
 
@@ -718,42 +795,101 @@ class ReconstructionSlab : public Reconstruction {
 public:
     ReconstructionSlab(Parameters &p, int threads, bool cull) : Reconstruction(p), threads_(threads), cull_(cull)
     {
+        resolve();
         check_memory(requiredGPUMemory(), method_name(p.recon.method));
     }
-    uint64_t requiredGPUMemory() override
-    {
-        const uint64_t sv = dtype_bytes(storage());
-        const uint64_t P = uint64_t(ds_->width() * ds_->height()), As = uint64_t(ds_->concurrent());
-        const uint64_t acc = uint64_t(p_.xpr.accumulator / 8);
-        const uint64_t slab = uint64_t(std::min<int64_t>(p_.xpr.slab_height, ds_->vheight())) *
-                              uint64_t(ds_->vwidth() * ds_->vwidth());
-        return 2 * slab * (sv + 4) + As * P * (sv + acc + sv + 4) + ds_->projdata().size() * 4;
-    }
+    uint64_t requiredGPUMemory() override { return need(store_, hs_, resident_); }
 
     void exec() override
     {
         AllocStreamScope scope(stream_);
         auto &ds = *ds_;
         ds.load(stream_);
-        const int st = storage();
+        st_ = storage();
+        sb_ = dtype_bytes(st_);
         const int64_t W = ds.vwidth(), H = ds.vheight(), P = ds.width() * ds.height();
-        const int64_t As = ds.concurrent(), S = p_.recon.sit;
-        const int64_t Hs = std::min<int64_t>(p_.xpr.slab_height, H);
-        const size_t layer = size_t(W * W), slab_n = size_t(Hs) * layer;
-        const int64_t nslab = (H + Hs - 1) / Hs;
-        DevBuf img(uint64_t(As * P) * dtype_bytes(st)), sums(uint64_t(As * P) * uint64_t(p_.xpr.accumulator / 8)),
-            err(uint64_t(As * P) * dtype_bytes(st)), pdb, stage;
-        DevBuf dslab[2] = {DevBuf(slab_n * dtype_bytes(st)), DevBuf(slab_n * dtype_bytes(st))};
-        DevBuf dstage[2] = {DevBuf(slab_n * 4), DevBuf(slab_n * 4)};
-        HostBuf hslab[2] = {HostBuf(slab_n * 4), HostBuf(slab_n * 4)};
-        Event up[2], done[2], e0, e1;
+        const int64_t As = ds.concurrent(), S = p_.recon.sit, mods = p_.detector.module_number;
+        layer_ = size_t(W * W);
+        nslab_ = (H + hs_ - 1) / hs_;
+        const size_t slab_n = size_t(hs_) * layer_;
+        DevBuf sums(uint64_t(As * P) * uint64_t(p_.xpr.accumulator / 8)), err(uint64_t(As * P) * sb_);
+        std::vector<DevBuf> images(static_cast<size_t>(resident_ ? S : 1));
+        for (auto &b : images) b.alloc(uint64_t(As * P) * sb_);
+        // ProjData of every sub-iteration and, for method_1c, the views of each
+        // slab, uploaded once (the culling widens the slab by a voxel, so it
+        // does not depend on the jitter)
+        pds_.clear();
+        pds_.resize(size_t(S));
+        cviews_.clear();
+        cviews_.resize(size_t(S));
+        for (int64_t s = 0; s < S; ++s) {
+            const auto pd = ds.sit_projdata(s);
+            pds_[size_t(s)].alloc(pd.size() * 4);
+            check(xpr_upload(stream_, pds_[size_t(s)].get(), pd.data(), pd.size() * 4), "upload");
+            if (!cull_) continue;
+            const auto lists = culled(pd, nslab_, hs_);
+            cviews_[size_t(s)].resize(lists.size());
+            for (size_t b = 0; b < lists.size(); ++b) {
+                SlabDev &d = cviews_[size_t(s)][b];
+                d.n = int64_t(lists[b].index.size());
+                d.pd.alloc(std::max<size_t>(lists[b].pd.size(), 20) * 4);
+                d.idx.alloc(std::max<size_t>(lists[b].index.size(), 1) * 4);
+                if (d.n > 0) {
+                    check(xpr_upload(stream_, d.pd.get(), lists[b].pd.data(), lists[b].pd.size() * 4), "upload");
+                    check(xpr_upload(stream_, d.idx.get(), lists[b].index.data(), lists[b].index.size() * 4), "upload");
+                }
+            }
+        }
+        if (store_ == kDevice) {
+            vol_.alloc(uint64_t(H) * layer_ * sb_);
+            check(xpr_fill(stream_, vol_.get(), st_, 1.0f, uint64_t(H) * layer_), "fill");
+        } else {
+            for (int k = 0; k < 2; ++k) {
+                dslab_[k].alloc(slab_n * sb_);
+                if (st_ != XPR_F32) dup_[k].alloc(slab_n * 4);
+                if (store_ == kZstd) {
+                    hin_[k].alloc(slab_n * 4);
+                    hout_[k].alloc(slab_n * 4);
+                }
+            }
+            check(xpr_fill(stream_, dslab_[0].get(), st_, 1.0f, slab_n), "fill");
+        }
         Operators ops(stream_, p_.xpr.accumulator);
         ops.setSumImagesBuffer(sums.get());
-        ops.setErrorBuffer(err.get(), st);
-        ops.setImagesBuffer(img.get(), st);
+        ops.setErrorBuffer(err.get(), st_);
         ops.setImageParameters(ds.width(), ds.height(), As);
         ops.setImageOffset(0, int(ds.height()));
+        ops.setAngleNumber(As, mods);
+        {
+            // tile shapes timed on a central slab holding 1.0 (error 0 keeps it)
+            const int64_t b = nslab_ / 2, l0 = b * hs_, l1 = std::min(H, l0 + hs_);
+            void *v = store_ == kDevice ? static_cast<void *>(static_cast<char *>(vol_.get()) + size_t(l0) * layer_ * sb_)
+                                        : dslab_[0].get();
+            xpr_proj_args ta = ops.args();
+            ta.vmin[0] = 0; ta.vmax[0] = int32_t(W);
+            ta.vmin[1] = int32_t(l0); ta.vmax[1] = int32_t(l1);
+            ta.vmin[2] = 0; ta.vmax[2] = int32_t(W);
+            for (int i = 0; i < 3; ++i) { ta.origin[i] = ds.orig()[size_t(i)]; ta.voxel[i] = ds.vx(); }
+            tune_shapes(v, st_, sums.get(), err.get(), st_, pds_[0].as<float>(), ta);
+        }
         fill_sums(stream_, sums, p_.xpr.accumulator, 1ull << 22, uint64_t(As * P));
+        // previous-pass events start recorded, so the first pass waits on nothing
+        check(xpr_event_record(compute_done_.e, stream_), "event");
+        check(xpr_event_record(dl_done_.e, copy_back_), "event");
+        // images: every sub-iteration's resident in HBM, preprocessed on the copy
+        // stream ahead of the sub-iteration that first reads them, or one
+        // sub-iteration's at a time on the compute stream
+        Event ready;
+        std::vector<Event> up_img(static_cast<size_t>(resident_ ? S : 0));
+        check(xpr_event_record(ready.e, stream_), "event");
+        check(xpr_stream_wait_event(copy_, ready.e), "wait");
+        auto upload = [&](int64_t s) {
+            ds.upload_sit_images(copy_, s, 0, ds.height(), images[size_t(s)].get(), st_);
+            check(xpr_event_record(up_img[size_t(s)].e, copy_), "event");
+        };
+        if (resident_) upload(0);
+        note("launches_per_pass", Json::number(double(nslab_)));
+        Event e0, e1, fb, fe, bb, be;
         for (int64_t mit = 0; mit < p_.recon.it; ++mit) {
             ops.setWeight(next_weight(mit));
             for (int64_t s = 0; s < S; ++s) {
@@ -761,43 +897,123 @@ public:
                 const bool last = mit == p_.recon.it - 1 && s == S - 1;
                 check(xpr_event_record(e0.e, stream_), "event");
                 ops.setOrigin(next_origin());
-                ds.upload_sit_images(stream_, s, 0, ds.height(), img.get(), st);
-                auto pd = ds.sit_projdata(s);
-                pdb.alloc(pd.size() * 4);
-                check(xpr_upload(stream_, pdb.get(), pd.data(), pd.size() * 4), "upload");
-                check(xpr_stream_sync(stream_), "sync");
-                // views of each slab that can reach the detector rows held
-                std::vector<SlabViews> slab_pd;
-                if (cull_) slab_pd = culled(pd, nslab, Hs);
-                if (!first) pass(false, nslab, Hs, slab_pd, pd, pdb, dslab, dstage, hslab, up, done, ops, last);
-                ops.setAngleNumber(As, p_.detector.module_number);
+                if (resident_) {
+                    if (mit == 0) check(xpr_stream_wait_event(stream_, up_img[size_t(s)].e), "wait");
+                    ops.setImagesBuffer(images[size_t(s)].get(), st_);
+                } else {
+                    ds.upload_sit_images(stream_, s, 0, ds.height(), images[0].get(), st_);
+                    ops.setImagesBuffer(images[0].get(), st_);
+                }
+                if (!first) {
+                    check(xpr_event_record(fb.e, stream_), "event");
+                    pass(false, false, false, s, ops);
+                    check(xpr_event_record(fe.e, stream_), "event");
+                }
+                ops.setAngleNumber(As, mods);
                 ops.setViewIndex(nullptr);
+                ops.setVolumeParameters({0, 0, 0}, {int(W), int(H), int(W)}, ds.vx());
                 ops.error();
-                pass(true, nslab, Hs, slab_pd, pd, pdb, dslab, dstage, hslab, up, done, ops, last,
-                     first);
+                check(xpr_event_record(bb.e, stream_), "event");
+                pass(true, first, last, s, ops);
+                check(xpr_event_record(be.e, stream_), "event");
                 fill_sums(stream_, sums, p_.xpr.accumulator, 0, uint64_t(As * P));
                 check(xpr_event_record(e1.e, stream_), "event");
+                if (resident_ && mit == 0 && s + 1 < S) upload(s + 1);
                 check(xpr_event_sync(e1.e), "sync");
-                float ms = 0;
+                float ms = 0, fms = -1.0f, bms = 0;
                 check(xpr_event_elapsed_ms(e0.e, e1.e, &ms), "elapsed");
-                log_sit(mit, s, ms);
+                if (!first) check(xpr_event_elapsed_ms(fb.e, fe.e, &fms), "elapsed");
+                check(xpr_event_elapsed_ms(bb.e, be.e, &bms), "elapsed");
+                log_sit(mit, s, ms, fms, bms);
             }
         }
+        if (store_ == kDevice) write_final(ds, stream_, vol_.get(), st_, 0, H, stage_);
+        check(xpr_stream_sync(copy_back_), "sync");
+        check(xpr_stream_sync(copy_), "sync");
+        check(xpr_stream_sync(stream_), "sync");
     }
 
 private:
+    enum Store { kDevice, kRam, kZstd };
     int threads_;
     bool cull_;
-    DevBuf cpd_[2], cidx_[2];
+    Store store_ = kZstd;
+    int64_t hs_ = 1;                 // slab height
+    bool resident_ = false;          // every sub-iteration's images in HBM
+    int st_ = XPR_F32;
+    size_t sb_ = 4, layer_ = 0;
+    int64_t nslab_ = 0;
+    // dup_[i]: float32 staging of slab buffer i for 16-bit storage, used in
+    // both directions (the upload's conversion out of it is done before the
+    // slab's kernel, the download's conversion into it after)
+    DevBuf vol_, stage_, dslab_[2], dup_[2];
+    HostBuf hin_[2], hout_[2];
+    std::vector<DevBuf> pds_;
+    // Events of the slab pipelines: up_ (slab uploaded into buffer i), kd_
+    // (slab updated, ready to go back), freed_ (buffer i free again),
+    // compute_done_ / dl_done_ (end of the previous pass on the compute and
+    // the download stream).
+    Event up_[2], kd_[2], freed_[2], compute_done_, dl_done_;
+
+    struct SlabViews {
+        std::vector<float> pd;
+        std::vector<uint32_t> index;
+    };
+    struct SlabDev {
+        DevBuf pd, idx;
+        int64_t n = 0;
+    };
+    std::vector<std::vector<SlabDev>> cviews_;
+
+    // Device bytes of the schedule for a store, slab height and image residency.
+    uint64_t need(Store st, int64_t Hs, bool resident) const
+    {
+        const uint64_t sv = dtype_bytes(storage()), W = uint64_t(ds_->vwidth()), H = uint64_t(ds_->vheight());
+        const uint64_t P = uint64_t(ds_->width() * ds_->height()), As = uint64_t(ds_->concurrent());
+        const uint64_t S = uint64_t(p_.recon.sit), acc = uint64_t(p_.xpr.accumulator / 8);
+        const uint64_t layer = W * W;
+        // the preprocessing's two upload slots of up to 16 MiB (or one
+        // sub-iteration's views), when the projections come from the host
+        const uint64_t staging = ds_->device_io() ? 0 : 2 * std::min(std::max<uint64_t>(16ull << 20, P * 4), As * P * 4);
+        uint64_t m = As * P * (acc + sv) + (resident ? S : 1) * As * P * sv +
+                     ds_->projdata().size() * 4 + staging;
+        if (st == kDevice) m += H * layer * sv;
+        else m += 2 * uint64_t(Hs) * layer * (sv + (sv != 4 ? 4 : 0));
+        return m;
+    }
+
+    // The layer store, slab height and image residency under the device budget.
+    void resolve()
+    {
+        const std::string &ls = p_.xpr.layer_store;
+        const int64_t H = ds_->vheight();
+        if (ls == "device") store_ = kDevice;
+        else if (ls == "ram") store_ = kRam;
+        else if (ls == "zstd") store_ = kZstd;
+        else store_ = need(kDevice, H, false) <= budget() ? kDevice : kZstd;
+        if (p_.xpr.slab_height > 0) {
+            hs_ = std::min<int64_t>(p_.xpr.slab_height, H);
+        } else if (store_ == kDevice) {
+            hs_ = H;                                // one launch per projector pass
+        } else {
+            // the tallest slab whose double buffers fit beside the rest
+            const uint64_t base = need(store_, 0, false), per = need(store_, 1, false) - base;
+            const uint64_t b = budget();
+            hs_ = b > base ? int64_t(std::min<uint64_t>((b - base) / per, uint64_t(H))) : 1;
+            hs_ = std::max<int64_t>(hs_, 1);
+        }
+        resident_ = need(store_, hs_, true) <= budget();
+        p_.xpr.layer_store = store_ == kDevice ? "device" : store_ == kRam ? "ram" : "zstd";
+        if (store_ == kZstd) ds_->ensure_output_dir();
+        note("layer_store", Json::string(p_.xpr.layer_store));
+        note("slab_height", Json::number(double(hs_)));
+        note("images_resident", Json::boolean(resident_));
+    }
 
     // Per slab: the views whose projection of the slab can meet detector rows
     // [vpb, vpt) (method_1c's per-layer view lists, geometry.hpp:141-171).  A
     // dropped view holds no in-bounds voxel of the slab, so the sums and the
     // updates are unchanged; only the work shrinks.
-    struct SlabViews {
-        std::vector<float> pd;
-        std::vector<uint32_t> index;
-    };
     std::vector<SlabViews> culled(const std::vector<float> &pd, int64_t nslab, int64_t Hs)
     {
         auto &ds = *ds_;
@@ -838,91 +1054,126 @@ private:
         return out;
     }
 
-    void pass(bool backward, int64_t nslab, int64_t Hs, const std::vector<SlabViews> &slab_pd,
-              const std::vector<float> &pd, DevBuf &pdb, DevBuf (&dslab)[2], DevBuf (&dstage)[2],
-              HostBuf (&hslab)[2], Event (&up)[2], Event (&done)[2], Operators &ops, bool last,
-              bool first = false)
+    // the kernel arguments of slab b (all views, or the slab's culled views)
+    void set_slab(Operators &ops, int64_t s, int64_t b, void *vol)
+    {
+        const int64_t l0 = b * hs_, l1 = std::min(ds_->vheight(), l0 + hs_);
+        ops.setVolumeBuffer(vol, st_);
+        ops.setVolumeParameters({0, int(l0), 0}, {int(ds_->vwidth()), int(l1), int(ds_->vwidth())},
+                                ds_->vx());
+        if (cull_) {
+            // a slab no view reaches runs with zero views: the forward adds
+            // nothing and the backward leaves the slab unchanged
+            const SlabDev &v = cviews_[size_t(s)][size_t(b)];
+            ops.setProjDataBuffer(v.pd.as<float>());
+            ops.setViewIndex(v.idx.as<uint32_t>());
+            ops.setAngleNumber(v.n, p_.detector.module_number);
+        } else {
+            ops.setProjDataBuffer(pds_[size_t(s)].as<float>());
+            ops.setViewIndex(nullptr);
+            ops.setAngleNumber(ds_->concurrent(), p_.detector.module_number);
+        }
+    }
+
+    void pass(bool backward, bool first, bool last, int64_t s, Operators &ops)
+    {
+        if (store_ == kDevice) {
+            // the layer store is the resident volume: slabs are launched in place
+            for (int64_t b = 0; b < nslab_; ++b) {
+                set_slab(ops, s, b, static_cast<char *>(vol_.get()) + size_t(b * hs_) * layer_ * sb_);
+                if (backward) ops.backward();
+                else ops.forward();
+            }
+            return;
+        }
+        host_pass(backward, first, last, s, ops);
+    }
+
+    // Slabs through two device buffers, three streams: uploads on copy_,
+    // kernels on the compute stream, downloads on copy_back_; the host only
+    // enqueues ("ram": DMA straight from and to the pinned store) or also
+    // decompresses slab b and compresses slab b-1 while the device works
+    // ("zstd").
+    void host_pass(bool backward, bool first, bool last, int64_t s, Operators &ops)
     {
         auto &ds = *ds_;
-        const int st = storage();
-        const int64_t W = ds.vwidth(), H = ds.vheight();
-        const size_t layer = size_t(W * W);
-        (void)pd;
-        // host-side fill of slab b into pinned buffer i
-        auto load = [&](int64_t b, int i) {
-            const int64_t l0 = b * Hs, l1 = std::min(H, l0 + Hs);
-            float *dst = hslab[i].as<float>();
-            parallel_for(l1 - l0, threads_, [&](int64_t k) { ds.load_layer(l0 + k, dst + size_t(k) * layer); });
-        };
-        for (int64_t b = 0; b < nslab; ++b) {
+        const int64_t H = ds.vheight();
+        const bool zstd = store_ == kZstd;
+        // the previous pass's kernels and downloads are done with the slab
+        // buffers and the store
+        check(xpr_stream_wait_event(copy_, compute_done_.e), "wait");
+        check(xpr_stream_wait_event(copy_, dl_done_.e), "wait");
+        auto save = [&](int64_t b) {                     // zstd: slab b back to the spill files
             const int i = int(b & 1);
-            const int64_t l0 = b * Hs, l1 = std::min(H, l0 + Hs);
-            const size_t n = size_t(l1 - l0) * layer;
-            // the previous user of buffer i (slab b-2) has finished with it
-            if (b >= 2) check(xpr_event_sync(done[i].e), "sync");
-            if (backward && first) {
-                check(xpr_fill(copy_, dslab[i].get(), st, 1.0f, n), "fill");
-            } else {
-                load(b, i);
-                if (st == XPR_F32) {
-                    check(xpr_upload(copy_, dslab[i].get(), hslab[i].get(), n * 4), "upload");
-                } else {
-                    check(xpr_upload(copy_, dstage[i].get(), hslab[i].get(), n * 4), "upload");
-                    check(xpr_convert(copy_, dslab[i].get(), st, dstage[i].get(), XPR_F32, n), "convert");
-                }
-            }
-            check(xpr_event_record(up[i].e, copy_), "event");
-            check(xpr_stream_wait_event(stream_, up[i].e), "wait");
-            ops.setVolumeBuffer(dslab[i].get(), st);
-            ops.setVolumeParameters({0, int(l0), 0}, {int(W), int(l1), int(W)}, ds.vx());
-            if (cull_) {
-                const SlabViews &sv = slab_pd[size_t(b)];
-                // (a slab no view reaches runs with zero views: the forward adds
-                // nothing and the backward leaves the slab unchanged)
-                cpd_[i].alloc(std::max<size_t>(sv.pd.size(), 20) * 4);
-                cidx_[i].alloc(std::max<size_t>(sv.index.size(), 1) * 4);
-                if (!sv.index.empty()) {
-                    check(xpr_upload(stream_, cpd_[i].get(), sv.pd.data(), sv.pd.size() * 4), "upload");
-                    check(xpr_upload(stream_, cidx_[i].get(), sv.index.data(), sv.index.size() * 4), "upload");
-                }
-                ops.setProjDataBuffer(cpd_[i].as<float>());
-                ops.setViewIndex(cidx_[i].as<uint32_t>());
-                ops.setAngleNumber(int64_t(sv.index.size()), p_.detector.module_number);
-            } else {
-                ops.setProjDataBuffer(pdb.as<float>());
-                ops.setViewIndex(nullptr);
-                ops.setAngleNumber(ds.concurrent(), p_.detector.module_number);
-            }
-            if (backward) {
-                ops.backward();
-                if (st == XPR_F32) {
-                    check(xpr_download(stream_, hslab[i].get(), dslab[i].get(), n * 4), "download");
-                } else {
-                    check(xpr_convert(stream_, dstage[i].get(), XPR_F32, dslab[i].get(), st, n), "convert");
-                    check(xpr_download(stream_, hslab[i].get(), dstage[i].get(), n * 4), "download");
-                }
-                check(xpr_event_record(done[i].e, stream_), "event");
-                check(xpr_event_sync(done[i].e), "sync");
-                float *src = hslab[i].as<float>();
-                if (last) {
-                    ds.save_final_layers(src, l0, l1);
-                    for (int64_t l = l0; l < l1; ++l) {
-                        if (!ds.ram_layers()) {
-                            std::error_code ec;
-                            fs::remove(numbered_path(p_.recon.output, "layer", l, 5, "zstd"), ec);
-                        }
-                    }
-                } else {
+            const int64_t l0 = b * hs_, l1 = std::min(H, l0 + hs_);
+            check(xpr_event_sync(freed_[i].e), "sync");
+            const float *src = hout_[i].as<float>();
+            parallel_for(l1 - l0, threads_, [&](int64_t k) {
+                ds.save_layer(src + size_t(k) * layer_, l0 + k, false);
+            });
+        };
+        for (int64_t b = 0; b < nslab_; ++b) {
+            const int i = int(b & 1);
+            const int64_t l0 = b * hs_, l1 = std::min(H, l0 + hs_);
+            const size_t n = size_t(l1 - l0) * layer_;
+            const bool fill = backward && first;
+            const float *src = nullptr;
+            if (!fill) {
+                if (zstd) {
+                    // hin_[i] is free once the upload of slab b-2 is done
+                    check(xpr_event_sync(up_[i].e), "sync");
+                    float *dst = hin_[i].as<float>();
                     parallel_for(l1 - l0, threads_, [&](int64_t k) {
-                        ds.save_layer(src + size_t(k) * layer, l0 + k, false);
+                        ds.load_layer(l0 + k, dst + size_t(k) * layer_);
                     });
+                    src = dst;
+                } else {
+                    src = ds.ram_layer(l0);
                 }
-            } else {
-                ops.forward();
-                check(xpr_event_record(done[i].e, stream_), "event");
             }
+            // device buffer i is free once slab b-2 is done with it
+            if (b >= 2) check(xpr_stream_wait_event(copy_, freed_[i].e), "wait");
+            if (fill) {
+                check(xpr_fill(copy_, dslab_[i].get(), st_, 1.0f, n), "fill");
+            } else if (st_ == XPR_F32) {
+                check(xpr_upload(copy_, dslab_[i].get(), src, n * 4), "upload");
+            } else {
+                check(xpr_upload(copy_, dup_[i].get(), src, n * 4), "upload");
+                check(xpr_convert(copy_, dslab_[i].get(), st_, dup_[i].get(), XPR_F32, n), "convert");
+            }
+            check(xpr_event_record(up_[i].e, copy_), "event");
+            check(xpr_stream_wait_event(stream_, up_[i].e), "wait");
+            set_slab(ops, s, b, dslab_[i].get());
+            if (!backward) {
+                ops.forward();
+                check(xpr_event_record(freed_[i].e, stream_), "event");
+                continue;
+            }
+            ops.backward();
+            if (last) {
+                write_final(ds, stream_, dslab_[i].get(), st_, l0, l1, stage_);
+                check(xpr_event_record(freed_[i].e, stream_), "event");
+                if (zstd)
+                    for (int64_t l = l0; l < l1; ++l) {
+                        std::error_code ec;
+                        fs::remove(numbered_path(p_.recon.output, "layer", l, 5, "zstd"), ec);
+                    }
+                continue;
+            }
+            const void *back = dslab_[i].get();
+            if (st_ != XPR_F32) {
+                check(xpr_convert(stream_, dup_[i].get(), XPR_F32, dslab_[i].get(), st_, n), "convert");
+                back = dup_[i].get();
+            }
+            check(xpr_event_record(kd_[i].e, stream_), "event");
+            check(xpr_stream_wait_event(copy_back_, kd_[i].e), "wait");
+            check(xpr_download(copy_back_, zstd ? hout_[i].get() : ds.ram_layer(l0), back, n * 4), "download");
+            check(xpr_event_record(freed_[i].e, copy_back_), "event");
+            if (zstd && b >= 1) save(b - 1);
         }
-        check(xpr_stream_sync(stream_), "sync");
+        if (zstd && backward && !last && nslab_ >= 1) save(nslab_ - 1);
+        check(xpr_event_record(compute_done_.e, stream_), "event");
+        check(xpr_event_record(dl_done_.e, copy_back_), "event");
     }
 };
 
@@ -1062,7 +1313,8 @@ public:
             pds[size_t(s)].alloc(pd.size() * 4);
             check(xpr_upload(stream_, pds[size_t(s)].get(), pd.data(), pd.size() * 4), "upload");
         }
-        Event e0, e1;
+        Event e0, e1, fb, fe, bb, be;
+        note("launches_per_pass", Json::number(1));
         for (const Chunk &c : chunks_) {
             std::cout << "[" << c.vOffset << "-" << c.vOffset + c.vSize << "]... " << std::flush;
             const int64_t l0 = c.vOffset - c.vPadTop, l1 = c.vOffset + c.vSize + c.vPadBot;
@@ -1087,21 +1339,30 @@ public:
                     ds.upload_sit_images(stream_, s, c.iOffset, c.iSize, img.get(), st);
                     ops.setOrigin(next_origin());
                     ops.setProjDataBuffer(pds[size_t(s)].as<float>());
-                    if (s > 0 || mit > 0) ops.forward();
+                    const bool fwd = s > 0 || mit > 0;
+                    if (fwd) {
+                        check(xpr_event_record(fb.e, stream_), "event");
+                        ops.forward();
+                        check(xpr_event_record(fe.e, stream_), "event");
+                    }
                     ops.error();
+                    check(xpr_event_record(bb.e, stream_), "event");
                     ops.backward();
+                    check(xpr_event_record(be.e, stream_), "event");
                     fill_sums(stream_, sums, p_.xpr.accumulator, 0, npx);
                     check(xpr_event_record(e1.e, stream_), "event");
                     check(xpr_event_sync(e1.e), "sync");
-                    float ms = 0;
+                    float ms = 0, fms = -1.0f, bms = 0;
                     check(xpr_event_elapsed_ms(e0.e, e1.e, &ms), "elapsed");
-                    log_sit(mit, s, ms);
+                    if (fwd) check(xpr_event_elapsed_ms(fb.e, fe.e, &fms), "elapsed");
+                    check(xpr_event_elapsed_ms(bb.e, be.e, &bms), "elapsed");
+                    log_sit(mit, s, ms, fms, bms);
                 }
             }
             // keep only the chunk's own layers (reconstruction_chunk.hpp:121-127)
-            std::vector<float> out(nv);
-            download_f32(stream_, out.data(), vol, st, nv, stage);
-            ds.save_final_layers(out.data() + size_t(c.vPadTop) * layer, c.vOffset, c.vOffset + c.vSize);
+            write_final(ds, stream_,
+                        static_cast<const char *>(vol.get()) + size_t(c.vPadTop) * layer * dtype_bytes(st),
+                        st, c.vOffset, c.vOffset + c.vSize, stage);
             std::cout << "Done." << std::endl;
         }
     }
@@ -1124,7 +1385,7 @@ void Reconstruction::forward_exec()
     std::vector<float> vol(layer * size_t(H));
     parallel_for(H, io_threads(p_), [&](int64_t l) {
         float *dst = vol.data() + size_t(l) * layer;
-        ds.load_layer(l, dst);
+        ds.load_final_layer(l, dst);
         for (size_t k = 0; k < layer; ++k) dst[k] /= float(samples);
         for (int64_t j = 0; j < W; ++j)
             for (int64_t i = 0; i < W; ++i)

@@ -113,18 +113,27 @@ public:
     // chunk overlaps the transfer of the other; everything is ordered on s.
     void upload_sit_images(xpr_stream s, int64_t sit, int64_t row0, int64_t rows, void *dst, int dt);
     // The caller's volume buffer of an in-process job ([vheight][vwidth][vwidth]),
-    // or null; after writing it, finish_volume_sink() applies the
-    // de-normalisation of saveLayer(finalize) (data_loader.hpp:135-139).
+    // or null; a device address when device_io().
     float *volume_sink() const { return p_.xpr.mem_volume_out; }
-    void finish_volume_sink();
+    bool device_io() const { return p_.xpr.mem_on_device; }
+    bool normalize() const { return p_.recon.normalize; }
     float norm_min() const { return mn_; }
     float norm_max() const { return mx_; }
 
-    // Layer store between passes (getLayer/saveLayer, data_loader.hpp:118-146).
+    // Layer store between passes (getLayer/saveLayer, data_loader.hpp:118-146):
+    // "ram" is one pinned host block of the whole volume (slabs move by DMA
+    // straight from and to it), "zstd" per-layer spill files.
     void save_layer(const float *data, int64_t layer, bool finalize);
     void load_layer(int64_t layer, float *out) const;
     bool ram_layers() const { return p_.xpr.layer_store == "ram"; }
-    void save_final_layers(const float *vol, int64_t l0, int64_t l1);   // TIFF, de-normalised
+    float *ram_layer(int64_t layer) const;
+    // The output directory for spill files (created once).
+    void ensure_output_dir();
+    // Final layers [l0, l1) of host float data: into the host sink or TIFF
+    // files, de-normalised (saveLayer(finalize), data_loader.hpp:133-146).
+    void save_final_layers(const float *vol, int64_t l0, int64_t l1);
+    // The final volume as written (sink or TIFF layers), for reprojection.
+    void load_final_layer(int64_t layer, float *out) const;
     void save_projection(const float *img, int64_t w, int64_t h, int64_t id) const;
 
 private:
@@ -138,7 +147,7 @@ private:
     std::vector<std::vector<float>> raw_;       // raw projections of TIFF sources
     HostBuf pin_[2];                            // pinned staging slots
     xpr_event slot_done_[2] = {nullptr, nullptr};
-    std::vector<std::vector<float>> ram_store_; // layer store when "ram"
+    HostBuf ram_block_;                         // layer store when "ram" (pinned)
     float mn_ = 0.0f, mx_ = 1.0f;
     std::string proj_dir_;
     bool loaded_ = false;
@@ -213,7 +222,9 @@ protected:
     int storage() const { return p_.xpr.storage; }
     float next_weight(int64_t mit) const;
     void note(const std::string &k, Json v) { record_.set(k, std::move(v)); }
-    void log_sit(int64_t mit, int64_t sit, double ms);
+    // sub-iteration device time, and the time of its forward and backward
+    // projector passes (negative: no such pass)
+    void log_sit(int64_t mit, int64_t sit, double ms, double fwd_ms = -1.0, double bwd_ms = -1.0);
     // Times the projectors' tile shapes (xpr_tune_*) for launches of this
     // shape, once per shape in the process, when xpr.autotune is set.  vol_slab
     // must hold 1.0 (it keeps it: the error used is 0); sums is overwritten.
@@ -222,12 +233,14 @@ protected:
 
     Parameters &p_;
     std::unique_ptr<Dataset> ds_;
-    xpr_stream stream_ = nullptr, copy_ = nullptr;
+    xpr_stream stream_ = nullptr, copy_ = nullptr, copy_back_ = nullptr;
     uint64_t budget_ = 0;
     std::mt19937 rng_;
     Json record_ = Json::object();
     Json jitter_log_ = Json::array();
     Json sit_ms_ = Json::array();
+    Json fwd_ms_ = Json::array();
+    Json bwd_ms_ = Json::array();
 };
 
 std::unique_ptr<Reconstruction> make_reconstruction(Parameters &p);

@@ -374,7 +374,10 @@ struct BatchJob {
             check(xpr_download(stream, out, stage.get(), V * 4), "download");
         }
         check(xpr_stream_sync(stream), "sync");
-        ds->finish_volume_sink();
+        if (ds->normalize()) {                // saveLayer(finalize), data_loader.hpp:135-139
+            const float scale = ds->norm_max() - ds->norm_min();
+            for (uint64_t k = 0; k < V; ++k) out[k] = out[k] * scale;
+        }
     }
 };
 

@@ -47,6 +47,13 @@ __global__ __launch_bounds__(256) void k_fill_raw(T *__restrict__ dst, T value, 
         dst[i] = value;
 }
 
+__global__ __launch_bounds__(256) void k_scale(float *__restrict__ x, uint64_t n, float factor)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        x[i] = x[i] * factor;
+}
+
 template <typename DS, typename SS>
 __global__ __launch_bounds__(256) void k_convert(typename DS::T *__restrict__ dst,
                                                  const typename SS::T *__restrict__ src,

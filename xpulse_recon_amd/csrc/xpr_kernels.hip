@@ -989,6 +989,15 @@ int xpr_q22_to_float(xpr_stream s, const void *sums, int acc_bits, float *out, u
     return XPR_OK;
 }
 
+int xpr_scale(xpr_stream s, float *x, uint64_t n, float factor)
+{
+    if (n == 0) return XPR_OK;
+    if (!x) return fail(XPR_E_INVALID, "null buffer");
+    hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)s, x, n, factor);
+    XPR_LAUNCH_CHECK();
+    return XPR_OK;
+}
+
 int xpr_minmax_init(xpr_stream s, int32_t *minmax)
 {
     if (!minmax) return fail(XPR_E_INVALID, "null buffer");
