@@ -1,23 +1,34 @@
 #!/usr/bin/env python3
 """Benchmark of the iterative cone-beam hot path on one MI355X.
 
-Workload (the reference's plain-FP32-at-scale configuration): 768x768 detector
--> 768^3 grid (vx = px*so/sd), 720 views over 360 deg, sit = 20 -> A_s = 36 views
-per sub-iteration, so=100, sd=1000, px=0.1 mm, rx=2.5 px, one module rolled
-0.3 deg, I0 = 5e4 Poisson, weight 1.0, weight_factor 0.9, method_1a schedule
-with slab height 64 (12 slab launches per projector), device-resident.
+Headline workload (the reference's plain-FP32-at-scale configuration): 768x768
+detector -> 768^3 grid (vx = px*so/sd), 720 views over 360 deg, sit = 20 ->
+A_s = 36 views per sub-iteration, it = 3 (60 sub-iterations), so=100, sd=1000,
+px=0.1 mm, rx=2.5 px, one module rolled 0.3 deg, Poisson I0 = 5e4, weight 1.0,
+weight_factor 0.9, method_1a with slab height 64 (12 slab launches per
+projector pass), FP32 storage.
 
-One step = one steady-state sub-iteration: Forward over the 12 slabs, Error,
-Backward + update over the 12 slabs, sums cleared.  value = voxel-views of
-both projectors (2 * V * A_s) per second of device time.
+One step = one complete reconstruction by the library's engine
+(xpr_reconstruct_dev): -log preprocessing of the 720 raw projections, the
+sub-iterations of method_1a with its layer store in HBM, the volume written
+out.  The projections are resident in HBM before the timed region and the
+volume stays there.  value = voxel-views of both projectors over the
+reconstruction ((2*it*sit - 1) * V * A_s) per second of wall time.
+
+Beside it, in the same run: the same reconstructions with FP16 and BF16
+storage (and a final FP32 sub-iteration), their RMSE against the phantom and
+their speed against plain FP32; p50/p99 time-to-reconstruction of a short
+serving run of mixed sizes; the VALU roofline of the dominant projector; the
+CPU baseline (the oracle on the host's cores).
 
 Synthetic data: seeded phantom of 12 random ellipsoids (512^3 phantom units),
-FP64 analytic line integrals on the GPU, Poisson counts, transmission,
--log preprocessing by the library's preprocess kernel.
+FP64 analytic line integrals on the GPU, Poisson counts, transmission.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
+import hashlib
 import json
 import math
 import os
@@ -30,7 +41,16 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
+# MI355X dense FP32 vector peak 157.3 TFLOP/s counts packed FP32 FMAs (two
+# per lane, two flops each); this build issues unpacked FP32, so its VALU
+# lane-instruction peak is 157.3e12 / 4.
+FP32_PEAK_TFLOPS = 157.3
+VALU_PEAK_PUBLISHED = FP32_PEAK_TFLOPS * 1e12 / 4.0
 METRIC = "voxel-proj updates/s vs plain FP32 at fixed RMSE; p50/p99 recon latency"
+D, NV, SIT, IT, SLAB = 768, 720, 20, 3, 64
+PROFILE = os.path.join("profiles", "pmc_768cube_f32.md")
+KERNEL_SOURCES = [os.path.join("xpulse_recon_amd", "csrc", "kernels", f)
+                  for f in ("common.hpp", "forward.hpp", "backward.hpp")]
 
 
 def log(msg):
@@ -40,24 +60,23 @@ def log(msg):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--dtype", choices=["f32", "f16", "bf16"], default="f32",
-                    help="storage of volume, projections and error (math is FP32)")
-    ap.add_argument("--slab", type=int, default=64)
-    ap.add_argument("--det", type=int, default=768)
-    ap.add_argument("--views", type=int, default=720)
-    ap.add_argument("--sit", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=5, help="timed FP32 reconstructions")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mp-steps", type=int, default=0,
+                    help="timed FP16 and BF16 reconstructions each (default: max(2, steps // 2))")
+    ap.add_argument("--store", choices=["device", "ram", "zstd"], default="device",
+                    help="method_1a layer store of the headline (device: HBM)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-tune", action="store_true",
-                    help="keep the default tile shapes (no xpr_tune_* before the timed steps)")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the short serving run (p50/p99) of the default line")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the FP16/BF16 runs")
     ap.add_argument("--serve", action="store_true",
                     help="serving run instead: Poisson arrivals of reconstructions of "
                          "mixed sizes, p50/p99 time-to-reconstruction")
     ap.add_argument("--recon", action="store_true",
-                    help="complete reconstructions (it=3, sit=20) per storage type through "
+                    help="complete reconstructions (it=1..3, sit=20) per storage type through "
                          "the naive schedule: updates/s and RMSE against the phantom per "
                          "iteration")
     ap.add_argument("--ooc", action="store_true",
@@ -69,23 +88,39 @@ def parse():
                     choices=["method_1a", "method_1b", "method_1c", "method_2"])
     ap.add_argument("--ooc-it", type=int, default=2)
     ap.add_argument("--ooc-sit", type=int, default=16)
-    ap.add_argument("--ooc-slab", type=int, default=64, help="slab height of method_1a/1b/1c")
+    ap.add_argument("--ooc-slab", type=int, default=0,
+                    help="slab height of method_1a/1b/1c (0: from the budget)")
+    ap.add_argument("--ooc-store", choices=["ram", "zstd", "device", "auto"], default="ram",
+                    help="layer store of method_1a/1b/1c")
+    ap.add_argument("--ooc-budget-gb", type=float, default=16.0)
     ap.add_argument("--jobs", type=int, default=24, help="jobs of the serving run")
     ap.add_argument("--rate", type=float, default=4.0, help="mean arrivals per second")
     ap.add_argument("--sizes", default="128,384,768",
                     help="serving: detector sizes the jobs are drawn from")
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--max-concurrent", type=int, default=8)
+    ap.add_argument("--dtype", choices=["f32", "f16", "bf16"], default="f16",
+                    help="serving: storage of the jobs")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one plain 1 GiB float copy (k_convert) so hardware "
                          "byte counters can be scaled against a known size")
     return ap.parse_args()
 
 
+def host_threads():
+    """The host threads this process may use: its CPU affinity, or
+    OMP_NUM_THREADS when that is set lower (the share a job is given)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n), os.cpu_count() or 1
+
+
 def cpu_baseline(pd_sit, images_host, W, H, A_s, orig, vx, target_s):
     """The oracle (C, OpenMP) on a bounded slab of the same sub-iteration."""
     from oracle import oracle as O
-    threads = min(16, os.cpu_count() or 1)
+    threads, visible = host_threads()
     rows = images_host.shape[1]
 
     def run(L):
@@ -106,54 +141,63 @@ def cpu_baseline(pd_sit, images_host, W, H, A_s, orig, vx, target_s):
         t = run(L)
     vv = 2.0 * L * W * W * A_s
     return {"value": vv / t, "unit": "voxel-views/s", "cores": threads, "kind": "port",
+            "host_cpus_visible": visible,
             "sample": f"one sub-iteration (Forward + Error + Backward, {A_s} views) over a "
-                      f"{L}-layer central slab of the {W}x{H}x{W} grid, FP32 canonical oracle, "
-                      f"{t:.1f} s"}
+                      f"{L}-layer central slab of the {W}x{H}x{W} grid, FP32 canonical oracle "
+                      f"(C + OpenMP, {threads} threads), {t:.1f} s"}
 
 
-def measured_traffic(kernel, args):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary of this workload (calibrated against a copy
-    of known size), or None when the summary does not cover it."""
-    if (args.det, args.views, args.sit, args.slab) != (768, 720, 20, 64):
-        return None
-    path = os.path.join(HERE, "profiles", "pmc_traffic_768cube_f32.md")
+def kernel_sha16():
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(os.path.join(HERE, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def profile_counters():
+    """Per-launch hardware counters of the projectors on the headline
+    workload, from the committed rocprofv3 summary: lines
+    "counters <kernel> <SQ_INSTS_VALU> <HBM bytes read> <HBM bytes written> <sha16>",
+    used only when sha16 is that of this build's kernel sources."""
+    out, sha = {}, kernel_sha16()
     try:
-        for line in open(path):
-            parts = line.split()
-            if len(parts) == 3 and parts[0] == "traffic" and parts[1] == kernel:
-                return int(parts[2])
+        with open(os.path.join(HERE, PROFILE)) as f:
+            for line in f:
+                p = line.split()
+                if len(p) == 6 and p[0] == "counters":
+                    if p[5] != sha:
+                        return {}, f"{PROFILE} was made from other kernel sources ({p[5]} != {sha})"
+                    out[p[1]] = {"valu_instr": float(p[2]), "fetch_bytes": float(p[3]),
+                                 "write_bytes": float(p[4])}
     except OSError:
-        return None
-    return None
+        return {}, f"{PROFILE} missing"
+    return out, PROFILE if out else f"{PROFILE} holds no counters"
 
 
-def measured_valu(kernel, args):
-    """VALU figures of `kernel` on this workload from the committed rocprofv3
-    summary: instructions per voxel-view and the VALU-busy fraction
-    (SQ_ACTIVE_INST_VALU * 4 / SIMDs / GRBM_GUI_ACTIVE), or None."""
-    if (args.det, args.views, args.sit, args.slab) != (768, 720, 20, 64):
-        return None
-    path = os.path.join(HERE, "profiles", "kernel_stats_768cube_f32.md")
-    try:
-        for line in open(path):
-            parts = line.split()
-            if len(parts) == 4 and parts[0] == "valu" and parts[1] == kernel:
-                return {"instr_per_voxel_view": float(parts[2]), "busy_frac": float(parts[3])}
-    except OSError:
-        return None
-    return None
+def valu_ceiling():
+    from xpulse_recon_amd import _lib
+    v = C.c_double()
+    _lib.call("xpr_valu_ceiling", None, C.byref(v))
+    return v.value
+
+
+def stats(xs):
+    xs = np.asarray(xs, np.float64)
+    return {"median": round(float(np.median(xs)), 4), "p99": round(float(np.percentile(xs, 99)), 4),
+            "n": int(xs.size)}
 
 
 SERVE_CLASSES = [(128, 180, 128), (384, 360, 256), (768, 720, 512)]   # detector, views, phantom
 
 
-def serve_main(args):
+def serve_run(n_jobs, rate, sizes, storage, budget_gb, max_concurrent):
     """Serving workload: jobs drawn uniformly from three sizes (128^2 detector /
     180 views, 384^2 / 360, 768^2 / 720; grid = detector width), so ~ U[80,150],
     sd ~ U[800,1200] mm, 12-ellipsoid phantoms, I0 = 1e4 Poisson, it=1, sit=10,
     FP16 storage with a final FP32 sub-iteration, admitted by an HBM budget.
-    Arrivals are a Poisson process; latency = completion - arrival."""
+    Arrivals are a Poisson process; latency = completion - arrival (including
+    the transfer of the job's host buffers)."""
     import torch
     from xpulse_recon_amd import synth
     from xpulse_recon_amd.geometry import Geometry
@@ -163,35 +207,34 @@ def serve_main(args):
     rng = np.random.default_rng(2024)
     jobs = []
     t0 = time.perf_counter()
-    classes = [c for c in SERVE_CLASSES if str(c[0]) in args.sizes.split(",")]
-    for j in range(args.jobs):
-        D, N, n_ph = classes[int(rng.integers(0, len(classes)))]
+    classes = [c for c in SERVE_CLASSES if str(c[0]) in sizes.split(",")]
+    for j in range(n_jobs):
+        dd, nv, n_ph = classes[int(rng.integers(0, len(classes)))]
         so, sd = float(rng.uniform(80, 150)), float(rng.uniform(800, 1200))
-        geom = Geometry(dwidth=D, dheight=D, projections=N, so=so, sd=sd, px=0.1)
+        geom = Geometry(dwidth=dd, dheight=dd, projections=nv, so=so, sd=sd, px=0.1)
         _, grid = geom.projdata()
         ell = synth.random_ellipsoids(3000 + j, 12, grid["vwidth"] * grid["vx"], n_ph)
-        proj = np.empty((N, D, D), np.float32)
-        for v0 in range(0, N, 90):
-            views = list(range(v0, min(N, v0 + 90)))
+        proj = np.empty((nv, dd, dd), np.float32)
+        for v0 in range(0, nv, 90):
+            views = list(range(v0, min(nv, v0 + 90)))
             p = synth.line_integrals(geom, views, ell, dev)
             proj[v0:v0 + len(views)] = synth.transmission(p, 1e4, 5000 + j * 8 + v0).cpu().numpy()
             del p
-        jobs.append((geom, proj, D, N))
+        jobs.append((geom, proj, dd, nv))
     torch.cuda.synchronize()
-    gaps = rng.exponential(1.0 / args.rate, args.jobs)
+    gaps = rng.exponential(1.0 / rate, n_jobs)
     arrivals = np.cumsum(gaps) - gaps[0]
-    log(f"{args.jobs} jobs synthesised in {time.perf_counter() - t0:.1f} s "
+    log(f"{n_jobs} jobs synthesised in {time.perf_counter() - t0:.1f} s "
         f"(sizes {[d for (_, _, d, _) in jobs]})")
-    kw = dict(it=1, sit=10, storage=args.dtype if args.dtype != "f32" else "f16",
+    kw = dict(it=1, sit=10, storage=storage if storage != "f32" else "f16",
               final_fp32=True, jitter_seed=7)
-    budget = int(args.budget_gb * 1e9)
-    with Server(budget, args.max_concurrent) as sv:
+    with Server(int(budget_gb * 1e9), max_concurrent) as sv:
         g, p, _, _ = jobs[0]
         sv.wait(sv.submit(g, p, **kw))                 # module load, pools
         log("warm-up job done")
         ids = []
         start = time.perf_counter()
-        for (geom, proj, D, N), t in zip(jobs, arrivals):
+        for (geom, proj, _, _), t in zip(jobs, arrivals):
             dt = start + t - time.perf_counter()
             if dt > 0:
                 time.sleep(dt)
@@ -205,12 +248,11 @@ def serve_main(args):
     makespan = max(r.end_ms for r in res) - t_first
     vv = sum(r.voxel_views for r in res)
     by = {}
-    for (_, _, D, _), r in zip(jobs, res):
-        by.setdefault(D, []).append(r.latency_ms)
-    out = {
-        "metric": METRIC, "mode": "serve", "serve_mode": os.environ.get("XPR_SERVE_MODE", "streams"),
-        "value": vv / (makespan * 1e-3),
-        "unit": "voxel-views/s", "n_gpus": 1, "higher_is_better": True,
+    for (_, _, dd, _), r in zip(jobs, res):
+        by.setdefault(dd, []).append(r.latency_ms)
+    return {
+        "serve_mode": os.environ.get("XPR_SERVE_MODE", "streams"),
+        "voxel_views_per_s": vv / (makespan * 1e-3),
         "p50_ms": round(float(np.percentile(lat, 50)), 1),
         "p99_ms": round(float(np.percentile(lat, 99)), 1),
         "mean_ms": round(float(lat.mean()), 1), "makespan_ms": round(makespan, 1),
@@ -218,15 +260,24 @@ def serve_main(args):
         "queue_ms_mean": round(float(np.mean([r.start_ms - r.submit_ms for r in res])), 1),
         "per_size_p50_ms": {f"{d}^3": round(float(np.percentile(v, 50)), 1) for d, v in sorted(by.items())},
         "per_size_jobs": {f"{d}^3": len(v) for d, v in sorted(by.items())},
-        "streamed_jobs": sum(r.streamed for r in res),
-        "dtype": kw["storage"], "data": "synthetic: 12 random ellipsoids per job, FP64 line "
-                                        "integrals, Poisson I0=1e4",
-        "config": {"workload": f"{args.jobs} jobs, Poisson arrivals {args.rate}/s, sizes "
-                               "{128^3/180 views, 384^3/360, 768^3/720}, so U[80,150], sd "
-                               "U[800,1200], it=1 sit=10, f16 storage + final FP32 "
-                               "sub-iteration", "budget_gb": args.budget_gb,
-                   "max_concurrent": args.max_concurrent},
+        "streamed_jobs": sum(r.streamed for r in res), "jobs": n_jobs,
+        "workload": f"{n_jobs} jobs, Poisson arrivals {rate}/s, sizes drawn from "
+                    f"{{{', '.join(f'{d}^3/{n} views' for d, n, _ in classes)}}}, so U[80,150], "
+                    f"sd U[800,1200], it=1 sit=10, {kw['storage']} storage + final FP32 "
+                    f"sub-iteration, budget {budget_gb} GB, {max_concurrent} slots; latency "
+                    "includes the transfer of the job's host buffers",
     }
+
+
+def serve_main(args):
+    r = serve_run(args.jobs, args.rate, args.sizes, args.dtype, args.budget_gb, args.max_concurrent)
+    out = {"metric": METRIC, "mode": "serve", "value": r.pop("voxel_views_per_s"),
+           "unit": "voxel-views/s", "n_gpus": 1, "higher_is_better": True,
+           "dtype": args.dtype, "data": "synthetic: 12 random ellipsoids per job, FP64 line "
+                                        "integrals, Poisson I0=1e4",
+           "config": {"workload": r.pop("workload"), "budget_gb": args.budget_gb,
+                      "max_concurrent": args.max_concurrent}}
+    out.update(r)
     print(json.dumps(out), flush=True)
 
 
@@ -242,7 +293,7 @@ def recon_main(args):
     from xpulse_recon_amd.serve import Server
 
     dev = torch.device("cuda:0")
-    D, N, S = args.det, args.views, args.sit
+    N, S = NV, SIT
     geom = Geometry(dwidth=D, dheight=D, projections=N, so=100.0, sd=1000.0, px=0.1, rx=2.5,
                     roll=0.3)
     _, grid = geom.projdata()
@@ -293,6 +344,28 @@ def recon_main(args):
     print(json.dumps(out), flush=True)
 
 
+def pcie_bandwidth(nbytes=1 << 30):
+    """Host-to-device and device-to-host copy rates of pinned memory (best of
+    three 1 GiB copies each, timed with events)."""
+    import torch
+    h = torch.empty(nbytes // 4, dtype=torch.float32, pin_memory=True)
+    d = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    out = {}
+    for name, fn in (("h2d_GBps", lambda: d.copy_(h, non_blocking=True)),
+                     ("d2h_GBps", lambda: h.copy_(d, non_blocking=True))):
+        best = 1e30
+        for _ in range(3):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+        out[name] = round(nbytes / (best * 1e-3) / 1e9, 2)
+    del h, d
+    return out
+
+
 def ooc_main(args):
     """Out-of-core reconstruction: 2048^2 detector -> 2048^3 grid (32 GiB FP32),
     2048 views (32 GiB), 40 random ellipsoids + 200 spheres of radius 2-6 voxels
@@ -329,10 +402,10 @@ def ooc_main(args):
                         "weight_factor": 0.9},
               "geometry": {"so": 100.0, "sd": 1000.0},
               "detector": {"px": 0.1},
-              "xpr": {"device_budget_gb": 16.0, "jitter_seed": 7, "layer_store": "ram",
-                      "slab_height": args.ooc_slab, "quiet": True}}
+              "xpr": {"device_budget_gb": args.ooc_budget_gb, "jitter_seed": 7,
+                      "layer_store": args.ooc_store, "slab_height": args.ooc_slab, "quiet": True}}
     rec = C.create_string_buffer(1 << 20)
-    log(f"reconstruction ({method}, 16 GB budget) ...")
+    log(f"reconstruction ({method}, {args.ooc_budget_gb} GB budget) ...")
     t0 = time.perf_counter()
     rc = _lib.load().xpr_reconstruct_mem(json.dumps(params).encode(), proj.ctypes.data_as(C.c_void_p),
                                          D, D, N, vol.ctypes.data_as(C.c_void_p), rec, len(rec))
@@ -358,19 +431,257 @@ def ooc_main(args):
         n += d.numel()
         del truth, d
     finite = bool(np.isfinite(vol).all())
+    # the PCIe bound of a slab schedule with a host store: every forward pass
+    # uploads the volume, every backward pass uploads it (but the first) and
+    # downloads it, and the images go up per sub-iteration or once
+    pcie = None
+    if method != "method_2" and record.get("layer_store") in ("ram", "zstd"):
+        bw = pcie_bandwidth()
+        V4 = W * W * H * 4
+        n = IT * S
+        img = (1 if record.get("images_resident") else IT) * S * A_s * D * D * 4
+        h2d = (2 * n - 2) * V4 + img
+        d2h = n * V4
+        bound = max(h2d / bw["h2d_GBps"], d2h / bw["d2h_GBps"]) / 1e9
+        pcie = dict(bw, h2d_bytes=h2d, d2h_bytes=d2h, bound_s=round(bound, 3),
+                    wall_over_bound=round(wall / bound, 3),
+                    slab_height=record.get("slab_height"), layer_store=record.get("layer_store"))
+        log(f"PCIe bound {bound:.2f} s (h2d {h2d / 1e9:.1f} GB at {bw['h2d_GBps']:.1f} GB/s, d2h "
+            f"{d2h / 1e9:.1f} GB at {bw['d2h_GBps']:.1f} GB/s); wall {wall:.2f} s")
     out = {"metric": METRIC, "mode": "out-of-core", "value": vv / wall, "unit": "voxel-views/s",
            "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "wall_s": round(wall, 1),
            "device_s": round(dev_ms / 1e3, 1), "voxel_views": vv,
-           "chunks": record.get("chunks"), "rmse_vs_phantom": (sq / n) ** 0.5,
+           "chunks": record.get("chunks"), "rmse_vs_phantom": (sq / n) ** 0.5, "pcie": pcie,
            "maxabs_vs_phantom": mx, "finite": finite,
            "data": "synthetic: 40 random ellipsoids + 200 spheres r 2-6 voxels mu 0.05/voxel, "
                    "FP64 line integrals, Poisson I0=5e4",
            "config": {"workload": f"{W}^2x{H} grid, {N} views of {D}x{D}, it={IT} sit={S}, "
-                                  f"{method}" + (f" slab {args.ooc_slab}" if method != "method_2"
+                                  f"{method}" + (f" slab {record.get('slab_height')} store {record.get('layer_store')}" if method != "method_2"
                                                  else "") +
-                                  ", device budget 16 GB, volume and projections in "
+                                  f", device budget {args.ooc_budget_gb} GB, volume and projections in "
                                   "host memory (value = nominal voxel-views / wall time)"}}
     print(json.dumps(out), flush=True)
+
+
+
+
+def headline(args):
+    """Complete reconstructions of the at-scale workload through the engine."""
+    import torch
+    import xpulse_recon_amd as X
+    from xpulse_recon_amd import _lib, synth
+    from xpulse_recon_amd.geometry import Geometry
+
+    dev = torch.device("cuda:0")
+    geom = Geometry(dwidth=D, dheight=D, projections=NV, so=100.0, sd=1000.0, px=0.1, rx=2.5,
+                    roll=0.3)
+    pd_all, grid = geom.projdata()
+    W, H, vx = grid["vwidth"], grid["vheight"], grid["vx"]
+    A_s = NV // SIT
+    V = W * W * H
+    vv_recon = (2 * IT * SIT - 1) * V * A_s
+    log(f"grid {W}x{H}x{W}, {NV} views, sit {SIT}, it {IT}, {A_s} views per sub-iteration")
+
+    # ---- synthetic projections, resident in HBM (FP64 chords -> Poisson) ----
+    t0 = time.perf_counter()
+    ell = synth.random_ellipsoids(1002, 12, W * vx, 512)
+    proj = torch.empty((NV, D, D), dtype=torch.float32, device=dev)
+    for v0 in range(0, NV, 60):
+        views = list(range(v0, min(NV, v0 + 60)))
+        p = synth.line_integrals(geom, views, ell, dev)
+        proj[v0:v0 + len(views)] = synth.transmission(p, 5e4, 1003 + v0)
+        del p
+    truth = synth.phantom(grid, ell, dev) * float(vx)
+    vol = torch.empty((H, W, W), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    log(f"synthetic data ready in {time.perf_counter() - t0:.1f} s")
+    lib = _lib.load()
+
+    def params(storage):
+        return {"recon": {"method": "method_1a", "it": IT, "sit": SIT, "mlog": True,
+                          "weight": 1.0, "weight_factor": 0.9},
+                "geometry": {"so": 100.0, "sd": 1000.0},
+                "detector": {"px": 0.1, "rx": 2.5, "roll": 0.3},
+                "xpr": {"storage": storage, "final_fp32": True, "layer_store": args.store,
+                        "slab_height": SLAB, "jitter_seed": 1004, "quiet": True}}
+
+    seen = []                    # every run record (tuning is noted by the first)
+
+    def run(storage, steps, warmup):
+        text = json.dumps(params(storage)).encode()
+        rec = C.create_string_buffer(1 << 20)
+
+        def one():
+            w0 = time.perf_counter()
+            rc = lib.xpr_reconstruct_dev(text, C.c_void_p(proj.data_ptr()), D, D, NV,
+                                         C.c_void_p(vol.data_ptr()), rec, len(rec))
+            w1 = time.perf_counter()
+            if rc != 0:
+                raise SystemExit(f"xpr_reconstruct_dev failed ({rc}): "
+                                 f"{lib.xpr_last_error().decode()}")
+            return (w1 - w0) * 1e3, json.loads(rec.value.decode())
+
+        for _ in range(warmup):
+            seen.append(one()[1])
+        torch.cuda.synchronize()
+        walls, recs = [], []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            w, r = one()
+            walls.append(w)
+            recs.append(r)
+            seen.append(r)
+        torch.cuda.synchronize()
+        total = time.perf_counter() - t0
+        if not torch.isfinite(vol).all():
+            raise SystemExit(f"non-finite volume ({storage})")
+        d = vol.double() - truth.double()
+        rmse = float(torch.sqrt((d * d).mean()))
+        maxabs = float(d.abs().max())
+        del d
+        dev_ms = [sum(r["subiteration_ms"]) for r in recs]
+        log(f"{storage}: {total / steps * 1e3:.1f} ms per reconstruction "
+            f"({np.mean(dev_ms):.1f} ms device), RMSE vs phantom {rmse:.4e}")
+        return {"total_s": total, "walls": walls, "recs": recs, "rmse": rmse, "maxabs": maxabs,
+                "dev_ms": dev_ms}
+
+    f32 = run("f32", args.steps, args.warmup)
+    value = vv_recon * args.steps / f32["total_s"]
+    ms = f32["total_s"] / args.steps * 1e3
+    rec0 = f32["recs"][-1]
+    L = int(rec0["launches_per_pass"])
+
+    # ---- roofline of the dominant projector: its launches timed by the
+    # engine with HIP events on the stream they run on, the VALU ceiling
+    # measured now, instruction and byte counters from the committed profile
+    fwd = np.concatenate([np.asarray(r["forward_pass_ms"]) for r in f32["recs"]]) / L
+    bwd = np.concatenate([np.asarray(r["backward_pass_ms"]) for r in f32["recs"]]) / L
+    kernels = {"k_forward_bin": fwd, "k_backward": bwd}
+    name = max(kernels, key=lambda k: float(kernels[k].sum()))
+    launch_ms = float(kernels[name].mean())
+    ceiling = valu_ceiling()
+    counters, source = profile_counters()
+    vv_launch = W * W * SLAB * A_s
+    c = counters.get(name)
+    roof = {"bound": "valu", "unit": "lane-instr/s", "kernel": name,
+            "launch_ms": round(launch_ms, 4), "peak": ceiling,
+            "peak_note": "VALU lane-instructions per second of xpr_valu_ceiling (16 chains of "
+                         "v_fma_f32, v_mul_f32 or v_add_f32 per lane, 8 waves per SIMD, best "
+                         "kind) measured in this run; achieved = SQ_INSTS_VALU per launch "
+                         "(committed profile of these kernel sources) x 64 / launch time",
+            "peak_published_derived": VALU_PEAK_PUBLISHED,
+            "counters": source}
+    if c:
+        ach = c["valu_instr"] * 64.0 / (launch_ms * 1e-3)
+        roof.update({"achieved": ach, "frac": round(ach / ceiling, 4),
+                     "frac_of_published_derived": round(ach / VALU_PEAK_PUBLISHED, 4),
+                     "valu_instr_per_launch": c["valu_instr"],
+                     "valu_instr_per_voxel_view": round(c["valu_instr"] * 64.0 / vv_launch, 3),
+                     "traffic": int(c["fetch_bytes"] + c["write_bytes"])})
+        if roof["frac"] > 1.0:
+            roof["invalid"] = "achieved above the measured ceiling"
+    else:
+        roof.update({"achieved": None, "frac": None, "traffic": None})
+    hbm = {}
+    alg = {}
+    for k, xs in kernels.items():
+        t = float(xs.mean()) * 1e-3
+        cc = counters.get(k)
+        if cc:
+            by = cc["fetch_bytes"] + cc["write_bytes"]
+            hbm[k] = {"bytes_per_launch": int(by), "GBps": round(by / t / 1e9, 1),
+                      "frac_of_8TBps": round(by / t / 1e9 / HBM_PEAK_GBS, 4),
+                      "valu_frac": round(cc["valu_instr"] * 64.0 / t / ceiling, 4)}
+        alg[k] = {"TFLOPs": round(27.0 * vv_launch / t / 1e12, 2),
+                  "frac_of_157_3": round(27.0 * vv_launch / t / 1e12 / FP32_PEAK_TFLOPS, 4)}
+    roof["other"] = {"measured_hbm": hbm, "algorithmic_flops_27_per_voxel_view": alg}
+
+    # ---- mixed precision (FP16 / BF16 storage, final FP32 sub-iteration) ----
+    mixed = {}
+    if not args.no_mixed:
+        k2 = args.mp_steps or max(2, args.steps // 2)
+        for st in ("f16", "bf16"):
+            r = run(st, k2, 1)
+            v = vv_recon * k2 / r["total_s"]
+            mixed[st] = {"updates_per_s": v, "speedup": round(v / value, 4),
+                         "ms_per_reconstruction": round(r["total_s"] / k2 * 1e3, 2),
+                         "device_ms_per_reconstruction": round(float(np.mean(r["dev_ms"])), 2),
+                         "rmse_vs_phantom": r["rmse"], "maxabs_vs_phantom": r["maxabs"],
+                         "rmse_ratio": round(r["rmse"] / f32["rmse"], 5), "steps": k2}
+
+    # ---- secondary figures ----------------------------------------------------
+    subit = [x for r in f32["recs"] for x in r["subiteration_ms"][1:]]
+    secondary = {
+        "reconstruction_ms": stats(f32["walls"]),
+        "device_ms_per_reconstruction": round(float(np.mean(f32["dev_ms"])), 2),
+        "device_voxel_views_per_s": vv_recon / (float(np.mean(f32["dev_ms"])) * 1e-3),
+        "sub_iteration_ms": stats(subit),
+        "launch_ms": {k: stats(xs) for k, xs in kernels.items()},
+        "rmse_vs_phantom": f32["rmse"], "maxabs_vs_phantom": f32["maxabs"],
+        "compulsory_hbm": {
+            "k_forward_bin": {"bytes_per_launch": W * W * SLAB * 4 + 2 * A_s * D * D * 8,
+                              "frac": round((W * W * SLAB * 4 + 2 * A_s * D * D * 8) /
+                                            (float(fwd.mean()) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+            "k_backward": {"bytes_per_launch": 2 * W * W * SLAB * 4 + A_s * D * D * 4,
+                           "frac": round((2 * W * W * SLAB * 4 + A_s * D * D * 4) /
+                                         (float(bwd.mean()) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}},
+        "flops_per_voxel_view": {"algorithmic": 27,
+                                 "note": "3 dot4 as 12 FMA + rcp + 2 mul + sub; the kernels "
+                                         "hoist the (y,z,w) part of each dot per row and share W "
+                                         "and its reciprocal over 4 layers when the w row terms "
+                                         "agree (this workload)"},
+        "kernel_sources_sha16": kernel_sha16(),
+    }
+    if not args.no_serving:
+        log("serving run ...")
+        secondary["serving"] = serve_run(16, 4.0, "128,384,768", "f16", 200.0, 8)
+
+    if args.calibrate:
+        n = 1 << 28                      # 1 GiB read + 1 GiB written
+        a = torch.empty(n, dtype=torch.float32, device=dev)
+        b = torch.empty(n, dtype=torch.float32, device=dev)
+        X.fill(a, 1.0)
+        X.convert(b, a)
+        torch.cuda.synchronize()
+        del a, b
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle) ...")
+        views = list(range(0, NV, SIT))[:A_s]
+        img = torch.empty((A_s, D, D), dtype=torch.float32, device=dev)
+        X.preprocess(proj[views].contiguous(), img, normalize=False, mlog=True)
+        orig = [float(np.float32(grid["orig"][i])) for i in range(3)]
+        cpu = cpu_baseline(geom.sit_projdata(pd_all, SIT, 0), img.cpu().numpy(), W, H, A_s, orig,
+                           float(np.float32(vx)), args.cpu_seconds)
+        log(f"cpu baseline {cpu['value']:.3e} voxel-views/s on {cpu['cores']} threads")
+
+    tuned = {k: r[k] for r in seen
+             for k in ("tuned_forward_tile_cap", "tuned_backward_voxels_per_lane") if k in r}
+    return {
+        "metric": METRIC, "value": value, "unit": "voxel-views/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "vs_plain_fp32": mixed or None,
+        "p50_ms": secondary.get("serving", {}).get("p50_ms"),
+        "p99_ms": secondary.get("serving", {}).get("p99_ms"),
+        "dtype": "f32",
+        "data": "synthetic: 12 random ellipsoids (seed 1002, 512^3 phantom units), FP64 analytic "
+                "line integrals, Poisson I0=5e4, raw transmission resident in HBM",
+        "config": {"workload": f"{W}^3 grid from a {D}x{D} detector, {NV} views over 360 deg, "
+                               f"sit={SIT} (A_s={A_s}), it={IT}, rx 2.5 px, roll 0.3 deg, "
+                               f"method_1a slab {SLAB} with the layer store in "
+                               f"{rec0['layer_store']}, FP32 storage; one step = one complete "
+                               "reconstruction (preprocessing, 60 sub-iterations, volume out) "
+                               "through xpr_reconstruct_dev",
+                   "grid": [W, H, W], "views": NV, "sit": SIT, "it": IT,
+                   "views_per_subiteration": A_s, "method": "method_1a", "slab": SLAB,
+                   "layer_store": rec0["layer_store"], "images_resident": rec0["images_resident"],
+                   "voxel_views_per_reconstruction": vv_recon, "tile_shapes": tuned or "defaults"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "secondary": secondary,
+    }
 
 
 def main():
@@ -389,207 +700,7 @@ def main():
         return recon_main(args)
     if args.ooc:
         return ooc_main(args)
-    import xpulse_recon_amd as X
-    from xpulse_recon_amd import synth
-    from xpulse_recon_amd.geometry import Geometry
-
-    dev = torch.device("cuda:0")
-    tdt = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16}[args.dtype]
-    sbytes = 4 if args.dtype == "f32" else 2
-    D, N, S = args.det, args.views, args.sit
-    geom = Geometry(dwidth=D, dheight=D, projections=N, so=100.0, sd=1000.0, px=0.1, rx=2.5,
-                    roll=0.3)
-    pd_all, grid = geom.projdata(fp64=False)
-    W, H, vx = grid["vwidth"], grid["vheight"], grid["vx"]
-    A_s = N // S
-    V = W * W * H
-    log(f"grid {W}x{H}x{W}, {A_s} views/sub-iteration, slab {args.slab}, storage {args.dtype}")
-
-    # ---- synthetic projections (FP64 chords -> Poisson -> -log) ------------
-    t0 = time.perf_counter()
-    fov = W * vx
-    ell = synth.random_ellipsoids(1002, 12, fov, 512)
-    images = []
-    for s in range(S):
-        views = list(range(s, N, S))[:A_s]
-        p = synth.line_integrals(geom, views, ell, dev)
-        tr = synth.transmission(p, 5e4, 1003 + s)
-        del p
-        img = torch.empty(tr.shape, dtype=tdt, device=dev)
-        X.preprocess(tr, img, normalize=False, mlog=True)
-        images.append(img)
-        del tr
-    pds = [torch.from_numpy(geom.sit_projdata(pd_all, S, s)).to(dev) for s in range(S)]
-    torch.cuda.synchronize()
-    log(f"synthetic data ready in {time.perf_counter() - t0:.1f} s")
-
-    vol = torch.empty((H, W, W), dtype=tdt, device=dev)
-    X.fill(vol, 1.0)
-    sums = torch.zeros((A_s, D, D), dtype=torch.int64, device=dev)
-    err = torch.empty((A_s, D, D), dtype=tdt, device=dev)
-    slabs = [(l0, min(H, l0 + args.slab)) for l0 in range(0, H, args.slab)]
-    jit = np.random.default_rng(1004).uniform(0.0, vx, (args.warmup + args.steps, 3))
-    weight = 1.0
-
-    def args_for(l0, l1, origin):
-        return X.proj_args(D, A_s, 1, (0, l0, 0), (W, l1, W), 0, D, origin, (vx, vx, vx))
-
-    def step(k, ev=None, lev=None):
-        s = k % S
-        origin = [np.float32(np.float32(grid["orig"][i]) + np.float32(jit[k][i])) for i in range(3)]
-        sl = [args_for(l0, l1, origin) for (l0, l1) in slabs]
-        if ev: ev[0].record()
-        for i, ((l0, l1), a) in enumerate(zip(slabs, sl)):
-            X.forward(vol[l0:l1], sums, pds[s], a)
-            if lev: lev[0][i].record()
-        if ev: ev[1].record()
-        X.error(images[s], sums, err, weight)
-        if ev: ev[2].record()
-        for i, ((l0, l1), a) in enumerate(zip(slabs, sl)):
-            X.backward(vol[l0:l1], err, pds[s], a)
-            if lev: lev[1][i].record()
-        if ev: ev[3].record()
-        X.fill_u64(sums, 0)
-        if ev: ev[4].record()
-
-    tuned = None
-    if not args.no_tune:
-        # the library's tile-shape tuning, on scratch copies of the first slab's buffers
-        a0 = args_for(slabs[0][0], slabs[0][1], [np.float32(grid["orig"][i]) for i in range(3)])
-        cap = X.tune_forward(vol[slabs[0][0]:slabs[0][1]].clone(), torch.zeros_like(sums), pds[0], a0)
-        X.error(images[0], sums, err, weight)
-        jb = X.tune_backward(vol[slabs[0][0]:slabs[0][1]].clone(), err, pds[0], a0)
-        sums.zero_()
-        tuned = {"forward_tile_cap": cap, "backward_voxels_per_lane": jb}
-        log(f"tuned tile shapes: {tuned}")
-
-    for k in range(args.warmup):
-        step(k)
-    torch.cuda.synchronize()
-    log("warmup done")
-
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(args.steps)]
-    # per-launch events: each slab launch ends at its own event (the launches
-    # are back to back on one stream, so a launch runs from the previous event)
-    levs = [[[torch.cuda.Event(enable_timing=True) for _ in slabs] for _ in range(2)]
-            for _ in range(args.steps)]
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    start.record()
-    for k in range(args.steps):
-        step(args.warmup + k, evs[k], levs[k])
-    stop.record()
-    torch.cuda.synchronize()
-    total_ms = start.elapsed_time(stop)
-    fp_ms = sum(e[0].elapsed_time(e[1]) for e in evs) / args.steps
-    er_ms = sum(e[1].elapsed_time(e[2]) for e in evs) / args.steps
-    bp_ms = sum(e[2].elapsed_time(e[3]) for e in evs) / args.steps
-    fl_ms = sum(e[3].elapsed_time(e[4]) for e in evs) / args.steps
-    ms = total_ms / args.steps
-    if not torch.isfinite(vol.float()).all():
-        raise SystemExit("non-finite volume after the timed steps")
-    log(f"per step {ms:.2f} ms: forward {fp_ms:.2f}, error {er_ms:.2f}, backward {bp_ms:.2f}, "
-        f"clear {fl_ms:.2f}")
-
-    vv_step = 2.0 * V * A_s
-    value = vv_step / (ms * 1e-3)
-    nsl = len(slabs)
-    v_slab = W * W * args.slab
-    fp_bytes = v_slab * A_s * 4 + v_slab * sbytes
-    bp_bytes = v_slab * A_s * sbytes + 2 * v_slab * sbytes
-    fp_launch, bp_launch = fp_ms / nsl, bp_ms / nsl
-    if bp_ms >= fp_ms:
-        name, launch_ms, nbytes = "k_backward", bp_launch, bp_bytes
-    else:
-        name, launch_ms, nbytes = "k_forward_bin", fp_launch, fp_bytes
-    ach = nbytes / (launch_ms * 1e-3) / 1e9
-    traffic = measured_traffic(name, args) if args.dtype == "f32" else None
-    roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": name,
-            "launch_ms": round(launch_ms, 4),
-            "algorithmic_bytes_per_launch": int(nbytes),
-            "other": {"forward_launch_ms": round(fp_launch, 4),
-                      "forward_frac": round(fp_bytes / (fp_launch * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                      "backward_launch_ms": round(bp_launch, 4),
-                      "backward_frac": round(bp_bytes / (bp_launch * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                      "error_ms": round(er_ms, 4)}}
-
-    # secondary figures: per-launch median / p99, compulsory HBM traffic
-    # (volume slab and detector stack only), VALU issue from the committed
-    # rocprofv3 counters, flops per voxel-view
-    launches = {"k_forward_bin": [], "k_backward": []}
-    for k in range(args.steps):
-        e, le = evs[k], levs[k]
-        prev = e[0]
-        for ev_ in le[0]:
-            launches["k_forward_bin"].append(prev.elapsed_time(ev_))
-            prev = ev_
-        prev = e[2]
-        for ev_ in le[1]:
-            launches["k_backward"].append(prev.elapsed_time(ev_))
-            prev = ev_
-    steps_ms = np.array([e[0].elapsed_time(e[4]) for e in evs])
-    secondary_steps = {"median": round(float(np.median(steps_ms)), 4),
-                       "p99": round(float(np.percentile(steps_ms, 99)), 4),
-                       "n": int(steps_ms.size)}
-    acc_bytes = 8
-    comp = {"k_forward_bin": v_slab * sbytes + 2 * A_s * D * D * acc_bytes,
-            "k_backward": 2 * v_slab * sbytes + A_s * D * D * sbytes}
-    secondary = {"sub_iteration_ms": secondary_steps, "launch_ms": {}, "compulsory_hbm": {},
-                 "valu": {},
-                 "flops_per_voxel_view": {"algorithmic": 27, "executed": 7.75,
-                                          "note": "3 dot4 as 12 FMA + rcp + 2 mul + sub; the "
-                                                  "kernels hoist the (y,z,w) part of each dot "
-                                                  "per row and share W and its reciprocal over "
-                                                  "4 layers when the w row terms agree (this "
-                                                  "workload): 2 FMA + 2 mul per voxel-view, "
-                                                  "FMA + rcp + Newton (2 FMA) per 4"}}
-    for kname, xs in launches.items():
-        xs = np.array(xs)
-        med = float(np.median(xs))
-        secondary["launch_ms"][kname] = {"median": round(med, 4),
-                                         "p99": round(float(np.percentile(xs, 99)), 4),
-                                         "n": int(xs.size)}
-        secondary["compulsory_hbm"][kname] = {
-            "bytes_per_launch": int(comp[kname]),
-            "frac": round(comp[kname] / (med * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-        secondary["valu"][kname] = measured_valu(kname, args) if args.dtype == "f32" else None
-
-    if args.calibrate:
-        n = 1 << 28                      # 1 GiB read + 1 GiB written
-        a = torch.empty(n, dtype=torch.float32, device=dev)
-        b = torch.empty(n, dtype=torch.float32, device=dev)
-        X.fill(a, 1.0)
-        X.convert(b, a)
-        torch.cuda.synchronize()
-        del a, b
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        log("cpu baseline (oracle) ...")
-        img_host = images[0].float().cpu().numpy()
-        orig = [float(np.float32(grid["orig"][i])) for i in range(3)]
-        cpu = cpu_baseline(pds[0].cpu().numpy(), img_host, W, H, A_s, orig, float(np.float32(vx)),
-                           args.cpu_seconds)
-        log(f"cpu baseline {cpu['value']:.3e} voxel-views/s on {cpu['cores']} threads")
-
-    out = {
-        "metric": METRIC, "value": value, "unit": "voxel-views/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": args.dtype,
-        "data": "synthetic: 12 random ellipsoids (seed 1002), FP64 analytic line integrals, "
-                "Poisson I0=5e4, -log preprocessed on the GPU",
-        "config": {"workload": f"{W}^3 grid from a {D}x{D} detector, {N} views, sit={S} "
-                               f"(A_s={A_s}), method_1a slab {args.slab} device-resident, one "
-                               f"sub-iteration per step, {args.dtype} storage",
-                   "grid": [W, H, W], "views_per_subiteration": A_s, "slab": args.slab,
-                   "voxel_views_per_step": vv_step, "tile_shapes": tuned or "defaults"},
-        "roofline": roof,
-        "cpu_baseline": cpu,
-        "secondary": secondary,
-    }
-    print(json.dumps(out), flush=True)
+    print(json.dumps(headline(args)), flush=True)
 
 
 if __name__ == "__main__":

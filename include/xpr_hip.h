@@ -226,6 +226,15 @@ int xpr_tune_reset(void);
  * (where the kernels use the IEEE division).  Synchronous. */
 int xpr_check_fast_paths(xpr_stream s, uint64_t mismatches[3]);
 
+/* No counterpart in the reference: the device's VALU issue ceiling, the
+ * roofline of the projectors (VALU-bound: a dozen vector instructions per
+ * voxel-view against well under a byte of HBM traffic).  Kernels of 16
+ * independent v_fma_f32, v_mul_f32 or v_add_f32 chains per lane, one VGPR
+ * read per instruction, at 8 waves per SIMD on every CU; returns vector
+ * lane-instructions per second, the best of five launches of each kind
+ * after a warm-up.  Synchronous; allocates a small output buffer. */
+int xpr_valu_ceiling(xpr_stream s, double *lane_instr_per_s);
+
 /* ---- geometry (Geometry, dataset/geometry.hpp:35-76, 182-328) ----------- */
 
 /* JSON "geometry" and "detector" sections (global_parameters.hpp:23-54).

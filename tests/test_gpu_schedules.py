@@ -127,6 +127,23 @@ def test_streaming_schedules_equal_naive(scan, method, xpr):
     assert not list((t / name).glob("*.zstd"))          # spill removed at the end
 
 
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_slab_final_fp32_equals_naive(scan, storage):
+    """16-bit storage with the slab schedule's store in HBM: its final
+    sub-iteration in float32 storage equals the naive schedule's bit for bit."""
+    t, spec, d = scan
+    naive, ln = _run(t, f"naive_f32last_{storage}", d, method="naive",
+                     xpr={"storage": storage, "final_fp32": True})
+    vol, log = _run(t, f"slab_f32last_{storage}", d, method="method_1a",
+                    xpr={"storage": storage, "final_fp32": True, "layer_store": "device",
+                         "slab_height": 7})
+    assert log["final_fp32"] is True
+    np.testing.assert_array_equal(vol, naive)
+    plain, _ = _run(t, f"naive_no32_{storage}", d, method="naive",
+                    xpr={"storage": storage, "final_fp32": False})
+    assert not np.array_equal(vol, plain)
+
+
 def test_auto_layer_store(scan):
     """layer_store auto: HBM when the whole volume fits the budget (one launch
     per pass when no slab height is given), else zstd spill files."""

@@ -143,6 +143,7 @@ _SIGS = {
     "xpr_tune_reset": (C.c_int, []),
     "xpr_check_fast_paths": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "xpr_q22_to_float": (C.c_int, [_P, _P, C.c_int, _P, C.c_uint64]),
+    "xpr_valu_ceiling": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "xpr_scale": (C.c_int, [_P, _P, C.c_uint64, C.c_float]),
     "xpr_minmax_init": (C.c_int, [_P, _P]),
     "xpr_minmax_normal": (C.c_int, [_P, _P, C.c_uint64, _P]),

@@ -840,6 +840,16 @@ public:
                 }
             }
         }
+        // 16-bit storage with the volume in HBM: the last sub-iteration in
+        // float32 storage (volume converted, images and error in float32)
+        const bool final32 = final32_;
+        DevBuf vol32, img32, err32;
+        Event up32;
+        if (final32) {
+            vol32.alloc(uint64_t(H) * layer_ * 4);
+            img32.alloc(uint64_t(As * P) * 4);
+            err32.alloc(uint64_t(As * P) * 4);
+        }
         if (store_ == kDevice) {
             vol_.alloc(uint64_t(H) * layer_ * sb_);
             check(xpr_fill(stream_, vol_.get(), st_, 1.0f, uint64_t(H) * layer_), "fill");
@@ -854,6 +864,9 @@ public:
             }
             check(xpr_fill(stream_, dslab_[0].get(), st_, 1.0f, slab_n), "fill");
         }
+        cvol_ = vol_.get();
+        cst_ = st_;
+        csb_ = sb_;
         Operators ops(stream_, p_.xpr.accumulator);
         ops.setSumImagesBuffer(sums.get());
         ops.setErrorBuffer(err.get(), st_);
@@ -886,6 +899,10 @@ public:
         auto upload = [&](int64_t s) {
             ds.upload_sit_images(copy_, s, 0, ds.height(), images[size_t(s)].get(), st_);
             check(xpr_event_record(up_img[size_t(s)].e, copy_), "event");
+            if (final32 && s == S - 1) {
+                ds.upload_sit_images(copy_, s, 0, ds.height(), img32.get(), XPR_F32);
+                check(xpr_event_record(up32.e, copy_), "event");
+            }
         };
         if (resident_) upload(0);
         note("launches_per_pass", Json::number(double(nslab_)));
@@ -903,6 +920,17 @@ public:
                 } else {
                     ds.upload_sit_images(stream_, s, 0, ds.height(), images[0].get(), st_);
                     ops.setImagesBuffer(images[0].get(), st_);
+                }
+                if (last && final32) {
+                    if (resident_) check(xpr_stream_wait_event(stream_, up32.e), "wait");
+                    else ds.upload_sit_images(stream_, s, 0, ds.height(), img32.get(), XPR_F32);
+                    check(xpr_convert(stream_, vol32.get(), XPR_F32, vol_.get(), st_, uint64_t(H) * layer_),
+                          "convert");
+                    cvol_ = vol32.get();
+                    cst_ = XPR_F32;
+                    csb_ = 4;
+                    ops.setImagesBuffer(img32.get(), XPR_F32);
+                    ops.setErrorBuffer(err32.get(), XPR_F32);
                 }
                 if (!first) {
                     check(xpr_event_record(fb.e, stream_), "event");
@@ -927,7 +955,7 @@ public:
                 log_sit(mit, s, ms, fms, bms);
             }
         }
-        if (store_ == kDevice) write_final(ds, stream_, vol_.get(), st_, 0, H, stage_);
+        if (store_ == kDevice) write_final(ds, stream_, cvol_, cst_, 0, H, stage_);
         check(xpr_stream_sync(copy_back_), "sync");
         check(xpr_stream_sync(copy_), "sync");
         check(xpr_stream_sync(stream_), "sync");
@@ -942,6 +970,10 @@ private:
     bool resident_ = false;          // every sub-iteration's images in HBM
     int st_ = XPR_F32;
     size_t sb_ = 4, layer_ = 0;
+    bool final32_ = false;           // last sub-iteration with float32 storage
+    void *cvol_ = nullptr;           // the HBM volume the kernels update, its dtype and size
+    int cst_ = XPR_F32;
+    size_t csb_ = 4;
     int64_t nslab_ = 0;
     // dup_[i]: float32 staging of slab buffer i for 16-bit storage, used in
     // both directions (the upload's conversion out of it is done before the
@@ -979,6 +1011,7 @@ private:
                      ds_->projdata().size() * 4 + staging;
         if (st == kDevice) m += H * layer * sv;
         else m += 2 * uint64_t(Hs) * layer * (sv + (sv != 4 ? 4 : 0));
+        if (st == kDevice && sv != 4 && p_.xpr.final_fp32) m += H * layer * 4 + As * P * 8;
         return m;
     }
 
@@ -1003,6 +1036,8 @@ private:
             hs_ = std::max<int64_t>(hs_, 1);
         }
         resident_ = need(store_, hs_, true) <= budget();
+        final32_ = store_ == kDevice && storage() != XPR_F32 && p_.xpr.final_fp32;
+        note("final_fp32", Json::boolean(final32_));
         p_.xpr.layer_store = store_ == kDevice ? "device" : store_ == kRam ? "ram" : "zstd";
         if (store_ == kZstd) ds_->ensure_output_dir();
         note("layer_store", Json::string(p_.xpr.layer_store));
@@ -1058,7 +1093,7 @@ private:
     void set_slab(Operators &ops, int64_t s, int64_t b, void *vol)
     {
         const int64_t l0 = b * hs_, l1 = std::min(ds_->vheight(), l0 + hs_);
-        ops.setVolumeBuffer(vol, st_);
+        ops.setVolumeBuffer(vol, cst_);
         ops.setVolumeParameters({0, int(l0), 0}, {int(ds_->vwidth()), int(l1), int(ds_->vwidth())},
                                 ds_->vx());
         if (cull_) {
@@ -1080,7 +1115,7 @@ private:
         if (store_ == kDevice) {
             // the layer store is the resident volume: slabs are launched in place
             for (int64_t b = 0; b < nslab_; ++b) {
-                set_slab(ops, s, b, static_cast<char *>(vol_.get()) + size_t(b * hs_) * layer_ * sb_);
+                set_slab(ops, s, b, static_cast<char *>(cvol_) + size_t(b * hs_) * layer_ * csb_);
                 if (backward) ops.backward();
                 else ops.forward();
             }

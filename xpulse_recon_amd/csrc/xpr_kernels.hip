@@ -62,6 +62,7 @@ int fail(int code, const std::string &msg)
 #include "kernels/forward.hpp"
 #include "kernels/elementwise.hpp"
 #include "kernels/batch.hpp"
+#include "kernels/calibrate.hpp"
 
 namespace {
 
@@ -971,6 +972,56 @@ int xpr_check_fast_paths(xpr_stream s, uint64_t mismatches[3])
     (void)hipFreeAsync(d, st);
     (void)hipStreamSynchronize(st);
     return rc;
+}
+
+int xpr_valu_ceiling(xpr_stream s, double *lane_instr_per_s)
+{
+    if (!lane_instr_per_s) return fail(XPR_E_INVALID, "null output");
+    hipStream_t st = (hipStream_t)s;
+    int dev = 0;
+    hipDeviceProp_t prop;
+    XPR_HIP_CHECK(hipGetDevice(&dev));
+    XPR_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+    // 8 workgroups of 4 waves per CU: 8 waves per SIMD
+    const unsigned blocks = (unsigned)prop.multiProcessorCount * 8u;
+    const uint32_t iters = 4096;
+    float *out = nullptr;
+    XPR_HIP_CHECK(hipMallocAsync((void **)&out, (size_t)blocks * 256 * sizeof(float), st));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = XPR_OK;
+    float best = 3.4e38f;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = fail(XPR_E_HIP, "event");
+    } else {
+        // per kind: one warm-up, then the best of five
+        for (int r = 0; r < 18 && rc == XPR_OK; ++r) {
+            (void)hipEventRecord(e0, st);
+            if (r < 6)
+                hipLaunchKernelGGL(k_valu_ceiling<0>, dim3(blocks), dim3(256), 0, st, out, iters,
+                                   0.999f);
+            else if (r < 12)
+                hipLaunchKernelGGL(k_valu_ceiling<1>, dim3(blocks), dim3(256), 0, st, out, iters,
+                                   0.999f);
+            else
+                hipLaunchKernelGGL(k_valu_ceiling<2>, dim3(blocks), dim3(256), 0, st, out, iters,
+                                   0.999f);
+            (void)hipEventRecord(e1, st);
+            float ms = 0.0f;
+            if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                rc = fail(XPR_E_HIP, "VALU ceiling launch");
+            else if (r % 6 != 0 && ms < best)
+                best = ms;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFreeAsync(out, st);
+    (void)hipStreamSynchronize(st);
+    if (rc != XPR_OK) return rc;
+    const double lane_instr = (double)blocks * 256.0 * (double)iters * (double)kValuChains;
+    *lane_instr_per_s = lane_instr / ((double)best * 1e-3);
+    return XPR_OK;
 }
 
 int xpr_q22_to_float(xpr_stream s, const void *sums, int acc_bits, float *out, uint64_t n)
