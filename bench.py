@@ -438,10 +438,10 @@ def ooc_main(args):
     if method != "method_2" and record.get("layer_store") in ("ram", "zstd"):
         bw = pcie_bandwidth()
         V4 = W * W * H * 4
-        n = IT * S
+        nsub = IT * S
         img = (1 if record.get("images_resident") else IT) * S * A_s * D * D * 4
-        h2d = (2 * n - 2) * V4 + img
-        d2h = n * V4
+        h2d = (2 * nsub - 2) * V4 + img
+        d2h = nsub * V4
         bound = max(h2d / bw["h2d_GBps"], d2h / bw["d2h_GBps"]) / 1e9
         pcie = dict(bw, h2d_bytes=h2d, d2h_bytes=d2h, bound_s=round(bound, 3),
                     wall_over_bound=round(wall / bound, 3),

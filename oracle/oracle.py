@@ -79,6 +79,7 @@ class GeomParams(C.Structure):
         ("module_center_offset_z", C.c_double),
         ("dwidth", C.c_int64), ("dheight", C.c_int64), ("projections", C.c_int64),
         ("angle_list", C.POINTER(C.c_double)),
+        ("roll", C.c_double), ("pitch", C.c_double), ("yaw", C.c_double),
     ]
 
 
@@ -106,6 +107,9 @@ def _declare(L):
     L.orc_error_f64.argtypes = [_P, _P, _P, C.c_uint64, C.c_double]
     L.orc_backward_f64.argtypes = [_P, _P, _P, C.POINTER(Geom64)]
     L.orc_backward_f64.restype = C.c_uint64
+    L.orc_forward_f64nn.argtypes = [_P, _P, _P, C.POINTER(Geom), C.c_int, C.c_int]
+    L.orc_backward_f64nn.argtypes = [_P, _P, _P, C.POINTER(Geom), C.c_int]
+    L.orc_backward_f64nn.restype = C.c_uint64
     L.orc_minmax_normal.argtypes = [_P, C.c_uint64, C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.orc_preprocess.argtypes = [_P, C.c_uint64, C.c_int, C.c_int, C.c_float, C.c_float]
     L.orc_geometry.argtypes = [C.POINTER(GeomParams), C.c_int, _P, _P]
@@ -145,6 +149,9 @@ class GeometrySpec:
     module_angle_offset: float | None = None
     module_center_offset_z: float = 0.0
     angle_list: list | None = None
+    roll: float = 0.0          # deg, module rotation extension (FP64 geometry only)
+    pitch: float = 0.0
+    yaw: float = 0.0
 
     def _params(self):
         p = GeomParams()
@@ -155,6 +162,7 @@ class GeometrySpec:
         p.module_angle_offset = 0.0 if self.module_angle_offset is None else self.module_angle_offset
         p.module_center_offset_z = self.module_center_offset_z
         p.dwidth, p.dheight, p.projections = self.dwidth, self.dheight, self.projections
+        p.roll, p.pitch, p.yaw = self.roll, self.pitch, self.yaw
         keep = None
         if self.angle_list is not None:
             keep = np.ascontiguousarray(self.angle_list, dtype=np.float64)
@@ -172,6 +180,8 @@ def geometry(spec: GeometrySpec, fp64: bool):
     else:
         out = np.zeros((n, 20), np.float32)
         g = lib().orc_geometry(C.byref(p), 0, None, _ptr(out))
+        if g.vwidth < 0:
+            raise ValueError("roll/pitch/yaw: the oracle builds these geometries in FP64 only")
     del keep
     grid = {"vx": g.vx, "vwidth": g.vwidth, "vheight": g.vheight, "orig": tuple(g.orig)}
     return out, grid
@@ -273,6 +283,19 @@ def error_f64(images, sums, weight, out=None):
 
 def backward_f64(vol, err, projdata, g: Geom64) -> int:
     return lib().orc_backward_f64(_ptr(vol), _ptr(err), _ptr(projdata), C.byref(g))
+
+
+def forward_f64nn(vol, sums, projdata32, g: Geom, threads: int = 1, q22: bool = False):
+    """q22: truncate each contribution to a multiple of 2^-22 first
+    (FLOAT_TO_FIXED, constants.cl:22), modelling the Q.22 sums' quantisation."""
+    assert vol.dtype == np.float64 and sums.dtype == np.float64 and projdata32.dtype == np.float32
+    lib().orc_forward_f64nn(_ptr(vol), _ptr(sums), _ptr(projdata32), C.byref(g), threads,
+                            int(q22))
+
+
+def backward_f64nn(vol, err, projdata32, g: Geom, threads: int = 1) -> int:
+    assert vol.dtype == np.float64 and err.dtype == np.float64 and projdata32.dtype == np.float32
+    return lib().orc_backward_f64nn(_ptr(vol), _ptr(err), _ptr(projdata32), C.byref(g), threads)
 
 
 def minmax_normal(data: np.ndarray, mn=np.finfo(np.float32).max, mx=np.finfo(np.float32).tiny):
@@ -540,3 +563,38 @@ def reproject(volume: np.ndarray, projdata: np.ndarray, grid: dict, det_width: i
         for j, v in enumerate(views):
             out[v] = img[j]
     return out
+
+
+def reconstruct_naive_f64nn(images: np.ndarray, projdata: np.ndarray, grid: dict, it: int,
+                            sit: int, weight: float, weight_factor: float, jitter,
+                            modules: int = 1, threads: int = 1, q22: bool = False):
+    """The naive schedule (reconstruction_n.hpp:51-78) with FP64 values on the
+    canonical float32 pixel choice: the same float32 ProjData, float32 origin
+    (orig + jitter) and voxel positions as the FP32 path decide which pixel
+    each voxel-view meets; sums, error, weights and the update are double
+    (first-pass sums 1.0).  q22 truncates each forward contribution to a
+    multiple of 2^-22 as the Q.22 sums do; without it there is no
+    quantisation."""
+    n, h, w = images.shape
+    W, Hv = grid["vwidth"], grid["vheight"]
+    a_s = n // sit
+    vol = np.ones((Hv, W, W), np.float64)
+    sums = np.ones((a_s, h, w), np.float64)
+    wgt = float(np.float32(weight))
+    vx = grid["vx"]
+    for mit in range(it):
+        for s in range(sit):
+            views = sit_views(n, sit, s)
+            pd = np.ascontiguousarray(
+                np.concatenate([projdata[v * modules:(v + 1) * modules] for v in views]))
+            img = np.ascontiguousarray(images[views], np.float64)
+            o = [np.float32(np.float32(grid["orig"][i]) + np.float32(jitter[mit * sit + s][i]))
+                 for i in range(3)]
+            g = make_geom(w, a_s, modules, (0, 0, 0), (W, Hv, W), 0, h, o, (vx, vx, vx))
+            if s > 0 or mit > 0:
+                forward_f64nn(vol, sums, pd, g, threads, q22)
+            err = error_f64(img, sums, wgt)
+            backward_f64nn(vol, err, pd, g, threads)
+            sums[...] = 0.0
+        wgt = float(np.float32(np.float32(wgt) * np.float32(weight_factor)))
+    return vol

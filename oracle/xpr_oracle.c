@@ -278,6 +278,73 @@ uint64_t orc_backward_f64(double *vol, const double *err, const double *projdata
 }
 
 /* ------------------------------------------------------------------------ */
+/* FP64 values on the canonical float32 pixel choice: the voxel-views and     */
+/* pixels of the FP32 kernels (project_f32 on float32 ProjData and            */
+/* positions), with the sums, the error and the update in double.  Against   */
+/* it only value arithmetic differs from the FP32 path.  q22 != 0 truncates  */
+/* each forward contribution to a whole multiple of 2^-22 first, as          */
+/* FLOAT_TO_FIXED does (constants.cl:22), so that the Q.22 quantisation is   */
+/* modelled and only float rounding remains between the two.                 */
+/* ------------------------------------------------------------------------ */
+void orc_forward_f64nn(const double *vol, double *sums, const float *projdata,
+                       const orc_geom *g, int threads, int q22)
+{
+    int64_t nx, nl, nk;
+    slab_voxels(g, &nx, &nl, &nk);
+    const uint32_t nm = g->n_views * g->modules;
+    if (threads <= 0) threads = 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
+    for (int64_t row = 0; row < nl * nk; ++row) {
+        const int64_t l = row / nk, k = row % nk;
+        const float py = fmaf((float)(l + g->vmin[1]), g->voxel[1], g->origin[1]);
+        const float pz = fmaf((float)(k + g->vmin[2]), g->voxel[2], g->origin[2]);
+        for (int64_t i = 0; i < nx; ++i) {
+            const float px = fmaf((float)(i + g->vmin[0]), g->voxel[0], g->origin[0]);
+            double val = vol[(uint64_t)row * nx + i];
+            if (q22) val = val > 0.0 ? floor(val * 4194304.0) / 4194304.0 : 0.0;
+            for (uint32_t m = 0; m < nm; ++m) {
+                uint64_t idx;
+                if (!project_f32(projdata + 20 * (size_t)m, px, py, pz, g, m / g->modules, &idx))
+                    continue;
+#pragma omp atomic
+                sums[idx] += val;
+            }
+        }
+    }
+}
+
+uint64_t orc_backward_f64nn(double *vol, const double *err, const float *projdata,
+                            const orc_geom *g, int threads)
+{
+    int64_t nx, nl, nk;
+    slab_voxels(g, &nx, &nl, &nk);
+    const uint32_t nm = g->n_views * g->modules;
+    uint64_t empty = 0;
+    if (threads <= 0) threads = 1;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : empty) num_threads(threads)
+    for (int64_t row = 0; row < nl * nk; ++row) {
+        const int64_t l = row / nk, k = row % nk;
+        const float py = fmaf((float)(l + g->vmin[1]), g->voxel[1], g->origin[1]);
+        const float pz = fmaf((float)(k + g->vmin[2]), g->voxel[2], g->origin[2]);
+        for (int64_t i = 0; i < nx; ++i) {
+            const float px = fmaf((float)(i + g->vmin[0]), g->voxel[0], g->origin[0]);
+            double sum = 0.0;
+            uint32_t n = 0;
+            for (uint32_t m = 0; m < nm; ++m) {
+                uint64_t idx;
+                if (project_f32(projdata + 20 * (size_t)m, px, py, pz, g, m / g->modules, &idx)) {
+                    sum += err[idx];
+                    ++n;
+                }
+            }
+            if (n == 0) { ++empty; continue; }
+            vol[(uint64_t)row * nx + i] *= exp(sum / (double)n);
+        }
+    }
+    return empty;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Preprocessing (data_loader.hpp:55-86).                                   */
 /* ------------------------------------------------------------------------ */
 void orc_minmax_normal(const float *data, uint64_t n, float *mn, float *mx)
@@ -475,6 +542,28 @@ static f44 gen_proj_f(f3 pe, f3 pa, f3 pb, f3 pc, float vpl, float vpr,
     return f44_mul(&a, &mS);
 }
 
+/* Rotation matrix (row-major) by angle t about the unit axis k:
+ * cos t I + sin t [k]x + (1 - cos t) k k^T. */
+static void axis_matrix(double M[9], d3 k, double t)
+{
+    const double c = cos(t), s = sin(t), q = 1.0 - c;
+    M[0] = c + q * k.x * k.x;       M[1] = q * k.x * k.y - s * k.z; M[2] = q * k.x * k.z + s * k.y;
+    M[3] = q * k.y * k.x + s * k.z; M[4] = c + q * k.y * k.y;       M[5] = q * k.y * k.z - s * k.x;
+    M[6] = q * k.z * k.x - s * k.y; M[7] = q * k.z * k.y + s * k.x; M[8] = c + q * k.z * k.z;
+}
+static void mat3_mul(double R[9], const double A[9], const double B[9])
+{
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            R[3 * i + j] = A[3 * i] * B[j] + A[3 * i + 1] * B[3 + j] + A[3 * i + 2] * B[6 + j];
+}
+static d3 mat3_apply(const double M[9], d3 v)
+{
+    d3 r = {M[0] * v.x + M[1] * v.y + M[2] * v.z, M[3] * v.x + M[4] * v.y + M[5] * v.z,
+            M[6] * v.x + M[7] * v.y + M[8] * v.z};
+    return r;
+}
+
 static double angle_deg_d(const orc_geom_params *p, int64_t i)
 {
     if (p->angle_list) return p->angle_list[i];
@@ -517,6 +606,22 @@ static void frame_d(const orc_geom_params *p, int64_t a, int64_t mod, d_frame *f
                    d3_mul(vdir, 0.5 * height * p->px));
     d3 ur = d3_add(ul, d3_mul(u, (double)width * p->px));
     d3 bl = d3_add(ul, d3_mul(vdir, height * p->px));
+    /* module rotation about the detector centre (the build's extension of the
+     * formula commented out at geometry.hpp:260-262): the corners' offsets
+     * from the centre turned by yaw about u, then pitch about v, then roll
+     * about the beam axis n, all fixed axes: R = R_n(roll) R_v(pitch) R_u(yaw) */
+    if (p->roll != 0.0 || p->pitch != 0.0 || p->yaw != 0.0) {
+        const d3 nb = d3_norm(d3_sub(tgt, pos));
+        double Ru[9], Rv[9], Rn[9], T1[9], R[9];
+        axis_matrix(Ru, u, p->yaw * (M_PI / 180.0));
+        axis_matrix(Rv, vdir, p->pitch * (M_PI / 180.0));
+        axis_matrix(Rn, nb, p->roll * (M_PI / 180.0));
+        mat3_mul(T1, Rv, Ru);
+        mat3_mul(R, Rn, T1);
+        ul = d3_add(det, mat3_apply(R, d3_sub(ul, det)));
+        ur = d3_add(det, mat3_apply(R, d3_sub(ur, det)));
+        bl = d3_add(det, mat3_apply(R, d3_sub(bl, det)));
+    }
     /* modules (:277-284) */
     const d3 mc = d3_add(det, d3_mul(d3_norm(d3_sub(pos, tgt)),
                                      p->sd - p->module_center_offset_z));
@@ -559,7 +664,12 @@ orc_grid orc_geometry(const orc_geom_params *p, int fp64, double *out64,
             }
         return grid;
     }
-    /* float32 path, op for op (geometry.hpp:35-76, 244-292, 298-328). */
+    /* float32 path, op for op (geometry.hpp:35-76, 244-292, 298-328); the
+     * module rotation extension has an FP64 construction only. */
+    if (p->roll != 0.0 || p->pitch != 0.0 || p->yaw != 0.0) {
+        grid.vwidth = -1;
+        return grid;
+    }
     const float so = (float)p->so, sd = (float)p->sd, px = (float)p->px;
     const float sx = (float)p->sx, sy = (float)p->sy;
     const float rx = (float)p->rx_px * px;                     /* parameters.hpp:39 */

@@ -80,6 +80,15 @@ void orc_error_f64(const double *images, const double *sums, double *err,
 uint64_t orc_backward_f64(double *vol, const double *err, const double *projdata,
                           const orc_geom64 *g);
 
+/* FP64 values on the canonical float32 pixel choice (project_f32): the
+ * voxel-views of the FP32 kernels with double sums, error and update, so
+ * that only value arithmetic differs from the FP32 path.  q22 != 0 truncates
+ * each forward contribution to a multiple of 2^-22 (FLOAT_TO_FIXED). */
+void orc_forward_f64nn(const double *vol, double *sums, const float *projdata,
+                       const orc_geom *g, int threads, int q22);
+uint64_t orc_backward_f64nn(double *vol, const double *err, const float *projdata,
+                            const orc_geom *g, int threads);
+
 /* ---- preprocessing (data_loader.hpp:55-86) ----------------------------- */
 
 /* Global min/max over isnormal() samples (data_loader.hpp:55-74).
@@ -102,6 +111,8 @@ typedef struct {
     double module_center_offset_z;
     int64_t dwidth, dheight, projections;
     const double *angle_list;      /* NULL: i*angle/projections              */
+    double roll, pitch, yaw;       /* deg, module rotation about the detector
+                                      centre (FP64 construction only)        */
 } orc_geom_params;
 
 typedef struct {

@@ -120,6 +120,8 @@ def test_streaming_schedules_equal_naive(scan, method, xpr):
     vol, log = _run(t, name, d, method=method, xpr=dict(xpr, final_fp32=False))
     np.testing.assert_array_equal(vol, naive)
     assert log["layer_store"] == xpr["layer_store"]
+    if xpr["layer_store"] == "zstd":
+        assert log["spill_compression"] == "zstd"          # libzstd.so.1 present
     if "slab_height" not in xpr:
         assert log["slab_height"] == 13
     assert log["launches_per_pass"] == -(-H // log["slab_height"])

@@ -124,3 +124,48 @@ def test_float32_compat_close_to_fp64(spec):
             worst = max(worst, float(np.abs(a).max()))
     assert worst < 5e-3
     np.testing.assert_array_equal(pd32[:, 16:], pd64[:, 16:].astype(np.float32))
+
+
+@pytest.mark.parametrize("roll", [0.3, -7.5, 30.0])
+def test_roll_is_a_rotation_of_the_detector_image(roll):
+    """A detector rolled by phi about the beam axis through its centre lies in
+    the same plane; a point's pixel coordinates are the unrolled ones turned
+    about the detector centre (W/2, H/2) by -phi in the (column, row) frame:
+    (u' - W/2, v' - H/2) = [[cos, sin], [-sin, cos]] (u - W/2, v - H/2)."""
+    kw = dict(dwidth=64, dheight=48, projections=7, rx=2.5)
+    flat = O.GeometrySpec(**kw)
+    rolled = O.GeometrySpec(**kw, roll=roll)
+    c, s = math.cos(math.radians(roll)), math.sin(math.radians(roll))
+    rng = np.random.default_rng(3)
+    pd, grid = O.geometry(rolled, fp64=True)
+    half = 0.5 * grid["vwidth"] * grid["vx"]
+    for a in range(7):
+        for _ in range(20):
+            p = rng.uniform(-half, half, 3)
+            u0, v0 = O.project_physical(flat, a, 0, *p)
+            u1, v1 = O.project_physical(rolled, a, 0, *p)
+            du, dv = u0 - 32.0, v0 - 24.0
+            assert u1 - 32.0 == pytest.approx(c * du + s * dv, abs=1e-9)
+            assert v1 - 24.0 == pytest.approx(-s * du + c * dv, abs=1e-9)
+            # and the matrices project the same way
+            mu, mv = _proj(pd[a], p)
+            assert mu == pytest.approx(u1, abs=1e-8) and mv == pytest.approx(v1, abs=1e-8)
+
+
+def test_pitch_and_yaw_matrices_equal_physical_projection():
+    """Pitch and yaw tilt the detector plane: the matrices still agree with the
+    ray/plane intersection through the turned corners, and a point on the
+    beam axis through the detector centre stays at the centre."""
+    spec = small_spec(64, 48, 6, pitch=4.0, yaw=-6.0, roll=2.0)
+    pd, grid = O.geometry(spec, fp64=True)
+    rng = np.random.default_rng(4)
+    half = 0.5 * grid["vwidth"] * grid["vx"]
+    for a in range(6):
+        for _ in range(10):
+            p = rng.uniform(-half, half, 3)
+            u, v = O.project_physical(spec, a, 0, *p)
+            mu, mv = _proj(pd[a], p)
+            assert mu == pytest.approx(u, abs=1e-8) and mv == pytest.approx(v, abs=1e-8)
+        # the source-detector axis passes through the rotation centre (rx = 0)
+        cu, cv = O.project_physical(spec, a, 0, 0.0, 0.0, 0.0)
+        assert cu == pytest.approx(32.0, abs=1e-9) and cv == pytest.approx(24.0, abs=1e-9)

@@ -298,3 +298,75 @@ def test_phantom_voxels_quadratic_form():
     assert (got > 0).mean() > 0.01
     # points within rounding of the surface may fall either side
     assert np.mean(got != want) < 1e-3
+
+
+def test_fp64_on_float32_pixels_equals_q22_sums_for_exact_values():
+    """The FP64 model on the canonical float32 pixel choice meets exactly the
+    voxel-views of the FP32 Forward: for volumes whose values are whole
+    multiples of 2^-22 (truncation exact, double sums exact) its sums are the
+    FP32 Q.22 sums times 2^-22, pixel for pixel; its hit counts are the
+    brute-force count of the FP32 path."""
+    spec = small_spec(40, 36, 10, rx=1.5, module_number=2, module_angle_offset=4.0)
+    pd, grid = projdata_for(spec, [1, 4, 7])
+    d = geom_args(spec, grid, 3, vmin=(0, 10, 0), vmax=(40, 22, 40), jitter=(0.002, 0.003, 0.001),
+                  line_offset=8, line_rows=20)
+    g = oracle_geom(d)
+    rng = np.random.default_rng(61)
+    q = rng.integers(0, 1 << 21, (12, 40, 40))
+    vol32 = (q / Q).astype(np.float32)
+    ref = np.zeros((3, 20, 40), np.uint64)
+    O.forward_f32(vol32, ref, pd, g)
+    s64 = np.zeros((3, 20, 40), np.float64)
+    O.forward_f64nn(vol32.astype(np.float64), s64, pd, g, 4)
+    np.testing.assert_array_equal(s64 * Q, ref.astype(np.float64))
+    ones = np.zeros((3, 20, 40), np.float64)
+    O.forward_f64nn(np.ones((12, 40, 40)), ones, pd, g)
+    assert ones.sum() == O.hits_f32(pd, g).sum()
+
+
+def test_fp64_q22_model_equals_q22_sums_for_any_float32_volume():
+    """With q22 the FP64 model truncates each contribution to a multiple of
+    2^-22; for any float32 volume (v * 2^22 is exact in float, so the FP32
+    truncation is the same floor) its sums are the FP32 Q.22 sums times
+    2^-22 exactly, while the unquantised model is strictly above them
+    wherever a voxel with a fractional part contributes."""
+    spec = small_spec(40, 36, 10, rx=1.5)
+    pd, grid = projdata_for(spec, [0, 5, 8])
+    d = geom_args(spec, grid, 3, jitter=(0.001, -0.002, 0.003))
+    g = oracle_geom(d)
+    rng = np.random.default_rng(63)
+    vol32 = rng.uniform(0.0, 0.01, (36, 40, 40)).astype(np.float32)
+    ref = np.zeros((3, 36, 40), np.uint64)
+    O.forward_f32(vol32, ref, pd, g)
+    s_q = np.zeros((3, 36, 40), np.float64)
+    O.forward_f64nn(vol32.astype(np.float64), s_q, pd, g, 4, q22=True)
+    np.testing.assert_array_equal(s_q * Q, ref.astype(np.float64))
+    s_raw = np.zeros((3, 36, 40), np.float64)
+    O.forward_f64nn(vol32.astype(np.float64), s_raw, pd, g, 4)
+    hit = ref > 0
+    assert hit.sum() > 1000
+    assert (s_raw[hit] > s_q[hit]).all()
+    # the truncation loses less than 2^-22 per contribution
+    cnt = np.zeros((3, 36, 40), np.float64)
+    O.forward_f64nn(np.ones((36, 40, 40)), cnt, pd, g)
+    assert ((s_raw - s_q)[hit] < cnt[hit] / Q).all()
+    assert (s_raw[~hit] == 0).all()
+
+
+def test_fp64_on_float32_pixels_backward_constant_error():
+    """Constant error c: v * exp(c) in double wherever the FP32 path sees a
+    view, untouched elsewhere (the count of untouched voxels returned)."""
+    spec = small_spec(32, 24, 12)
+    pd, grid = projdata_for(spec, [0, 3, 6, 9])
+    d = geom_args(spec, grid, 4, jitter=(0.001, 0.002, 0.0))
+    g = oracle_geom(d)
+    rng = np.random.default_rng(62)
+    vol = rng.uniform(0.5, 2.0, (24, 32, 32))
+    out = vol.copy()
+    c = 0.3
+    empty = O.backward_f64nn(out, np.full((4, 24, 32), c), pd, g, 2)
+    hits = O.hits_f32(pd, g)
+    seen = hits > 0
+    np.testing.assert_allclose(out[seen], vol[seen] * math.exp(c), rtol=1e-15)
+    np.testing.assert_array_equal(out[~seen], vol[~seen])
+    assert empty == int((~seen).sum())

@@ -59,6 +59,9 @@ Table *engine_table(Engine *e, int slot);       // pinned host table of a slot
 int engine_launch(Engine *e, int slot);
 // Waits for the slot's last replay; *ms receives its device time.
 int engine_wait(Engine *e, int slot, float *ms);
+// Waits for everything enqueued on the engine's stream (both slots), whatever
+// its outcome: after an error, no replay still uses a job's buffers.
+void engine_sync(Engine *e);
 void engine_destroy(Engine *e);
 
 } // namespace xpr_batch

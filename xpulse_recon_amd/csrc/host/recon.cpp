@@ -886,9 +886,9 @@ public:
             tune_shapes(v, st_, sums.get(), err.get(), st_, pds_[0].as<float>(), ta);
         }
         fill_sums(stream_, sums, p_.xpr.accumulator, 1ull << 22, uint64_t(As * P));
-        // previous-pass events start recorded, so the first pass waits on nothing
-        check(xpr_event_record(compute_done_.e, stream_), "event");
-        check(xpr_event_record(dl_done_.e, copy_back_), "event");
+        // the slab buffers are free once the tuning launches above are done
+        check(xpr_event_record(freed_[0].e, stream_), "event");
+        check(xpr_event_record(freed_[1].e, stream_), "event");
         // images: every sub-iteration's resident in HBM, preprocessed on the copy
         // stream ahead of the sub-iteration that first reads them, or one
         // sub-iteration's at a time on the compute stream
@@ -982,10 +982,9 @@ private:
     HostBuf hin_[2], hout_[2];
     std::vector<DevBuf> pds_;
     // Events of the slab pipelines: up_ (slab uploaded into buffer i), kd_
-    // (slab updated, ready to go back), freed_ (buffer i free again),
-    // compute_done_ / dl_done_ (end of the previous pass on the compute and
-    // the download stream).
-    Event up_[2], kd_[2], freed_[2], compute_done_, dl_done_;
+    // (slab updated, ready to go back), freed_ (buffer i free again: its
+    // slab's kernel done, or its download done after a backward).
+    Event up_[2], kd_[2], freed_[2];
 
     struct SlabViews {
         std::vector<float> pd;
@@ -1039,7 +1038,16 @@ private:
         final32_ = store_ == kDevice && storage() != XPR_F32 && p_.xpr.final_fp32;
         note("final_fp32", Json::boolean(final32_));
         p_.xpr.layer_store = store_ == kDevice ? "device" : store_ == kRam ? "ram" : "zstd";
-        if (store_ == kZstd) ds_->ensure_output_dir();
+        if (store_ == kZstd) {
+            ds_->ensure_output_dir();
+            // spill files are zstd frames; without libzstd.so.1 they are raw
+            // floats, said in the run record and on stderr
+            const bool z = zstd_available();
+            note("spill_compression", Json::string(z ? "zstd" : "none"));
+            if (!z)
+                std::cerr << "Warning: libzstd.so.1 not found, the layer spill files are written "
+                             "uncompressed" << std::endl;
+        }
         note("layer_store", Json::string(p_.xpr.layer_store));
         note("slab_height", Json::number(double(hs_)));
         note("images_resident", Json::boolean(resident_));
@@ -1134,10 +1142,6 @@ private:
         auto &ds = *ds_;
         const int64_t H = ds.vheight();
         const bool zstd = store_ == kZstd;
-        // the previous pass's kernels and downloads are done with the slab
-        // buffers and the store
-        check(xpr_stream_wait_event(copy_, compute_done_.e), "wait");
-        check(xpr_stream_wait_event(copy_, dl_done_.e), "wait");
         auto save = [&](int64_t b) {                     // zstd: slab b back to the spill files
             const int i = int(b & 1);
             const int64_t l0 = b * hs_, l1 = std::min(H, l0 + hs_);
@@ -1166,8 +1170,11 @@ private:
                     src = ds.ram_layer(l0);
                 }
             }
-            // device buffer i is free once slab b-2 is done with it
-            if (b >= 2) check(xpr_stream_wait_event(copy_, freed_[i].e), "wait");
+            // device buffer i is free once its last user (slab b-2, or a slab
+            // of the previous pass) is done with it.  Downloads run in order
+            // on copy_back_, so at b = 0, 1 this also waits for every write
+            // back of the previous pass into the store.
+            check(xpr_stream_wait_event(copy_, freed_[i].e), "wait");
             if (fill) {
                 check(xpr_fill(copy_, dslab_[i].get(), st_, 1.0f, n), "fill");
             } else if (st_ == XPR_F32) {
@@ -1207,8 +1214,6 @@ private:
             if (zstd && b >= 1) save(b - 1);
         }
         if (zstd && backward && !last && nslab_ >= 1) save(nslab_ - 1);
-        check(xpr_event_record(compute_done_.e, stream_), "event");
-        check(xpr_event_record(dl_done_.e, copy_back_), "event");
     }
 };
 

@@ -582,21 +582,37 @@ struct Server {
                     ready.pop_front();
                 }
             }
+            // a failed step: wait until neither slot's replay can still touch
+            // a job's buffers, then release every job in flight
+            auto fail_all = [&](const std::string &what) {
+                xpr_batch::engine_sync(eng);
+                busy[0] = busy[1] = false;
+                std::lock_guard<std::mutex> lk(m);
+                engine_error = what;
+                for (int k = 0; k < 2; ++k) {
+                    for (auto &r : members[k]) r->gpu_done = true;
+                    for (auto &r : finishing[k]) r->gpu_done = true;
+                    members[k].clear();
+                    finishing[k].clear();
+                }
+                for (auto &r : active) r->gpu_done = true;
+                active.clear();
+                cv.notify_all();
+            };
             // the slot's previous step: account its time, release finished jobs
             if (busy[slot]) {
                 float ms = 0.0f;
                 const bool ok = xpr_batch::engine_wait(eng, slot, &ms) == XPR_OK;
+                if (!ok) {
+                    fail_all(std::string("batch step: ") + xpr_last_error());
+                    continue;
+                }
                 busy[slot] = false;
                 std::lock_guard<std::mutex> lk(m);
-                if (!ok) engine_error = std::string("batch step: ") + xpr_last_error();
                 for (auto &r : members[slot]) r->step_ms += ms;
                 for (auto &r : finishing[slot]) r->gpu_done = true;
                 members[slot].clear();
                 finishing[slot].clear();
-                if (!ok) {
-                    for (auto &r : active) r->gpu_done = true;
-                    active.clear();
-                }
                 cv.notify_all();
             }
             if (active.empty()) {
@@ -636,14 +652,7 @@ struct Server {
             }
             active.swap(keep);
             if (xpr_batch::engine_launch(eng, slot) != XPR_OK) {
-                std::lock_guard<std::mutex> lk(m);
-                engine_error = std::string("batch step: ") + xpr_last_error();
-                for (auto &r : members[slot]) r->gpu_done = true;
-                for (auto &r : active) r->gpu_done = true;
-                members[slot].clear();
-                finishing[slot].clear();
-                active.clear();
-                cv.notify_all();
+                fail_all(std::string("batch step: ") + xpr_last_error());
                 continue;
             }
             busy[slot] = true;

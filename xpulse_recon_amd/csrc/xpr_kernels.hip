@@ -511,6 +511,12 @@ int engine_wait(Engine *e, int slot, float *ms)
     return XPR_OK;
 }
 
+void engine_sync(Engine *e)
+{
+    if (e && e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e) e->pending[0] = e->pending[1] = false;
+}
+
 void engine_destroy(Engine *e)
 {
     if (!e) return;
