@@ -72,6 +72,8 @@ def parse():
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the short serving run (p50/p99) of the default line")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP16/BF16 runs")
+    ap.add_argument("--no-opt-in", action="store_true",
+                    help="skip the reconstructions with float sums and with bilinear sampling")
     ap.add_argument("--serve", action="store_true",
                     help="serving run instead: Poisson arrivals of reconstructions of "
                          "mixed sizes, p50/p99 time-to-reconstruction")
@@ -497,18 +499,18 @@ def headline(args):
     log(f"synthetic data ready in {time.perf_counter() - t0:.1f} s")
     lib = _lib.load()
 
-    def params(storage):
+    def params(storage, extra=None):
         return {"recon": {"method": "method_1a", "it": IT, "sit": SIT, "mlog": True,
                           "weight": 1.0, "weight_factor": 0.9},
                 "geometry": {"so": 100.0, "sd": 1000.0},
                 "detector": {"px": 0.1, "rx": 2.5, "roll": 0.3},
                 "xpr": {"storage": storage, "final_fp32": True, "layer_store": args.store,
-                        "slab_height": SLAB, "jitter_seed": 1004, "quiet": True}}
+                        "slab_height": SLAB, "jitter_seed": 1004, "quiet": True, **(extra or {})}}
 
     seen = []                    # every run record (tuning is noted by the first)
 
-    def run(storage, steps, warmup):
-        text = json.dumps(params(storage)).encode()
+    def run(storage, steps, warmup, extra=None):
+        text = json.dumps(params(storage, extra)).encode()
         rec = C.create_string_buffer(1 << 20)
 
         def one():
@@ -609,6 +611,21 @@ def headline(args):
                          "rmse_vs_phantom": r["rmse"], "maxabs_vs_phantom": r["maxabs"],
                          "rmse_ratio": round(r["rmse"] / f32["rmse"], 5), "steps": k2}
 
+    # ---- opt-in modes outside the reference's arithmetic (same workload) ------
+    opt_in = {}
+    if not args.no_opt_in:
+        for key, extra in (("float_sums", {"accumulator": "f32"}),
+                           ("bilinear", {"interp": "bilinear"})):
+            r = run("f32", 1, 1, extra)
+            rr = r["recs"][-1]
+            opt_in[key] = {
+                "xpr": extra, "ms_per_reconstruction": round(r["total_s"] * 1e3, 2),
+                "updates_per_s": vv_recon / r["total_s"],
+                "speed_vs_plain_fp32": round(vv_recon / r["total_s"] / value, 4),
+                "forward_launch_ms": round(float(np.mean(rr["forward_pass_ms"])) / L, 4),
+                "backward_launch_ms": round(float(np.mean(rr["backward_pass_ms"])) / L, 4),
+                "rmse_vs_phantom": r["rmse"], "rmse_ratio": round(r["rmse"] / f32["rmse"], 5)}
+
     # ---- secondary figures ----------------------------------------------------
     subit = [x for r in f32["recs"] for x in r["subiteration_ms"][1:]]
     secondary = {
@@ -632,6 +649,8 @@ def headline(args):
                                          "agree (this workload)"},
         "kernel_sources_sha16": kernel_sha16(),
     }
+    if opt_in:
+        secondary["opt_in_modes"] = opt_in
     if not args.no_serving:
         log("serving run ...")
         secondary["serving"] = serve_run(16, 4.0, "128,384,768", "f16", 200.0, 8)

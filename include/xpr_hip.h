@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define XPR_ABI_VERSION 1
+#define XPR_ABI_VERSION 2
 
 /* status codes (stand in for CHECK -> std::runtime_error, gpu/error.hpp:17-27,
  * and std::bad_alloc, gpu/context.hpp:329-336) */
@@ -37,11 +37,27 @@ extern "C" {
 #define XPR_F16  1
 #define XPR_BF16 2
 
-/* Q.22 accumulator widths of the detector sums buffer.  32 reproduces the
- * reference's uint32 Q10.22 (wraps mod 2^32, constants.cl:17-22); 64 is the
- * same sum without wrap-around. */
+/* Accumulators of the detector sums buffer (the acc_bits arguments).  32
+ * reproduces the reference's uint32 Q10.22 (wraps mod 2^32,
+ * constants.cl:17-22); 64 is the same sum without wrap-around.  Both are
+ * integers: the sums do not depend on launch order. */
 #define XPR_ACC_U32 32
 #define XPR_ACC_U64 64
+/* Opt-in, not the reference's arithmetic: float32 sums, to which every
+ * voxel-view adds its float value (no Q.22 truncation) through LDS and
+ * global float atomics.  The order of those additions changes from launch to
+ * launch, so the sums do, within float rounding of the exact sum. */
+#define XPR_ACC_F32 (-32)
+
+/* Detector sampling of the backprojector (xpr_proj_args.interp). */
+#define XPR_INTERP_NEAREST  0  /* the reference's: the pixel roundf picks (KernelBackward.cl:34) */
+/* Opt-in, not the reference's arithmetic: the voxel-views the
+ * nearest-neighbour test accepts sample the error images bilinearly at the
+ * continuous coordinate (u, v) (pixel k centred at k), taps (floor u, floor
+ * v) .. (+1, +1) clamped to the accepted pixels (the reference's unused
+ * INTERP_BILIN branch, KernelForward.cl:38-53, states the intent; it never
+ * compiled). */
+#define XPR_INTERP_BILINEAR 1
 
 typedef void *xpr_stream;   /* a hipStream_t */
 
@@ -59,6 +75,10 @@ typedef struct {
     int32_t  line_rows;    /* lineOffsetY.y: detector rows held per view             */
     float    origin[3];    /* origin (mm) of voxel (0,0,0), jitter included          */
     float    voxel[3];     /* voxelSize (mm)                                         */
+    /* XPR_INTERP_NEAREST (0, the reference's) or XPR_INTERP_BILINEAR: how
+     * xpr_backward samples the error images.  The forward always uses the
+     * nearest pixel.  (It fills what would be padding before view_index.) */
+    int32_t  interp;
     /* Optional device array of n_views detector view indices: view j of this
      * call reads ProjData record j and detector view view_index[j] (the
      * per-layer view lists of ForwardH/BackwardH, KernelForwardH.cl:8-37,
@@ -118,6 +138,7 @@ int xpr_convert(xpr_stream s, void *dst, int dst_dtype, const void *src, int src
  * of the slab into the detector pixel each view rounds it to:
  *   volume   : slab [layer][depth][x] of (vmax-vmin) voxels, dtype vol_dtype
  *   sums     : uint32 or uint64 Q.22 [n_views][line_rows][det_width]
+ *              (acc_bits 32 / 64), or float32 (acc_bits XPR_ACC_F32: adds v)
  *   projdata : device float[n_views*modules][20] (ProjData, constants.cl:25-28)
  * Integer accumulation: the result does not depend on launch order. */
 int xpr_forward(xpr_stream s, const void *volume, int vol_dtype, void *sums, int acc_bits,
@@ -125,8 +146,9 @@ int xpr_forward(xpr_stream s, const void *volume, int vol_dtype, void *sums, int
 
 /* Error (Reconstruction::error, src/reconstruction.hpp:302-304; kernel Error,
  * KernelError.cl:8-24): err = log(max(img / max(sum*2^-22, eps), eps)) * weight,
- * eps = 2^-22, for n pixels.  err may alias sums when acc_bits == 32 and
- * err_dtype == XPR_F32 (the reference's in-place form). */
+ * eps = 2^-22, for n pixels (float sums, XPR_ACC_F32: sum itself in place of
+ * sum*2^-22).  err may alias sums when acc_bits == 32 and err_dtype ==
+ * XPR_F32 (the reference's in-place form). */
 int xpr_error(xpr_stream s, const void *images, int img_dtype, const void *sums, int acc_bits,
               void *err, int err_dtype, uint64_t n, float weight);
 
@@ -135,7 +157,15 @@ int xpr_error(xpr_stream s, const void *images, int img_dtype, const void *sums,
  *   v <- v * exp(sum_j err_j(pixel_j(v)) / n_in_bounds)
  * Voxels with no in-bounds view keep their value (the reference computes
  * exp(0/0)); if empty_count (device uint64) is non-NULL their number is added
- * to it. */
+ * to it.  a->interp == XPR_INTERP_BILINEAR (opt-in) samples err bilinearly
+ * at (u, v) instead, over the same in-bounds voxel-views: with
+ * fx = u - floor(u), fy = v - floor(v), columns i0 = max(floor u, cl),
+ * i1 = min(floor u + 1, cr) and rows j0, j1 likewise, [cl, cr] the accepted
+ * columns ceil(max(vpl, 0)) .. ceil(min(vpr, det_width)) - 1 (rows with vpb,
+ * vpt and line_rows),
+ *   top = fmaf(fx, E[j0][i1] - E[j0][i0], E[j0][i0]),
+ *   bot = fmaf(fx, E[j1][i1] - E[j1][i0], E[j1][i0]),
+ *   sample = fmaf(fy, bot - top, top). */
 int xpr_backward(xpr_stream s, void *volume, int vol_dtype, const void *err, int err_dtype,
                  const float *projdata, const xpr_proj_args *a, uint64_t *empty_count);
 
@@ -148,8 +178,8 @@ int xpr_backward(xpr_stream s, void *volume, int vol_dtype, const void *err, int
  *   err  <- xpr_error(images, sums, weight) over n_views*line_rows*det_width
  *           pixels;
  *   xpr_backward of err into the volume (update in place).
- * images: [n_views][line_rows][det_width] of img_dtype.  sums (acc_bits
- * Q.22) and err (err_dtype) are the caller's workspace, n_views*line_rows*
+ * images: [n_views][line_rows][det_width] of img_dtype.  sums (acc_bits)
+ * and err (err_dtype) are the caller's workspace, n_views*line_rows*
  * det_width elements each; err may alias sums when acc_bits == 32 and
  * err_dtype == XPR_F32.  Their contents on entry do not matter; on return
  * err holds the error images.  a->view_index must be NULL. */
@@ -163,7 +193,7 @@ int xpr_subiteration(xpr_stream s, void *volume, int vol_dtype, const void *imag
  * harnesses that forbid hipMalloc): the sums and error workspace live in a
  * static device array of XPR_STATIC_SCRATCH_BYTES per GPU, held from the
  * library's load until the process ends; the call allocates nothing.
- * n_views*line_rows*det_width*(acc_bits/8 + error sample bytes) + 256 must
+ * n_views*line_rows*det_width*(|acc_bits|/8 + error sample bytes) + 256 must
  * not exceed XPR_STATIC_SCRATCH_BYTES (XPR_E_INVALID otherwise: e.g. 36 views
  * of 384x384 with 64-bit sums and f32 error fit, 36 views of 768x768 do not).
  * The sums are cleared inside the call.  All calls on one GPU share its
@@ -175,8 +205,9 @@ int xpr_subiteration_static(xpr_stream s, void *volume, int vol_dtype, const voi
                             float weight, int first_pass, int acc_bits, int err_dtype,
                             uint64_t *empty_count);
 
-/* Q.22 sums to float (FIXED_TO_FLOAT, constants.cl:21) for n pixels; used to
- * write reprojected views (Reconstruction::forward_exec,
+/* Q.22 sums to float (FIXED_TO_FLOAT, constants.cl:21; float sums are
+ * copied) for n pixels; used to write reprojected views
+ * (Reconstruction::forward_exec,
  * src/reconstruction.hpp:139-153). */
 int xpr_q22_to_float(xpr_stream s, const void *sums, int acc_bits, float *out, uint64_t n);
 
@@ -354,7 +385,7 @@ typedef struct {
     int32_t mlog, normalize;
     int32_t storage;                 /* XPR_F32 / XPR_F16 / XPR_BF16 */
     int32_t final_fp32;              /* last sub-iteration in FP32 storage */
-    int32_t acc_bits;                /* XPR_ACC_U32 / XPR_ACC_U64 */
+    int32_t acc_bits;                /* XPR_ACC_U32 / XPR_ACC_U64 / XPR_ACC_F32 */
     int32_t jitter;                  /* 0: no origin jitter */
     int64_t jitter_seed;
     int64_t slab_height;             /* used by the streaming fallback */

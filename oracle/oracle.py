@@ -110,6 +110,10 @@ def _declare(L):
     L.orc_forward_f64nn.argtypes = [_P, _P, _P, C.POINTER(Geom), C.c_int, C.c_int]
     L.orc_backward_f64nn.argtypes = [_P, _P, _P, C.POINTER(Geom), C.c_int]
     L.orc_backward_f64nn.restype = C.c_uint64
+    L.orc_backward_bilinear_f32.argtypes = [_P, _P, _P, C.POINTER(Geom), C.c_int]
+    L.orc_backward_bilinear_f32.restype = C.c_uint64
+    L.orc_backward_bilinear_f64nn.argtypes = [_P, _P, _P, C.POINTER(Geom), C.c_int]
+    L.orc_backward_bilinear_f64nn.restype = C.c_uint64
     L.orc_minmax_normal.argtypes = [_P, C.c_uint64, C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.orc_preprocess.argtypes = [_P, C.c_uint64, C.c_int, C.c_int, C.c_float, C.c_float]
     L.orc_geometry.argtypes = [C.POINTER(GeomParams), C.c_int, _P, _P]
@@ -298,6 +302,25 @@ def backward_f64nn(vol, err, projdata32, g: Geom, threads: int = 1) -> int:
     return lib().orc_backward_f64nn(_ptr(vol), _ptr(err), _ptr(projdata32), C.byref(g), threads)
 
 
+def backward_bilinear_f32(vol, err, projdata, g: Geom, threads: int = 1) -> int:
+    """Opt-in bilinear backprojector, FP32 canonical form (the intent of the
+    unused INTERP_BILIN branch, KernelForward.cl:38-53): taps (floor u, floor
+    v)..(+1, +1) clamped to the pixels the nearest-neighbour test accepts, over
+    the voxel-views it accepts; three fmaf lerps, update
+    v * expf(sum / n).  Returns the count of voxels with no view."""
+    assert vol.dtype == np.float32 and err.dtype == np.float32 and projdata.dtype == np.float32
+    return lib().orc_backward_bilinear_f32(_ptr(vol), _ptr(err), _ptr(projdata), C.byref(g),
+                                           threads)
+
+
+def backward_bilinear_f64nn(vol, err, projdata32, g: Geom, threads: int = 1) -> int:
+    """FP64 model of the bilinear backprojector on the float32 accept set and
+    taps (double weights, sample, sum and update)."""
+    assert vol.dtype == np.float64 and err.dtype == np.float64 and projdata32.dtype == np.float32
+    return lib().orc_backward_bilinear_f64nn(_ptr(vol), _ptr(err), _ptr(projdata32), C.byref(g),
+                                             threads)
+
+
 def minmax_normal(data: np.ndarray, mn=np.finfo(np.float32).max, mx=np.finfo(np.float32).tiny):
     a, b = C.c_float(mn), C.c_float(mx)
     lib().orc_minmax_normal(_ptr(data), data.size, C.byref(a), C.byref(b))
@@ -322,12 +345,14 @@ def sit_views(n_proj: int, sit: int, s: int) -> list:
 
 def reconstruct_naive(images: np.ndarray, projdata: np.ndarray, grid: dict, it: int, sit: int,
                       weight: float, weight_factor: float, jitter, modules: int = 1,
-                      acc_bits: int = 64, threads: int = 1):
+                      acc_bits: int = 64, threads: int = 1, interp: str = "nearest"):
     """FP32 canonical naive schedule (reconstruction_n.hpp:51-78).
 
     images: [N, H, W] preprocessed projections; projdata: [N*modules, 20] f32;
     jitter: array [it*sit, 3] of origin offsets, one row per sub-iteration
-    (src/reconstruction.hpp:268-272 with explicit draws)."""
+    (src/reconstruction.hpp:268-272 with explicit draws).  interp "bilinear":
+    the opt-in bilinear backprojector (backward_bilinear_f32)."""
+    bwd = backward_bilinear_f32 if interp == "bilinear" else backward_f32
     n, h, w = images.shape
     W, Hv = grid["vwidth"], grid["vheight"]
     a_s = n // sit
@@ -348,7 +373,7 @@ def reconstruct_naive(images: np.ndarray, projdata: np.ndarray, grid: dict, it: 
             if s > 0 or mit > 0:
                 forward_f32(vol, sums, pd, g, threads)
             err = error_f32(img, sums, float(wgt))
-            backward_f32(vol, err, pd, g, threads)
+            bwd(vol, err, pd, g, threads)
             sums[...] = 0
         wgt = np.float32(wgt * np.float32(weight_factor))
     return vol
@@ -567,14 +592,17 @@ def reproject(volume: np.ndarray, projdata: np.ndarray, grid: dict, det_width: i
 
 def reconstruct_naive_f64nn(images: np.ndarray, projdata: np.ndarray, grid: dict, it: int,
                             sit: int, weight: float, weight_factor: float, jitter,
-                            modules: int = 1, threads: int = 1, q22: bool = False):
+                            modules: int = 1, threads: int = 1, q22: bool = False,
+                            interp: str = "nearest"):
     """The naive schedule (reconstruction_n.hpp:51-78) with FP64 values on the
     canonical float32 pixel choice: the same float32 ProjData, float32 origin
     (orig + jitter) and voxel positions as the FP32 path decide which pixel
     each voxel-view meets; sums, error, weights and the update are double
     (first-pass sums 1.0).  q22 truncates each forward contribution to a
     multiple of 2^-22 as the Q.22 sums do; without it there is no
-    quantisation."""
+    quantisation (the definition of float sums).  interp "bilinear": the
+    bilinear backprojector's FP64 model (backward_bilinear_f64nn)."""
+    bwd = backward_bilinear_f64nn if interp == "bilinear" else backward_f64nn
     n, h, w = images.shape
     W, Hv = grid["vwidth"], grid["vheight"]
     a_s = n // sit
@@ -594,7 +622,7 @@ def reconstruct_naive_f64nn(images: np.ndarray, projdata: np.ndarray, grid: dict
             if s > 0 or mit > 0:
                 forward_f64nn(vol, sums, pd, g, threads, q22)
             err = error_f64(img, sums, wgt)
-            backward_f64nn(vol, err, pd, g, threads)
+            bwd(vol, err, pd, g, threads)
             sums[...] = 0.0
         wgt = float(np.float32(np.float32(wgt) * np.float32(weight_factor)))
     return vol

@@ -345,6 +345,137 @@ uint64_t orc_backward_f64nn(double *vol, const double *err, const float *projdat
 }
 
 /* ------------------------------------------------------------------------ */
+/* Opt-in bilinear backprojector.  The reference's built kernels sample the  */
+/* nearest pixel only; KernelForward.cl:38-53 holds an unused bilinear       */
+/* branch (INTERP_BILIN, never defined) whose intent this follows.  The      */
+/* voxel-views counted are those the nearest-neighbour test accepts          */
+/* (KernelForward.cl:34-35, with the viewport clamped to the detector and    */
+/* the rows held); each samples the error image at the continuous (u, v) of  */
+/* the canonical projection, pixel k centred at coordinate k, from columns   */
+/* floor u, floor u + 1 and rows floor v, floor v + 1, each clamped to the   */
+/* accepted pixels [cl, cr - 1] x [rl, rr - 1] (clamp to edge: an edge       */
+/* pixel's value holds over the half pixel beyond its centre), with          */
+/*   cl = ceil(max(vpl, 0)), cr = ceil(min(vpr, det_width)),                 */
+/*   rl = ceil(max(vpb, 0)), rr = ceil(min(vpt, line_rows)).                 */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    float u, v;            /* continuous coordinate                         */
+    float fu, fv;          /* floor u, floor v                              */
+    uint64_t c0, c1, r0, r1;  /* clamped tap columns and rows               */
+} bil_taps;
+
+static inline int project_bilinear_f32(const float *pd, float px, float py, float pz,
+                                       const orc_geom *g, bil_taps *t)
+{
+    const float X = fmaf(pd[0], px, fmaf(pd[1], py, fmaf(pd[2], pz, pd[3])));
+    const float Y = fmaf(pd[4], px, fmaf(pd[5], py, fmaf(pd[6], pz, pd[7])));
+    const float W = fmaf(pd[12], px, fmaf(pd[13], py, fmaf(pd[14], pz, pd[15])));
+    const float r = 1.0f / W;
+    t->u = X * r;
+    t->v = Y * r - (float)g->line_offset;
+    const float cl = ceilf(fmaxf(pd[16], 0.0f)), cr = ceilf(fminf(pd[17], (float)g->det_width));
+    const float rl = ceilf(fmaxf(pd[18], 0.0f)), rr = ceilf(fminf(pd[19], (float)g->line_rows));
+    const float cu = roundf(t->u), cv = roundf(t->v);
+    if (!(cu >= cl && cu < cr && cv >= rl && cv < rr)) return 0;
+    t->fu = floorf(t->u);
+    t->fv = floorf(t->v);
+    t->c0 = (uint64_t)fmaxf(t->fu, cl);
+    t->c1 = (uint64_t)fminf(t->fu + 1.0f, cr - 1.0f);
+    t->r0 = (uint64_t)fmaxf(t->fv, rl);
+    t->r1 = (uint64_t)fminf(t->fv + 1.0f, rr - 1.0f);
+    return 1;
+}
+
+/* FP32 canonical form: weights fx = u - floor u, fy = v - floor v and three
+ * lerps over the clamped taps,
+ *   top = fmaf(fx, E[r0][c1] - E[r0][c0], E[r0][c0]),
+ *   bot = fmaf(fx, E[r1][c1] - E[r1][c0], E[r1][c0]),
+ *   sample = fmaf(fy, bot - top, top),
+ * summed in view order; update v * expf(sum / n) as Backward. */
+uint64_t orc_backward_bilinear_f32(float *vol, const float *err, const float *projdata,
+                                   const orc_geom *g, int threads)
+{
+    int64_t nx, nl, nk;
+    slab_voxels(g, &nx, &nl, &nk);
+    const uint32_t nm = g->n_views * g->modules;
+    const uint64_t W = g->det_width, view_elems = W * (uint64_t)g->line_rows;
+    uint64_t empty = 0;
+    if (threads <= 0) threads = 1;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : empty) num_threads(threads)
+    for (int64_t row = 0; row < nl * nk; ++row) {
+        const int64_t l = row / nk, k = row % nk;
+        const float py = fmaf((float)(l + g->vmin[1]), g->voxel[1], g->origin[1]);
+        const float pz = fmaf((float)(k + g->vmin[2]), g->voxel[2], g->origin[2]);
+        for (int64_t i = 0; i < nx; ++i) {
+            const float px = fmaf((float)(i + g->vmin[0]), g->voxel[0], g->origin[0]);
+            float sum = 0.0f;
+            uint32_t n = 0;
+            for (uint32_t m = 0; m < nm; ++m) {
+                bil_taps t;
+                if (!project_bilinear_f32(projdata + 20 * (size_t)m, px, py, pz, g, &t)) continue;
+                const float *E = err + (uint64_t)(m / g->modules) * view_elems;
+                const float fx = t.u - t.fu, fy = t.v - t.fv;
+                const float e00 = E[t.r0 * W + t.c0], e01 = E[t.r0 * W + t.c1];
+                const float e10 = E[t.r1 * W + t.c0], e11 = E[t.r1 * W + t.c1];
+                const float top = fmaf(fx, e01 - e00, e00);
+                const float bot = fmaf(fx, e11 - e10, e10);
+                sum = sum + fmaf(fy, bot - top, top);
+                ++n;
+            }
+            if (n == 0) { ++empty; continue; }
+            const uint64_t id = (uint64_t)row * nx + i;
+            vol[id] = vol[id] * expf(sum / (float)n);
+        }
+    }
+    return empty;
+}
+
+/* FP64 model of the same operator on the float32 accept set: which
+ * voxel-views count and their clamped taps come from project_bilinear_f32
+ * (as on the device); the weights use u and v evaluated in double from the
+ * same float32 positions and ProjData, fx = u - floor u (the float floor),
+ * and the sample
+ *   (1-fy) ((1-fx) E[r0][c0] + fx E[r0][c1]) + fy ((1-fx) E[r1][c0] + fx E[r1][c1]),
+ * the sum and the update are double. */
+uint64_t orc_backward_bilinear_f64nn(double *vol, const double *err, const float *projdata,
+                                     const orc_geom *g, int threads)
+{
+    int64_t nx, nl, nk;
+    slab_voxels(g, &nx, &nl, &nk);
+    const uint32_t nm = g->n_views * g->modules;
+    const uint64_t W = g->det_width, view_elems = W * (uint64_t)g->line_rows;
+    uint64_t empty = 0;
+    if (threads <= 0) threads = 1;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : empty) num_threads(threads)
+    for (int64_t row = 0; row < nl * nk; ++row) {
+        const int64_t l = row / nk, k = row % nk;
+        const float py = fmaf((float)(l + g->vmin[1]), g->voxel[1], g->origin[1]);
+        const float pz = fmaf((float)(k + g->vmin[2]), g->voxel[2], g->origin[2]);
+        for (int64_t i = 0; i < nx; ++i) {
+            const float px = fmaf((float)(i + g->vmin[0]), g->voxel[0], g->origin[0]);
+            double sum = 0.0;
+            uint32_t n = 0;
+            for (uint32_t m = 0; m < nm; ++m) {
+                const float *pd = projdata + 20 * (size_t)m;
+                bil_taps t;
+                if (!project_bilinear_f32(pd, px, py, pz, g, &t)) continue;
+                const double X = (double)pd[0] * px + (double)pd[1] * py + (double)pd[2] * pz + pd[3];
+                const double Y = (double)pd[4] * px + (double)pd[5] * py + (double)pd[6] * pz + pd[7];
+                const double Wd = (double)pd[12] * px + (double)pd[13] * py + (double)pd[14] * pz + pd[15];
+                const double fx = X / Wd - t.fu, fy = Y / Wd - g->line_offset - t.fv;
+                const double *E = err + (uint64_t)(m / g->modules) * view_elems;
+                sum += (1.0 - fy) * ((1.0 - fx) * E[t.r0 * W + t.c0] + fx * E[t.r0 * W + t.c1]) +
+                       fy * ((1.0 - fx) * E[t.r1 * W + t.c0] + fx * E[t.r1 * W + t.c1]);
+                ++n;
+            }
+            if (n == 0) { ++empty; continue; }
+            vol[(uint64_t)row * nx + i] *= exp(sum / (double)n);
+        }
+    }
+    return empty;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Preprocessing (data_loader.hpp:55-86).                                   */
 /* ------------------------------------------------------------------------ */
 void orc_minmax_normal(const float *data, uint64_t n, float *mn, float *mx)

@@ -89,6 +89,19 @@ void orc_forward_f64nn(const double *vol, double *sums, const float *projdata,
 uint64_t orc_backward_f64nn(double *vol, const double *err, const float *projdata,
                             const orc_geom *g, int threads);
 
+/* ---- opt-in bilinear backprojector (not the reference's arithmetic; the
+ *      intent of its unused INTERP_BILIN branch, KernelForward.cl:38-53) -- */
+
+/* FP32 canonical restatement: the voxel-views the nearest-neighbour test
+ * accepts, taps (floor u, floor v) .. (+1, +1) clamped to the accepted
+ * pixels, three fmaf lerps, update v * expf(sum/n). */
+uint64_t orc_backward_bilinear_f32(float *vol, const float *err, const float *projdata,
+                                   const orc_geom *g, int threads);
+/* FP64 model on the float32 accept set and clamped taps: double weights,
+ * sample, sum and update. */
+uint64_t orc_backward_bilinear_f64nn(double *vol, const double *err, const float *projdata,
+                                     const orc_geom *g, int threads);
+
 /* ---- preprocessing (data_loader.hpp:55-86) ----------------------------- */
 
 /* Global min/max over isnormal() samples (data_loader.hpp:55-74).

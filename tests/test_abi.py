@@ -35,7 +35,17 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, n)
     # every binding in the Python front end names a declared symbol
     assert set(_lib.exported_symbols()) <= set(declared_functions())
-    assert lib.xpr_abi_version() == 1
+    assert lib.xpr_abi_version() == 2
+
+
+def test_proj_args_layout_matches_the_header():
+    """xpr_proj_args as the kernels take it by value: interp fills the
+    padding before view_index (80 bytes, interp at 68, view_index at 72)."""
+    import ctypes as C
+    from xpulse_recon_amd import _lib
+    assert C.sizeof(_lib.ProjArgs) == 80
+    assert _lib.ProjArgs.interp.offset == 68
+    assert _lib.ProjArgs.view_index.offset == 72
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):

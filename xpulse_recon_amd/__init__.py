@@ -42,10 +42,15 @@ def dtype_code(t) -> int:
 
 
 def acc_bits(t) -> int:
+    """The accumulator a sums tensor stands for: 32- or 64-bit integers are
+    Q.22 sums (XPR_ACC_U32 / XPR_ACC_U64), float32 the opt-in float sums
+    (XPR_ACC_F32)."""
     import torch
-    m = {torch.int32: 32, torch.uint32: 32, torch.int64: 64, torch.uint64: 64}
+    m = {torch.int32: _lib.XPR_ACC_U32, torch.uint32: _lib.XPR_ACC_U32,
+         torch.int64: _lib.XPR_ACC_U64, torch.uint64: _lib.XPR_ACC_U64,
+         torch.float32: _lib.XPR_ACC_F32}
     if t.dtype not in m:
-        raise XprError(f"sums must be a 32- or 64-bit integer tensor, got {t.dtype}")
+        raise XprError(f"sums must be a 32- or 64-bit integer or a float32 tensor, got {t.dtype}")
     return m[t.dtype]
 
 
@@ -73,7 +78,8 @@ def _check_sizes(volume, det, projdata, a: ProjArgs):
 
 
 def forward(volume, sums, projdata, args: ProjArgs, stream=None) -> None:
-    """Forward projection: sums[view,row,col] += trunc(v*2^22) (KernelForward.cl:10-55)."""
+    """Forward projection: sums[view,row,col] += trunc(v*2^22) (KernelForward.cl:10-55);
+    float32 sums (the opt-in float accumulator) add v itself."""
     _need_cuda(volume, sums, projdata)
     _check_sizes(volume, sums, projdata, args)
     _lib.call("xpr_forward", _stream(stream), _ptr(volume), dtype_code(volume), _ptr(sums),

@@ -36,12 +36,17 @@ class ProjArgs(C.Structure):
         ("line_rows", C.c_int32),
         ("origin", C.c_float * 3),
         ("voxel", C.c_float * 3),
+        ("interp", C.c_int32),
         ("view_index", C.c_void_p),
     ]
 
 
+XPR_INTERP_NEAREST, XPR_INTERP_BILINEAR = 0, 1
+XPR_ACC_U32, XPR_ACC_U64, XPR_ACC_F32 = 32, 64, -32
+
+
 def proj_args(det_width, n_views, modules, vmin, vmax, line_offset, line_rows, origin, voxel,
-              view_index=None):
+              view_index=None, interp=XPR_INTERP_NEAREST):
     a = ProjArgs()
     a.det_width, a.n_views, a.modules = int(det_width), int(n_views), int(modules)
     for i in range(3):
@@ -49,6 +54,7 @@ def proj_args(det_width, n_views, modules, vmin, vmax, line_offset, line_rows, o
         a.origin[i], a.voxel[i] = float(origin[i]), float(voxel[i])
     a.line_offset, a.line_rows = int(line_offset), int(line_rows)
     a.view_index = None if view_index is None else int(view_index)
+    a.interp = int(interp)
     return a
 
 

@@ -1,5 +1,7 @@
 #include "params.hpp"
 
+#include "../../../include/xpr_hip.h"
+
 #include <fstream>
 #include <sstream>
 #include <stdexcept>
@@ -166,8 +168,22 @@ Parameters parameters_from_json(const Json &root)
             else bad("xpr.storage", "one of f32, f16, bf16");
         }
         rd(x, "final_fp32", q.final_fp32, "xpr");
-        rdn(x, "accumulator", q.accumulator, "xpr");
-        if (q.accumulator != 32 && q.accumulator != 64) bad("xpr.accumulator", "32 or 64");
+        if (const Json *a = x.get("accumulator")) {
+            if (a->kind == Json::String) {
+                if (a->str != "f32") bad("xpr.accumulator", "32, 64 or \"f32\"");
+                q.accumulator = XPR_ACC_F32;
+            } else {
+                rdn(x, "accumulator", q.accumulator, "xpr");
+                if (q.accumulator != 32 && q.accumulator != 64)
+                    bad("xpr.accumulator", "32, 64 or \"f32\"");
+            }
+        }
+        if (const Json *s = x.get("interp")) {
+            if (s->kind != Json::String) bad("xpr.interp", "expected a string");
+            if (s->str == "nearest") q.interp = XPR_INTERP_NEAREST;
+            else if (s->str == "bilinear") q.interp = XPR_INTERP_BILINEAR;
+            else bad("xpr.interp", "nearest or bilinear");
+        }
         rd(x, "jitter", q.jitter, "xpr");
         rdn(x, "jitter_seed", q.jitter_seed, "xpr");
         rd(x, "geometry_fp64", q.geometry_fp64, "xpr");
@@ -250,7 +266,9 @@ Json parameters_to_json(const Parameters &p)
     static const char *st[] = {"f32", "f16", "bf16"};
     x.set("storage", Json::string(st[p.xpr.storage]));
     x.set("final_fp32", Json::boolean(p.xpr.final_fp32));
-    x.set("accumulator", Json::number(p.xpr.accumulator));
+    x.set("accumulator", p.xpr.accumulator == XPR_ACC_F32 ? Json::string("f32")
+                                                          : Json::number(p.xpr.accumulator));
+    x.set("interp", Json::string(p.xpr.interp == XPR_INTERP_BILINEAR ? "bilinear" : "nearest"));
     x.set("jitter", Json::boolean(p.xpr.jitter));
     x.set("jitter_seed", Json::number(double(p.xpr.jitter_seed)));
     x.set("geometry_fp64", Json::boolean(p.xpr.geometry_fp64));

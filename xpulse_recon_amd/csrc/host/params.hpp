@@ -65,7 +65,10 @@ struct IprSection {                         // parse only (modality not built)
 struct XprSection {
     int storage = 0;                 // 0 f32, 1 f16, 2 bf16 for volume and projections
     bool final_fp32 = true;          // last sub-iteration with FP32 storage when storage != f32
-    int accumulator = 64;            // Q.22 sums width: 64, or 32 (the reference's uint32)
+    int accumulator = 64;            // detector sums: 64 or 32 (the reference's uint32) bit
+                                     // Q.22, or XPR_ACC_F32 ("f32": float sums, opt-in)
+    int interp = 0;                  // backprojector sampling: XPR_INTERP_NEAREST (the
+                                     // reference's) or XPR_INTERP_BILINEAR ("bilinear", opt-in)
     bool jitter = true;              // per-sub-iteration origin jitter U[0, vx)^3
     int64_t jitter_seed = -1;        // -1: std::random_device like the reference
     bool geometry_fp64 = false;      // FP64 ProjData construction (default: float32 as glm)

@@ -487,6 +487,11 @@ Reconstruction::Reconstruction(Parameters &p) : p_(p)
     const uint64_t reserve = 512ull << 20;                 // gpu/context.hpp:104-108
     budget_ = p.xpr.device_budget_gb > 0.0 ? uint64_t(p.xpr.device_budget_gb * 1e9)
                                            : (fr > reserve ? fr - reserve : 0);
+    // the arithmetic of the run: Q.22 sums and nearest-pixel sampling are the
+    // reference's; float sums and bilinear sampling are opt-in
+    note("accumulator", p.xpr.accumulator == XPR_ACC_F32 ? Json::string("f32")
+                                                        : Json::number(p.xpr.accumulator));
+    note("interp", Json::string(p.xpr.interp == XPR_INTERP_BILINEAR ? "bilinear" : "nearest"));
     if (p.xpr.jitter_seed >= 0) {
         rng_.seed(uint32_t(p.xpr.jitter_seed));
     } else {
@@ -638,10 +643,13 @@ void write_final(Dataset &ds, xpr_stream s, const void *dev, int dt, int64_t l0,
     }
 }
 
-void fill_sums(xpr_stream s, DevBuf &sums, int acc, uint64_t value, uint64_t n)
+// detector sums set to 1.0 (one: the first pass, reconstruction_n.hpp:60)
+// or 0, in the accumulator's own representation
+void fill_sums(xpr_stream s, DevBuf &sums, int acc, bool one, uint64_t n)
 {
-    if (acc == 64) check(xpr_fill_u64(s, sums.as<uint64_t>(), value, n), "fill");
-    else check(xpr_fill_u32(s, sums.as<uint32_t>(), uint32_t(value), n), "fill");
+    if (acc == XPR_ACC_U64) check(xpr_fill_u64(s, sums.as<uint64_t>(), one ? 1ull << 22 : 0, n), "fill");
+    else if (acc == XPR_ACC_F32) check(xpr_fill(s, sums.get(), XPR_F32, one ? 1.0f : 0.0f, n), "fill");
+    else check(xpr_fill_u32(s, sums.as<uint32_t>(), one ? 1u << 22 : 0u, n), "fill");
 }
 
 } // namespace
@@ -667,7 +675,7 @@ public:
         const uint64_t sv = dtype_bytes(storage()), V = uint64_t(ds_->vwidth() * ds_->vwidth() * ds_->vheight());
         const uint64_t P = uint64_t(ds_->width() * ds_->height());
         const uint64_t As = uint64_t(ds_->concurrent()), S = uint64_t(p_.recon.sit);
-        const uint64_t acc = uint64_t(p_.xpr.accumulator / 8);
+        const uint64_t acc = acc_bytes(p_.xpr.accumulator);
         uint64_t need = V * sv + S * As * P * sv + As * P * (acc + sv) + ds_->projdata().size() * 4;
         if (storage() != XPR_F32 && p_.xpr.final_fp32) need += V * 4 + As * P * 8;
         return need;
@@ -700,7 +708,7 @@ public:
             err32.alloc(uint64_t(As * P) * 4);
             vol32.alloc(V * 4);
         }
-        DevBuf vol(V * dtype_bytes(st)), sums(uint64_t(As * P) * uint64_t(p_.xpr.accumulator / 8)),
+        DevBuf vol(V * dtype_bytes(st)), sums(uint64_t(As * P) * acc_bytes(p_.xpr.accumulator)),
             err(uint64_t(As * P) * dtype_bytes(st));
         Event ready, up32;
         std::vector<Event> up(static_cast<size_t>(S));
@@ -714,7 +722,7 @@ public:
                 check(xpr_event_record(up32.e, copy_), "event");
             }
         };
-        Operators ops(stream_, p_.xpr.accumulator);
+        Operators ops(stream_, p_.xpr.accumulator, p_.xpr.interp);
         ops.setSumImagesBuffer(sums.get());
         ops.setErrorBuffer(err.get(), st);
         ops.setVolumeBuffer(vol.get(), st);
@@ -734,7 +742,7 @@ public:
             tune_shapes(static_cast<char *>(vol.get()) + size_t(l0) * size_t(W * W) * dtype_bytes(st), st,
                         sums.get(), err.get(), st, pds[0].as<float>(), ta);
         }
-        fill_sums(stream_, sums, p_.xpr.accumulator, 1ull << 22, uint64_t(As * P));
+        fill_sums(stream_, sums, p_.xpr.accumulator, true, uint64_t(As * P));
         upload(0);
         int vdt = st;
         Event e0, e1, fb, fe, bb, be;
@@ -767,7 +775,7 @@ public:
                 check(xpr_event_record(bb.e, stream_), "event");
                 ops.backward();
                 check(xpr_event_record(be.e, stream_), "event");
-                fill_sums(stream_, sums, p_.xpr.accumulator, 0, uint64_t(As * P));
+                fill_sums(stream_, sums, p_.xpr.accumulator, false, uint64_t(As * P));
                 check(xpr_event_record(e1.e, stream_), "event");
                 if (mit == 0 && s + 1 < S) upload(s + 1);
                 check(xpr_event_sync(e1.e), "sync");
@@ -812,7 +820,7 @@ public:
         layer_ = size_t(W * W);
         nslab_ = (H + hs_ - 1) / hs_;
         const size_t slab_n = size_t(hs_) * layer_;
-        DevBuf sums(uint64_t(As * P) * uint64_t(p_.xpr.accumulator / 8)), err(uint64_t(As * P) * sb_);
+        DevBuf sums(uint64_t(As * P) * acc_bytes(p_.xpr.accumulator)), err(uint64_t(As * P) * sb_);
         std::vector<DevBuf> images(static_cast<size_t>(resident_ ? S : 1));
         for (auto &b : images) b.alloc(uint64_t(As * P) * sb_);
         // ProjData of every sub-iteration and, for method_1c, the views of each
@@ -867,7 +875,7 @@ public:
         cvol_ = vol_.get();
         cst_ = st_;
         csb_ = sb_;
-        Operators ops(stream_, p_.xpr.accumulator);
+        Operators ops(stream_, p_.xpr.accumulator, p_.xpr.interp);
         ops.setSumImagesBuffer(sums.get());
         ops.setErrorBuffer(err.get(), st_);
         ops.setImageParameters(ds.width(), ds.height(), As);
@@ -885,7 +893,7 @@ public:
             for (int i = 0; i < 3; ++i) { ta.origin[i] = ds.orig()[size_t(i)]; ta.voxel[i] = ds.vx(); }
             tune_shapes(v, st_, sums.get(), err.get(), st_, pds_[0].as<float>(), ta);
         }
-        fill_sums(stream_, sums, p_.xpr.accumulator, 1ull << 22, uint64_t(As * P));
+        fill_sums(stream_, sums, p_.xpr.accumulator, true, uint64_t(As * P));
         // the slab buffers are free once the tuning launches above are done
         check(xpr_event_record(freed_[0].e, stream_), "event");
         check(xpr_event_record(freed_[1].e, stream_), "event");
@@ -944,7 +952,7 @@ public:
                 check(xpr_event_record(bb.e, stream_), "event");
                 pass(true, first, last, s, ops);
                 check(xpr_event_record(be.e, stream_), "event");
-                fill_sums(stream_, sums, p_.xpr.accumulator, 0, uint64_t(As * P));
+                fill_sums(stream_, sums, p_.xpr.accumulator, false, uint64_t(As * P));
                 check(xpr_event_record(e1.e, stream_), "event");
                 if (resident_ && mit == 0 && s + 1 < S) upload(s + 1);
                 check(xpr_event_sync(e1.e), "sync");
@@ -1001,7 +1009,7 @@ private:
     {
         const uint64_t sv = dtype_bytes(storage()), W = uint64_t(ds_->vwidth()), H = uint64_t(ds_->vheight());
         const uint64_t P = uint64_t(ds_->width() * ds_->height()), As = uint64_t(ds_->concurrent());
-        const uint64_t S = uint64_t(p_.recon.sit), acc = uint64_t(p_.xpr.accumulator / 8);
+        const uint64_t S = uint64_t(p_.recon.sit), acc = acc_bytes(p_.xpr.accumulator);
         const uint64_t layer = W * W;
         // the preprocessing's two upload slots of up to 16 MiB (or one
         // sub-iteration's views), when the projections come from the host
@@ -1360,10 +1368,10 @@ public:
             const int64_t l0 = c.vOffset - c.vPadTop, l1 = c.vOffset + c.vSize + c.vPadBot;
             const uint64_t nv = uint64_t(l1 - l0) * layer, npx = uint64_t(c.iSize * dw * As);
             DevBuf vol(nv * dtype_bytes(st)), img(npx * dtype_bytes(st)),
-                sums(npx * uint64_t(p_.xpr.accumulator / 8)), err(npx * dtype_bytes(st));
+                sums(npx * acc_bytes(p_.xpr.accumulator)), err(npx * dtype_bytes(st));
             check(xpr_fill(stream_, vol.get(), st, 1.0f, nv), "fill");
-            fill_sums(stream_, sums, p_.xpr.accumulator, 1ull << 22, npx);
-            Operators ops(stream_, p_.xpr.accumulator);
+            fill_sums(stream_, sums, p_.xpr.accumulator, true, npx);
+            Operators ops(stream_, p_.xpr.accumulator, p_.xpr.interp);
             ops.setSumImagesBuffer(sums.get());
             ops.setErrorBuffer(err.get(), st);
             ops.setImagesBuffer(img.get(), st);
@@ -1389,7 +1397,7 @@ public:
                     check(xpr_event_record(bb.e, stream_), "event");
                     ops.backward();
                     check(xpr_event_record(be.e, stream_), "event");
-                    fill_sums(stream_, sums, p_.xpr.accumulator, 0, npx);
+                    fill_sums(stream_, sums, p_.xpr.accumulator, false, npx);
                     check(xpr_event_record(e1.e, stream_), "event");
                     check(xpr_event_sync(e1.e), "sync");
                     float ms = 0, fms = -1.0f, bms = 0;

@@ -163,9 +163,15 @@ private:
 // Operator API: the reference's positional kernel state, kept on the host
 // and handed by value to every stateless C-ABI call.
 // ---------------------------------------------------------------------------
+// bytes per detector sum of an accumulator (XPR_ACC_*)
+inline uint64_t acc_bytes(int acc) { return acc == XPR_ACC_U64 ? 8u : 4u; }
+
 class Operators {
 public:
-    Operators(xpr_stream s, int acc_bits) : stream_(s), acc_(acc_bits) {}
+    Operators(xpr_stream s, int acc_bits, int interp = XPR_INTERP_NEAREST) : stream_(s), acc_(acc_bits)
+    {
+        a_.interp = interp;
+    }
     void setImagesBuffer(const void *img, int dtype) { img_ = img; img_dt_ = dtype; }
     void setSumImagesBuffer(void *sums) { sums_ = sums; }
     void setErrorBuffer(void *err, int dtype) { err_ = err; err_dt_ = dtype; }

@@ -111,7 +111,7 @@ void job_bytes(const xpr_job &j, uint64_t &resident, uint64_t &streamed)
     const uint64_t H = uint64_t(g.dheight) + uint64_t(std::fabs(g.heli_offset / vx));
     const uint64_t P = uint64_t(g.dwidth * g.dheight), S = uint64_t(j.sit);
     const uint64_t As = uint64_t(g.projections) / S;
-    const uint64_t sv = j.storage == XPR_F32 ? 4 : 2, acc = uint64_t(j.acc_bits / 8);
+    const uint64_t sv = j.storage == XPR_F32 ? 4 : 2, acc = acc_bytes(j.acc_bits);
     const uint64_t pd = uint64_t(g.projections * g.module_number) * 80;
     const uint64_t V = W * W * H;
     resident = V * sv + S * As * P * sv + As * P * (acc + sv) + pd + P * 8;
@@ -737,8 +737,8 @@ int xpr_server_submit(xpr_server *sv, const xpr_job *job, uint64_t *job_id)
         return xprint_fail(XPR_E_INVALID, "job without projections or output");
     if (job->sit < 1 || job->it < 1 || job->geometry.projections / job->sit < 1)
         return xprint_fail(XPR_E_INVALID, "job: it, sit or projection count");
-    if (job->acc_bits != XPR_ACC_U32 && job->acc_bits != XPR_ACC_U64)
-        return xprint_fail(XPR_E_INVALID, "job: acc_bits must be 32 or 64");
+    if (job->acc_bits != XPR_ACC_U32 && job->acc_bits != XPR_ACC_U64 && job->acc_bits != XPR_ACC_F32)
+        return xprint_fail(XPR_E_INVALID, "job: acc_bits must be XPR_ACC_U32, XPR_ACC_U64 or XPR_ACC_F32");
     if (job->storage < XPR_F32 || job->storage > XPR_BF16)
         return xprint_fail(XPR_E_UNSUPPORTED, "job: storage type");
     *job_id = sv->impl->submit(*job);

@@ -31,6 +31,19 @@ namespace {
 // group as well (an upright detector), so does the column u (a zero X gives
 // the same pixel and bounds result with either sign).  Other groups take the
 // canonical projection per voxel.
+//
+// Bilinear sampling (opt-in, BILIN): the same projection and the same
+// voxel-views as the nearest-pixel test accepts, each sampling the error
+// image at the continuous (u, v): taps columns floor u and floor u + 1 and
+// rows floor v and floor v + 1, each clamped to the accepted pixels
+// [ceil(vpl), ceil(vpr) - 1] (rows likewise; the bounds clamped to the
+// detector and the rows held), weights fx = u - floor u, fy = v - floor v,
+// three lerps:
+//   top = fmaf(fx, E01 - E00, E00), bot = fmaf(fx, E11 - E10, E10),
+//   sample = fmaf(fy, bot - top, top)     (E_rc: row +r, column +c)
+// so equal taps give that value exactly, and an edge pixel's value holds over
+// the half pixel beyond its centre.  The accepted pixels come from the
+// thresholds: ceil(ulo) and floor(uhi).
 // ---------------------------------------------------------------------------
 // LDS record of one view for one wave: k3 (the w row term at the group's first
 // layer), the x coefficients of rows 0, 1, 3, and the L row terms of u and of
@@ -139,6 +152,27 @@ __device__ __forceinline__ RowSetup<L> row_setup(const float *__restrict__ p,
     return r;
 }
 
+// One bilinear sample of the view's error image at (u, v), taps clamped to
+// the accepted columns [c0, c1] and rows [r0, r1].  The taps are read through
+// the indexed buffer, whose bound makes any index safe; the caller discards
+// the sample of a voxel-view that is out of bounds.
+template <typename ES>
+__device__ __forceinline__ float bilinear_sample(__amdgpu_buffer_rsrc_t rsrc, float u, float v,
+                                                 uint32_t dw, float c0, float c1, float r0,
+                                                 float r1)
+{
+    const float fu = floorf(u), fv = floorf(v);
+    const float fx = u - fu, fy = v - fv;       // exact: floor(u) is within a factor 2 of u
+    const uint32_t i0 = (uint32_t)(int32_t)fmaxf(fu, c0), i1 = (uint32_t)(int32_t)fminf(fu + 1.0f, c1);
+    const uint32_t j0 = __umul24((uint32_t)(int32_t)fmaxf(fv, r0), dw);
+    const uint32_t j1 = __umul24((uint32_t)(int32_t)fminf(fv + 1.0f, r1), dw);
+    const float e00 = sample_f32<ES>(rsrc, j0 + i0), e01 = sample_f32<ES>(rsrc, j0 + i1);
+    const float e10 = sample_f32<ES>(rsrc, j1 + i0), e11 = sample_f32<ES>(rsrc, j1 + i1);
+    const float top = __builtin_fmaf(fx, e01 - e00, e00);
+    const float bot = __builtin_fmaf(fx, e11 - e10, e10);
+    return __builtin_fmaf(fy, bot - top, top);
+}
+
 // Segment seg of one (layer group, depth) row of the backward, run by one
 // wave.  The per-view set-up is spread over the lanes (one view record each,
 // 64 at a time).  The view loop reads each record back from the wave's LDS
@@ -146,7 +180,7 @@ __device__ __forceinline__ RowSetup<L> row_setup(const float *__restrict__ p,
 // in order, so the table needs no barrier.  Every branch of a view leaves its
 // samples in smp[] and one accumulation follows, in view order, so the sums
 // equal the canonical ones.
-template <typename VS, typename ES, int J, int L, bool CROP>
+template <typename VS, typename ES, int J, int L, bool CROP, bool BILIN = false>
 __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
                                              const typename ES::T *__restrict__ err,
                                              const float *__restrict__ pd,
@@ -230,6 +264,50 @@ __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
                 (void *)(err + (uint64_t)view_of(a, view) * view_elems), (short)esz,
                 (int)view_elems, kBufferWord3);
             const uint32_t dw = a.det_width;
+            if constexpr (BILIN) {
+                const bool inside = (inside_mask >> i) & 1u;
+                n_inside += inside ? 1u : 0u;
+                const float ulo = lane_float(su.ulo, i), uhi = lane_float(su.uhi, i);
+                const float vlo = lane_float(su.vlo, i), vhi = lane_float(su.vhi, i);
+                // the accepted pixels: columns ceil(ulo) .. floor(uhi), rows likewise
+                const float c0 = ceilf(ulo), c1 = floorf(uhi), r0 = ceilf(vlo), r1 = floorf(vhi);
+                const bool fast = (fast_mask >> i) & 1u;
+                const bool wcol = (wcol_mask >> i) & 1u;
+                const float p13 = lane_float(su.p13, i), t3 = lane_float(su.t3, i);
+                auto rcp = [&](float W) {
+                    if (fast) {
+                        const float r0 = __builtin_amdgcn_rcpf(W);
+                        return __builtin_fmaf(__builtin_fmaf(-W, r0, 1.0f), r0, r0);
+                    }
+                    return recip(W);
+                };
+#pragma unroll
+                for (int j = 0; j < J; ++j) {
+                    const float rw = wcol ? rcp(__builtin_fmaf(c3x, px[j], k3)) : 0.0f;
+#pragma unroll
+                    for (int l = 0; l < L; ++l) {
+                        const float r = wcol ? rw
+                                             : rcp(__builtin_fmaf(c3x, px[j],
+                                                                  __builtin_fmaf(p13, py[l], t3)));
+                        const float u = __builtin_fmaf(c0x, px[j], k0[l]) * r;
+                        const float Y = __builtin_fmaf(c1x, px[j], k1[l]);
+                        const float v = CROP ? Y * r - line_off : Y * r;
+                        const float smp = bilinear_sample<ES>(rsrc, u, v, dw, c0, c1, r0, r1);
+                        if (inside) {
+                            sum[j][l] = sum[j][l] + smp;
+                        } else {
+                            const bool inb = (u >= ulo) & (u < uhi) & (v >= vlo) & (v < vhi);
+                            sum[j][l] = sum[j][l] + (inb ? smp : 0.0f);
+                            cnt[j][l] += inb ? 1u : 0u;
+                        }
+                    }
+                }
+                if (++mm == a.modules) {
+                    mm = 0;
+                    ++view;
+                }
+                continue;
+            }
             typename SampleOf<ES>::type smp[J][L];
             if ((inside_mask >> i) & 1u) {
                 ++n_inside;
@@ -359,7 +437,7 @@ __host__ __device__ __forceinline__ int64_t backward_rows(int32_t nl, int32_t nk
     return (int64_t)((nl + L - 1) / L) * nk;
 }
 
-template <typename VS, typename ES, int J, int L, bool CROP>
+template <typename VS, typename ES, int J, int L, bool CROP, bool BILIN = false>
 __global__ __launch_bounds__(kThreads) void k_backward(
     typename VS::T *__restrict__ vol, const typename ES::T *__restrict__ err,
     const float *__restrict__ pd, xpr_proj_args a, SlabShape s,
@@ -369,8 +447,9 @@ __global__ __launch_bounds__(kThreads) void k_backward(
     // segment blockIdx % nseg of the kWaves rows of row block blockIdx / nseg
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform
     const int64_t row = s.row_base + (int64_t)(blockIdx.x / s.nseg) * kWaves + wave;
-    backward_row<VS, ES, J, L, CROP>(vol, err, pd, a, s, row, (int)(blockIdx.x % s.nseg),
-                                     empty_count, rec + (threadIdx.x >> 6) * 64 * BwdRecord<L>::kQuads);
+    backward_row<VS, ES, J, L, CROP, BILIN>(vol, err, pd, a, s, row, (int)(blockIdx.x % s.nseg),
+                                            empty_count,
+                                            rec + (threadIdx.x >> 6) * 64 * BwdRecord<L>::kQuads);
 }
 
 } // namespace

@@ -247,6 +247,25 @@ __device__ __forceinline__ uint16_t sample_raw<BF16>(__amdgpu_buffer_rsrc_t r, u
     return struct_buffer_load_u16(r, index, 0, 0, 0);
 }
 
+// an indexed sample widened to float
+template <typename T>
+__device__ __forceinline__ float sample_f32(__amdgpu_buffer_rsrc_t r, uint32_t index);
+template <>
+__device__ __forceinline__ float sample_f32<F32>(__amdgpu_buffer_rsrc_t r, uint32_t index)
+{
+    return struct_buffer_load_f32(r, index, 0, 0, 0);
+}
+template <>
+__device__ __forceinline__ float sample_f32<F16>(__amdgpu_buffer_rsrc_t r, uint32_t index)
+{
+    return (float)__builtin_bit_cast(_Float16, struct_buffer_load_u16(r, index, 0, 0, 0));
+}
+template <>
+__device__ __forceinline__ float sample_f32<BF16>(__amdgpu_buffer_rsrc_t r, uint32_t index)
+{
+    return __builtin_bit_cast(float, (uint32_t)struct_buffer_load_u16(r, index, 0, 0, 0) << 16);
+}
+
 template <typename T>
 __device__ __forceinline__ float add_sample(float sum, typename SampleOf<T>::type s);
 template <>
@@ -284,6 +303,13 @@ struct SlabShape {
     int64_t nrows;         // wave rows in the slab (backward: layer groups x depths)
     int32_t inside = 1;    // 1: backward views wholly in bounds skip the per-voxel test
 };
+
+// The value of a detector sum: FIXED_TO_FLOAT of a Q.22 integer
+// (constants.cl:21), or the float sum itself (XPR_ACC_F32).
+template <typename AccT>
+__device__ __forceinline__ float sum_value(AccT s) { return (float)s / kQ22One; }
+template <>
+__device__ __forceinline__ float sum_value<float>(float s) { return s; }
 
 // FLOAT_TO_FIXED (constants.cl:22), saturating like the GPU conversion.
 __device__ __forceinline__ uint32_t q22_u32(float v)

@@ -24,7 +24,7 @@ __global__ __launch_bounds__(256) void k_error(const typename IS::T *__restrict_
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const float ref = IS::load(img, i);
-        const float got = (float)sums[i] / kQ22One;                 // FIXED_TO_FLOAT
+        const float got = sum_value(sums[i]);                       // FIXED_TO_FLOAT
         const float e = logf(fmaxf(ref / fmaxf(got, kQ22Eps), kQ22Eps)) * weight;
         ES::store(err, i, e);
     }
@@ -36,7 +36,7 @@ __global__ __launch_bounds__(256) void k_q22_to_float(const AccT *__restrict__ s
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        out[i] = (float)sums[i] / kQ22One;
+        out[i] = sum_value(sums[i]);
 }
 
 template <typename T>

@@ -30,6 +30,12 @@ namespace {
 // view whose W does not change across the layers computes W and its
 // reciprocal once per column (see the coherence modes below).
 //
+// Float sums (AccT = float, the opt-in XPR_ACC_F32): the same bricks, tiles
+// and paths with float tile entries (LDS float atomics) and global float
+// atomics; a voxel's 32-bit word then carries its float value instead of its
+// Q.22 integer.  The additions happen in whatever order the atomics land, so
+// the sums are within float rounding of the exact sum, not bit-reproducible.
+//
 // The footprint, bounds thresholds, matrix rows, order and fast-path flags
 // of a chunk of 32 views are computed once per workgroup (8 corner lanes per
 // view) into LDS.
@@ -47,11 +53,19 @@ struct BrickShape {
     int64_t nbricks;         // bricks in the slab
 };
 
+// (float: the hardware global_atomic_add_f32; the sums are coarse-grained
+// device memory, where it is supported)
 template <typename AccT>
 __device__ __forceinline__ void global_add(AccT *p, AccT v)
 {
     __hip_atomic_fetch_add((Glob<AccT> *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+
+// A voxel's 32-bit word added to a float tile entry or float sum: the float
+// whose bits it holds.  (The integer tiles and sums add the word itself; the
+// float forms sit in `if constexpr` branches so the integer kernels keep
+// exactly their own code.)
+__device__ __forceinline__ float word_float(uint32_t q) { return __builtin_bit_cast(float, q); }
 
 enum : uint32_t {
     kBinSkip = 1u,     // no voxel of the brick lands inside the view's bounds
@@ -310,8 +324,12 @@ template <typename TileT>
 __device__ __forceinline__ void lds_add(uint32_t addr, uint32_t q)
 {
     using L = __attribute__((address_space(3))) TileT;
-    __hip_atomic_fetch_add((L *)(uintptr_t)addr, (TileT)q, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_WORKGROUP);
+    if constexpr (std::is_same<TileT, float>::value)
+        __hip_atomic_fetch_add((L *)(uintptr_t)addr, word_float(q), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+    else
+        __hip_atomic_fetch_add((L *)(uintptr_t)addr, (TileT)q, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // one voxel-view: its Q.22 value goes to its pixel's tile entry (inside: the
@@ -333,7 +351,10 @@ __device__ __forceinline__ void deposit(float u, float v, uint32_t q, const View
         const uint32_t tv = (uint32_t)pixel_of(v) - (uint32_t)t.v0;
         const bool hit = (tu < (uint32_t)t.tw) & (tv < (uint32_t)t.th);
         const uint32_t e = __umul24(tv, (uint32_t)t.tw) + tu;
-        atomicAdd(&t.tl[hit ? e : t.sk], (TileT)(hit ? q : 0u));
+        if constexpr (std::is_same<TileT, float>::value)
+            atomicAdd(&t.tl[hit ? e : t.sk], word_float(hit ? q : 0u));
+        else
+            atomicAdd(&t.tl[hit ? e : t.sk], (TileT)(hit ? q : 0u));
     } else {
         const bool inb = (u >= c.ulo) & (u < c.uhi) & (v >= c.vlo) & (v < c.vhi);
         const uint32_t iu = (uint32_t)pixel_of(u), iv = (uint32_t)pixel_of(v);
@@ -341,9 +362,15 @@ __device__ __forceinline__ void deposit(float u, float v, uint32_t q, const View
         const uint32_t tv = iv - (uint32_t)t.v0;
         const bool hit = inb & t.use_tile & (tu < (uint32_t)t.tw) & (tv < (uint32_t)t.th);
         const uint32_t e = __umul24(tv, (uint32_t)t.tw) + tu;
-        atomicAdd(&t.tl[hit ? e : t.sk], (TileT)(hit ? q : 0u));
-        if (inb & !hit & (q != 0u))      // off the tile: direct global add
-            global_add(t.vsum + (__umul24(iv, t.det_width) + iu), (AccT)q);
+        if constexpr (std::is_same<TileT, float>::value) {
+            atomicAdd(&t.tl[hit ? e : t.sk], word_float(hit ? q : 0u));
+            if (inb & !hit & (q != 0u))  // off the tile: direct global add
+                global_add(t.vsum + (__umul24(iv, t.det_width) + iu), word_float(q));
+        } else {
+            atomicAdd(&t.tl[hit ? e : t.sk], (TileT)(hit ? q : 0u));
+            if (inb & !hit & (q != 0u))  // off the tile: direct global add
+                global_add(t.vsum + (__umul24(iv, t.det_width) + iu), (AccT)q);
+        }
     }
 }
 
@@ -601,7 +628,8 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
     const int32_t dk = lane & 15, xg = lane >> 4;     // depth-lane order
 
     // Q.22 voxel values in both lane orders (32 bits suffice unless some
-    // value is >= 1024), qx[d][l] and qd[xs][l].  Raw buffer loads from each
+    // value is >= 1024), qx[d][l] and qd[xs][l]; for float sums the float
+    // values' bits.  Raw buffer loads from each
     // layer's brick corner (x0, l0 + l, k0): out-of-slab voxels read 0 without
     // a branch, so all 32 loads are in flight together.
     uint32_t qx[kD][kWaves], qd[kD][kWaves];
@@ -638,13 +666,23 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
                 vd_[xs][l] = load_or_zero<VS>(rsrc, in ? off : kOutOfRange);
             }
         }
+        if constexpr (std::is_same<AccT, float>::value) {
 #pragma unroll
-        for (int d = 0; d < kD; ++d)
+            for (int d = 0; d < kD; ++d)
 #pragma unroll
-            for (int l = 0; l < kWaves; ++l) {
-                qx[d][l] = q22(vx_[d][l], true);
-                qd[d][l] = q22(vd_[d][l], false);
-            }
+                for (int l = 0; l < kWaves; ++l) {
+                    qx[d][l] = __builtin_bit_cast(uint32_t, vx_[d][l]);
+                    qd[d][l] = __builtin_bit_cast(uint32_t, vd_[d][l]);
+                }
+        } else {
+#pragma unroll
+            for (int d = 0; d < kD; ++d)
+#pragma unroll
+                for (int l = 0; l < kWaves; ++l) {
+                    qx[d][l] = q22(vx_[d][l], true);
+                    qd[d][l] = q22(vd_[d][l], false);
+                }
+        }
     }
     if (__syncthreads_or(big)) {
         forward_direct_u64<VS, AccT, CROP>(vol, sums, pd, a, s, x0, l0, k0);
@@ -661,7 +699,9 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
 #pragma unroll
             for (int l = 0; l < kWaves; ++l) wide |= qx[d][l] >= (1u << 20);
     }
-    if (sizeof(AccT) == 8 && __syncthreads_or(wide))
+    if constexpr (std::is_same<AccT, float>::value)
+        forward_views<VS, AccT, CROP, float>(sums, pd, a, s, x0, l0, k0, qx, qd, lds, bins);
+    else if (sizeof(AccT) == 8 && __syncthreads_or(wide))
         forward_views<VS, AccT, CROP, unsigned long long>(sums, pd, a, s, x0, l0, k0, qx, qd, lds, bins);
     else
         forward_views<VS, AccT, CROP, uint32_t>(sums, pd, a, s, x0, l0, k0, qx, qd, lds, bins);

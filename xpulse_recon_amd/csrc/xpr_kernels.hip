@@ -18,6 +18,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -85,9 +86,34 @@ int check_dtype(int dt)
     return XPR_OK;
 }
 
+// the projectors take xpr_proj_args by value: interp sits in what was the
+// padding before view_index, so the argument block kept its size and offsets
+static_assert(sizeof(xpr_proj_args) == 80 && offsetof(xpr_proj_args, interp) == 68 &&
+                  offsetof(xpr_proj_args, view_index) == 72,
+              "xpr_proj_args layout");
+
+int check_acc(int acc_bits)
+{
+    if (acc_bits != XPR_ACC_U32 && acc_bits != XPR_ACC_U64 && acc_bits != XPR_ACC_F32)
+        return fail(XPR_E_INVALID, "acc_bits must be XPR_ACC_U32, XPR_ACC_U64 or XPR_ACC_F32");
+    return XPR_OK;
+}
+
+// bytes per detector sum
+uint64_t acc_bytes(int acc_bits) { return acc_bits == XPR_ACC_U64 ? 8u : 4u; }
+
+// the acc_bits value of a sums type (tuning-table key)
+template <typename AccT>
+constexpr int acc_code()
+{
+    return std::is_same<AccT, float>::value ? XPR_ACC_F32 : (int)(8 * sizeof(AccT));
+}
+
 int check_args(const xpr_proj_args *a, SlabShape &s)
 {
     if (a == nullptr) return fail(XPR_E_INVALID, "null xpr_proj_args");
+    if (a->interp != XPR_INTERP_NEAREST && a->interp != XPR_INTERP_BILINEAR)
+        return fail(XPR_E_INVALID, "interp must be XPR_INTERP_NEAREST or XPR_INTERP_BILINEAR");
     const int64_t nx = (int64_t)a->vmax[0] - a->vmin[0];
     const int64_t nl = (int64_t)a->vmax[1] - a->vmin[1];
     const int64_t nk = (int64_t)a->vmax[2] - a->vmin[2];
@@ -214,20 +240,20 @@ int pick_shape(const int (&cands)[N], const float (&ms)[N], int fallback)
 // per lane so one 64*J segment leaves few lanes idle
 int backward_default_j(int32_t nx) { return nx <= 128 ? kBwdL : kBwdJ * kBwdL; }
 
-template <typename VS, typename ES, int J, int L>
+template <typename VS, typename ES, int J, int L, bool BILIN = false>
 int backward_launch(hipStream_t st, void *vol, const void *err, const float *pd,
                     const xpr_proj_args &a, SlabShape s, uint64_t *empty)
 {
     const bool crop = a.line_offset != 0;
     return launch_rows(s, J, L, [&](unsigned blocks, const SlabShape &t) {
         if (crop)
-            hipLaunchKernelGGL((k_backward<VS, ES, J, L, true>), dim3(blocks), dim3(kThreads), 0,
-                               st, (typename VS::T *)vol, (const typename ES::T *)err, pd, a, t,
-                               (unsigned long long *)empty);
+            hipLaunchKernelGGL((k_backward<VS, ES, J, L, true, BILIN>), dim3(blocks),
+                               dim3(kThreads), 0, st, (typename VS::T *)vol,
+                               (const typename ES::T *)err, pd, a, t, (unsigned long long *)empty);
         else
-            hipLaunchKernelGGL((k_backward<VS, ES, J, L, false>), dim3(blocks), dim3(kThreads), 0,
-                               st, (typename VS::T *)vol, (const typename ES::T *)err, pd, a, t,
-                               (unsigned long long *)empty);
+            hipLaunchKernelGGL((k_backward<VS, ES, J, L, false, BILIN>), dim3(blocks),
+                               dim3(kThreads), 0, st, (typename VS::T *)vol,
+                               (const typename ES::T *)err, pd, a, t, (unsigned long long *)empty);
     });
 }
 
@@ -240,6 +266,9 @@ int backward_impl(hipStream_t st, void *vol, const void *err, const float *pd,
         return fail(XPR_E_INVALID, "a detector view above 2 GiB");
     // tuning knob: XPR_BACKWARD_INSIDE=0 keeps the canonical bounds test in every view
     if (const char *e = std::getenv("XPR_BACKWARD_INSIDE")) s.inside = std::atoi(e) != 0;
+    // bilinear sampling (opt-in): the default shape only
+    if (a.interp == XPR_INTERP_BILINEAR)
+        return backward_launch<VS, ES, kBwdJ, kBwdL, true>(st, vol, err, pd, a, s, empty);
     if constexpr (std::is_same<VS, ES>::value) {       // the schedules' dtype pairs: all J compiled
         if (J < 0) {
             // tuning knob: force the voxels per lane
@@ -283,7 +312,7 @@ int forward_impl(hipStream_t st, const void *vol, void *sums, const float *pd,
     const int64_t nbl = (s.nl + kWaves - 1) / kWaves;
     b.nbricks = (int64_t)b.nbx * b.nbk * nbl;
     if (cap < 0 && !std::getenv("XPR_FORWARD_TILE_CAP"))
-        cap = tune_lookup(tune_key(kTuneForward, VS::code, (int)sizeof(AccT), a, s));
+        cap = tune_lookup(tune_key(kTuneForward, VS::code, acc_code<AccT>(), a, s));
     b.cap = cap < 0 ? forward_tile_cap() : std::min(cap, kTileCap);
     // tuning knob: XPR_FORWARD_EXACT=0 keeps the canonical bounds test in every view
     const char *ex = std::getenv("XPR_FORWARD_EXACT");
@@ -336,7 +365,29 @@ int error_impl(hipStream_t st, const void *img, const void *sums, int acc_bits, 
 {
     if (acc_bits == XPR_ACC_U32)
         return error_impl2<IS, uint32_t>(st, img, sums, err, err_dtype, n, w);
+    if (acc_bits == XPR_ACC_F32)
+        return error_impl2<IS, float>(st, img, sums, err, err_dtype, n, w);
     return error_impl2<IS, uint64_t>(st, img, sums, err, err_dtype, n, w);
+}
+
+// forward launch for the sums type acc_bits names (cap as forward_impl)
+template <typename AccT>
+int forward_dtype(hipStream_t st, const void *vol, int vol_dtype, void *sums, const float *pd,
+                  const xpr_proj_args &a, const SlabShape &s, int cap)
+{
+    switch (vol_dtype) {
+    case XPR_F32: return forward_impl<F32, AccT>(st, vol, sums, pd, a, s, cap);
+    case XPR_F16: return forward_impl<F16, AccT>(st, vol, sums, pd, a, s, cap);
+    default: return forward_impl<BF16, AccT>(st, vol, sums, pd, a, s, cap);
+    }
+}
+
+int forward_dispatch(hipStream_t st, const void *vol, int vol_dtype, void *sums, int acc_bits,
+                     const float *pd, const xpr_proj_args &a, const SlabShape &s, int cap = -1)
+{
+    if (acc_bits == XPR_ACC_U32) return forward_dtype<uint32_t>(st, vol, vol_dtype, sums, pd, a, s, cap);
+    if (acc_bits == XPR_ACC_F32) return forward_dtype<float>(st, vol, vol_dtype, sums, pd, a, s, cap);
+    return forward_dtype<uint64_t>(st, vol, vol_dtype, sums, pd, a, s, cap);
 }
 
 template <typename DS>
@@ -755,23 +806,10 @@ int xpr_forward(xpr_stream s, const void *volume, int vol_dtype, void *sums, int
     SlabShape sh;
     if (int rc = check_args(a, sh)) return rc;
     if (int rc = check_dtype(vol_dtype)) return rc;
-    if (acc_bits != XPR_ACC_U32 && acc_bits != XPR_ACC_U64)
-        return fail(XPR_E_INVALID, "acc_bits must be 32 or 64");
+    if (int rc = check_acc(acc_bits)) return rc;
     if (a->n_views == 0 || sh.nx == 0 || sh.nrows == 0) return XPR_OK;
     if (!volume || !sums || !projdata) return fail(XPR_E_INVALID, "null buffer");
-    hipStream_t st = (hipStream_t)s;
-    if (acc_bits == XPR_ACC_U32) {
-        switch (vol_dtype) {
-        case XPR_F32: return forward_impl<F32, uint32_t>(st, volume, sums, projdata, *a, sh);
-        case XPR_F16: return forward_impl<F16, uint32_t>(st, volume, sums, projdata, *a, sh);
-        default: return forward_impl<BF16, uint32_t>(st, volume, sums, projdata, *a, sh);
-        }
-    }
-    switch (vol_dtype) {
-    case XPR_F32: return forward_impl<F32, uint64_t>(st, volume, sums, projdata, *a, sh);
-    case XPR_F16: return forward_impl<F16, uint64_t>(st, volume, sums, projdata, *a, sh);
-    default: return forward_impl<BF16, uint64_t>(st, volume, sums, projdata, *a, sh);
-    }
+    return forward_dispatch((hipStream_t)s, volume, vol_dtype, sums, acc_bits, projdata, *a, sh);
 }
 
 int xpr_error(xpr_stream s, const void *images, int img_dtype, const void *sums, int acc_bits,
@@ -779,11 +817,10 @@ int xpr_error(xpr_stream s, const void *images, int img_dtype, const void *sums,
 {
     if (int rc = check_dtype(img_dtype)) return rc;
     if (int rc = check_dtype(err_dtype)) return rc;
-    if (acc_bits != XPR_ACC_U32 && acc_bits != XPR_ACC_U64)
-        return fail(XPR_E_INVALID, "acc_bits must be 32 or 64");
+    if (int rc = check_acc(acc_bits)) return rc;
     if (n == 0) return XPR_OK;
     if (!images || !sums || !err) return fail(XPR_E_INVALID, "null buffer");
-    if (err == sums && !(acc_bits == XPR_ACC_U32 && err_dtype == XPR_F32))
+    if (err == sums && !(acc_bits != XPR_ACC_U64 && err_dtype == XPR_F32))
         return fail(XPR_E_INVALID, "in-place error needs 32-bit sums and f32 error");
     hipStream_t st = (hipStream_t)s;
     switch (img_dtype) {
@@ -828,19 +865,19 @@ int xpr_subiteration(xpr_stream s, void *volume, int vol_dtype, const void *imag
     if (int rc = check_dtype(vol_dtype)) return rc;
     if (int rc = check_dtype(img_dtype)) return rc;
     if (int rc = check_dtype(err_dtype)) return rc;
-    if (acc_bits != XPR_ACC_U32 && acc_bits != XPR_ACC_U64)
-        return fail(XPR_E_INVALID, "acc_bits must be 32 or 64");
+    if (int rc = check_acc(acc_bits)) return rc;
     if (a->view_index)
         return fail(XPR_E_INVALID, "xpr_subiteration: indexed views need xpr_forward / xpr_backward");
     if (!volume || !images || !projdata || !sums || !err) return fail(XPR_E_INVALID, "null buffer");
-    if (err == sums && !(acc_bits == XPR_ACC_U32 && err_dtype == XPR_F32))
+    if (err == sums && !(acc_bits != XPR_ACC_U64 && err_dtype == XPR_F32))
         return fail(XPR_E_INVALID, "in-place error needs 32-bit sums and f32 error");
     const uint64_t n = (uint64_t)a->n_views * (uint64_t)a->line_rows * (uint64_t)a->det_width;
     // sums start at 0 before a forward, at 1.0 (Q.22) on the first pass
     // (reconstruction_n.hpp:60,75): set here, so the workspace may hold anything
     const uint64_t init = first_pass ? (1ull << 22) : 0ull;
-    int rc = acc_bits == XPR_ACC_U32 ? xpr_fill_u32(s, (uint32_t *)sums, (uint32_t)init, n)
-                                     : xpr_fill_u64(s, (uint64_t *)sums, init, n);
+    int rc = acc_bits == XPR_ACC_U32   ? xpr_fill_u32(s, (uint32_t *)sums, (uint32_t)init, n)
+             : acc_bits == XPR_ACC_F32 ? xpr_fill(s, sums, XPR_F32, first_pass ? 1.0f : 0.0f, n)
+                                       : xpr_fill_u64(s, (uint64_t *)sums, init, n);
     if (rc == XPR_OK && !first_pass)
         rc = xpr_forward(s, volume, vol_dtype, sums, acc_bits, projdata, a);
     if (rc == XPR_OK)
@@ -857,12 +894,11 @@ int xpr_subiteration_static(xpr_stream s, void *volume, int vol_dtype, const voi
 {
     if (a == nullptr) return fail(XPR_E_INVALID, "null xpr_proj_args");
     if (int rc = check_dtype(err_dtype)) return rc;
-    if (acc_bits != XPR_ACC_U32 && acc_bits != XPR_ACC_U64)
-        return fail(XPR_E_INVALID, "acc_bits must be 32 or 64");
+    if (int rc = check_acc(acc_bits)) return rc;
     if (a->line_rows < 0) return fail(XPR_E_INVALID, "line_rows < 0");
     const uint64_t per_view = (uint64_t)a->line_rows * (uint64_t)a->det_width;
     const uint64_t n = per_view * (uint64_t)a->n_views;
-    const uint64_t pixel_bytes = (uint64_t)(acc_bits / 8) + (err_dtype == XPR_F32 ? 4u : 2u);
+    const uint64_t pixel_bytes = acc_bytes(acc_bits) + (err_dtype == XPR_F32 ? 4u : 2u);
     // per_view bounded before the products, so they cannot wrap
     const bool fits = a->n_views == 0 ||
                       (per_view <= XPR_STATIC_SCRATCH_BYTES &&
@@ -873,7 +909,7 @@ int xpr_subiteration_static(xpr_stream s, void *volume, int vol_dtype, const voi
                         std::to_string(per_view) + " pixels need more than the " +
                         std::to_string((unsigned long long)XPR_STATIC_SCRATCH_BYTES) +
                         " bytes of static scratch (use xpr_subiteration with a workspace)");
-    const uint64_t sums_bytes = (n * (uint64_t)(acc_bits / 8) + 255) & ~255ull;
+    const uint64_t sums_bytes = (n * acc_bytes(acc_bits) + 255) & ~255ull;
     void *base = nullptr;
     if (int rc = static_scratch(&base)) return rc;
     unsigned char *p = (unsigned char *)base;
@@ -888,8 +924,7 @@ int xpr_tune_forward(xpr_stream s, const void *volume, int vol_dtype, void *sums
     SlabShape sh;
     if (int rc = check_args(a, sh)) return rc;
     if (int rc = check_dtype(vol_dtype)) return rc;
-    if (acc_bits != XPR_ACC_U32 && acc_bits != XPR_ACC_U64)
-        return fail(XPR_E_INVALID, "acc_bits must be 32 or 64");
+    if (int rc = check_acc(acc_bits)) return rc;
     if (!volume || !sums_scratch || !projdata || !choice) return fail(XPR_E_INVALID, "null buffer");
     if (a->n_views == 0 || sh.nx == 0 || sh.nrows == 0) return fail(XPR_E_INVALID, "empty launch");
     hipStream_t st = (hipStream_t)s;
@@ -897,24 +932,12 @@ int xpr_tune_forward(xpr_stream s, const void *volume, int vol_dtype, void *sums
     int ti = 0;
     for (int cap : kTileCaps) {
         auto launch = [&]() {
-            if (acc_bits == XPR_ACC_U32) {
-                switch (vol_dtype) {
-                case XPR_F32: return forward_impl<F32, uint32_t>(st, volume, sums_scratch, projdata, *a, sh, cap);
-                case XPR_F16: return forward_impl<F16, uint32_t>(st, volume, sums_scratch, projdata, *a, sh, cap);
-                default: return forward_impl<BF16, uint32_t>(st, volume, sums_scratch, projdata, *a, sh, cap);
-                }
-            }
-            switch (vol_dtype) {
-            case XPR_F32: return forward_impl<F32, uint64_t>(st, volume, sums_scratch, projdata, *a, sh, cap);
-            case XPR_F16: return forward_impl<F16, uint64_t>(st, volume, sums_scratch, projdata, *a, sh, cap);
-            default: return forward_impl<BF16, uint64_t>(st, volume, sums_scratch, projdata, *a, sh, cap);
-            }
+            return forward_dispatch(st, volume, vol_dtype, sums_scratch, acc_bits, projdata, *a, sh, cap);
         };
         if (int rc = time_best(st, 3, launch, &times[ti++])) return rc;
     }
     const int pick = pick_shape(kTileCaps, times, kTileCap);
-    const int acc_bytes = acc_bits / 8;
-    tune_store(tune_key(kTuneForward, vol_dtype, acc_bytes, *a, sh), pick);
+    tune_store(tune_key(kTuneForward, vol_dtype, acc_bits, *a, sh), pick);
     *choice = pick;
     return XPR_OK;
 }
@@ -1032,13 +1055,15 @@ int xpr_valu_ceiling(xpr_stream s, double *lane_instr_per_s)
 
 int xpr_q22_to_float(xpr_stream s, const void *sums, int acc_bits, float *out, uint64_t n)
 {
-    if (acc_bits != XPR_ACC_U32 && acc_bits != XPR_ACC_U64)
-        return fail(XPR_E_INVALID, "acc_bits must be 32 or 64");
+    if (int rc = check_acc(acc_bits)) return rc;
     if (n == 0) return XPR_OK;
     if (!sums || !out) return fail(XPR_E_INVALID, "null buffer");
     if (acc_bits == XPR_ACC_U32)
         hipLaunchKernelGGL(k_q22_to_float<uint32_t>, dim3(grid_for(n)), dim3(256), 0,
                            (hipStream_t)s, (const uint32_t *)sums, out, n);
+    else if (acc_bits == XPR_ACC_F32)
+        hipLaunchKernelGGL(k_q22_to_float<float>, dim3(grid_for(n)), dim3(256), 0,
+                           (hipStream_t)s, (const float *)sums, out, n);
     else
         hipLaunchKernelGGL(k_q22_to_float<uint64_t>, dim3(grid_for(n)), dim3(256), 0,
                            (hipStream_t)s, (const uint64_t *)sums, out, n);
