@@ -132,7 +132,7 @@ __device__ __forceinline__ void bin_view(const float *__restrict__ pd, uint32_t 
     const float X = __builtin_fmaf(p[0], cpx, __builtin_fmaf(p[1], cpy, __builtin_fmaf(p[2], cpz, p[3])));
     const float Y = __builtin_fmaf(p[4], cpx, __builtin_fmaf(p[5], cpy, __builtin_fmaf(p[6], cpz, p[7])));
     const float Wc = __builtin_fmaf(p[12], cpx, __builtin_fmaf(p[13], cpy, __builtin_fmaf(p[14], cpz, p[15])));
-    const float r = 1.0f / Wc;
+    const float r = recip(Wc);                          // 1.0f / Wc, exactly
     const float u = X * r, v = CROP ? Y * r - line_off : Y * r;
     float umin = u, vmin = v, umax = u, vmax = v;
     int ok = (Wc > 0.0f) & (u == u) & (v == v) & (fabsf(u) < 1e7f) & (fabsf(v) < 1e7f);
@@ -160,19 +160,23 @@ __device__ __forceinline__ void bin_view(const float *__restrict__ pd, uint32_t 
         ok &= __shfl_xor(ok, off);
         fast &= __shfl_xor(fast, off);
     }
-    // Layer coherence of the brick's rows (see the coherence modes below):
-    // the 8 lanes of the view check 8 of the 64 (depth, layer) pairs each,
-    // with the positions and FMA chains the splat evaluates.
+    // Layer coherence of the brick's rows (see the coherence modes below),
+    // with the positions and FMA chains the splat evaluates.  A row term
+    // fmaf(c, py, t) is the correctly rounded value of a function affine in
+    // py, and py = fmaf(layer, vx, origin) rounds monotonically in the layer:
+    // the 4 layers' terms are monotone in the layer, so they all agree when
+    // the first and last layer's do.  The 8 lanes of the view check 2 of the
+    // 16 depths each.
     int wsame = 1, usame = 1;
+    const float py0 = __builtin_fmaf((float)(lo[1] + a.vmin[1]), a.voxel[1], a.origin[1]);
+    const float py3 = __builtin_fmaf((float)(lo[1] + 3 + a.vmin[1]), a.voxel[1], a.origin[1]);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int pr = corner * 8 + i, d = pr >> 2, l = pr & 3;
-        const float py = __builtin_fmaf((float)(lo[1] + l + a.vmin[1]), a.voxel[1], a.origin[1]);
-        const float py0 = __builtin_fmaf((float)(lo[1] + a.vmin[1]), a.voxel[1], a.origin[1]);
+    for (int i = 0; i < 2; ++i) {
+        const int d = corner * 2 + i;
         const float pz = __builtin_fmaf((float)(lo[2] + d + a.vmin[2]), a.voxel[2], a.origin[2]);
         const float t0 = __builtin_fmaf(p[2], pz, p[3]), t3 = __builtin_fmaf(p[14], pz, p[15]);
-        wsame &= __builtin_fmaf(p[13], py, t3) == __builtin_fmaf(p[13], py0, t3);
-        usame &= __builtin_fmaf(p[1], py, t0) == __builtin_fmaf(p[1], py0, t0);
+        wsame &= __builtin_fmaf(p[13], py3, t3) == __builtin_fmaf(p[13], py0, t3);
+        usame &= __builtin_fmaf(p[1], py3, t0) == __builtin_fmaf(p[1], py0, t0);
     }
 #pragma unroll
     for (int off = 1; off < 8; off <<= 1) {
