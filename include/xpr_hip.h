@@ -138,7 +138,9 @@ int xpr_convert(xpr_stream s, void *dst, int dst_dtype, const void *src, int src
  * of the slab into the detector pixel each view rounds it to:
  *   volume   : slab [layer][depth][x] of (vmax-vmin) voxels, dtype vol_dtype
  *   sums     : uint32 or uint64 Q.22 [n_views][line_rows][det_width]
- *              (acc_bits 32 / 64), or float32 (acc_bits XPR_ACC_F32: adds v)
+ *              (acc_bits 32 / 64), or float32 (acc_bits XPR_ACC_F32, opt-in:
+ *              the same Q.22 values binned exactly per brick, the bins added
+ *              as trunc(v*2^22)*2^-22 sums with float atomics)
  *   projdata : device float[n_views*modules][20] (ProjData, constants.cl:25-28)
  * Integer accumulation: the result does not depend on launch order. */
 int xpr_forward(xpr_stream s, const void *volume, int vol_dtype, void *sums, int acc_bits,

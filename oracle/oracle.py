@@ -599,8 +599,9 @@ def reconstruct_naive_f64nn(images: np.ndarray, projdata: np.ndarray, grid: dict
     (orig + jitter) and voxel positions as the FP32 path decide which pixel
     each voxel-view meets; sums, error, weights and the update are double
     (first-pass sums 1.0).  q22 truncates each forward contribution to a
-    multiple of 2^-22 as the Q.22 sums do; without it there is no
-    quantisation (the definition of float sums).  interp "bilinear": the
+    multiple of 2^-22 as the Q.22 sums and the float sums (whose tiles bin
+    the same Q.22 integers) do; without it there is no quantisation.  interp
+    "bilinear": the
     bilinear backprojector's FP64 model (backward_bilinear_f64nn)."""
     bwd = backward_bilinear_f64nn if interp == "bilinear" else backward_f64nn
     n, h, w = images.shape

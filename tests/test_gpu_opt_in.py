@@ -12,15 +12,19 @@ Bars:
     ulp of float rounding, ~1e-5 px at 130 px, which moves the weights; taps
     of these N(0, 0.4) error images differ by 0.57 rms and up to ~3; see
     tests/test_oracle_bilinear.py).
-  * float Forward: each pixel within the summation bound of the exact sum of
-    its voxel values (forward_f64nn, no quantisation),
-    |s - S| <= (n - 1) 2^-24 sum|v| + 2^-24 |S|, n its voxel-views;
+  * float Forward: the voxel values are quantised to Q.22 as in the integer
+    sums (FLOAT_TO_FIXED, constants.cl:22) and binned exactly in integer LDS
+    tiles; only the additions into the detector pixels are float.  Each
+    pixel is within the summation bound of its exact Q.22 sum S (the
+    oracle's u64 sums x 2^-22): every term (a tile entry or a direct
+    voxel-view) is rounded once to float and at most n - 1 float additions
+    follow, n the pixel's voxel-views, so |s - S| <= n 2^-24 S (1 + 2 n 2^-24);
     relative L2 below 1e-6.  Volumes of multiples of 2^-8 below 1 (every
-    partial sum exact in float) give sums equal to the Q.22 sums x 2^-22 bit
-    for bit, whatever the order of the atomics.
+    term and partial sum exact in float) give sums equal to the Q.22 sums x
+    2^-22 bit for bit, whatever the order of the atomics.
   * complete reconstructions through XPRecon with "accumulator": "f32" and
     with "interp": "bilinear", against the oracle's FP64 naive schedule on
-    the float32 pixel choice with the same sums and sampling: relative RMSE
+    the float32 pixel choice with the same Q.22 quantisation and sampling: relative RMSE
     below 1e-4 of the volume norm.  The bilinear gather is not the adjoint
     of the nearest-pixel forward; its reconstruction error against the
     phantom stays within 1.6x that of the nearest-pixel reconstruction of
@@ -166,22 +170,23 @@ def test_float_forward_within_summation_bound(cuda, case, tilt):
     vol = rng.uniform(0.0, 2.0, (nl, nk, nx)).astype(np.float32)
     g = oracle_geom(d)
     shape = (d["n_views"], d["line_rows"], d["det_width"])
-    exact = np.zeros(shape)
-    O.forward_f64nn(vol.astype(np.float64), exact, pd, g, 8)
+    q22 = np.zeros(shape, np.uint64)
+    O.forward_f32(vol, q22, pd, g)
+    exact = q22.astype(np.float64) / Q
     cnt = np.zeros(shape)
     O.forward_f64nn(np.ones(vol.shape), cnt, pd, g, 8)
     print(f"float forward case {case} tilt {tilt}", flush=True)
     got = _float_forward(cuda, vol, pd, d).astype(np.float64)
     u = 2.0 ** -24
-    bound = np.maximum(cnt - 1, 0) * u * exact + u * exact
+    bound = cnt * u * exact * (1.0 + 2.0 * cnt * u)
     assert (np.abs(got - exact) <= bound).all()
     assert rel_l2(got, exact) < 1e-6
 
 
 def test_float_sums_subiteration(cuda):
     """xpr_subiteration with float sums: sums from 0, forward, error on the
-    float sums, backward; against the oracle's pieces (exact sums rounded to
-    float, error in float64 from them, FP32 backward), rtol 1e-5."""
+    float sums, backward; against the oracle's pieces (exact Q.22 sums rounded
+    to float, error in float64 from them, FP32 backward), rtol 1e-5."""
     import torch
     import xpulse_recon_amd as X
     spec, pd, d = _case(1)
@@ -196,9 +201,9 @@ def test_float_sums_subiteration(cuda):
     X.subiteration(tv, torch.from_numpy(img).to(cuda), torch.from_numpy(pd).to(cuda), _args(d),
                    0.8, False, sums, err)
     g = oracle_geom(d)
-    exact = np.zeros(shape)
-    O.forward_f64nn(vol.astype(np.float64), exact, pd, g, 8)
-    s32 = exact.astype(np.float32).astype(np.float64)
+    q22 = np.zeros(shape, np.uint64)
+    O.forward_f32(vol, q22, pd, g)
+    s32 = (q22.astype(np.float64) / Q).astype(np.float32).astype(np.float64)
     eps = 2.0 ** -22
     e = (np.log(np.maximum(img / np.maximum(s32, eps), eps)) * np.float32(0.8)).astype(np.float32)
     np.testing.assert_allclose(err.cpu().numpy(), e, rtol=1e-5, atol=1e-6)
@@ -260,7 +265,7 @@ def test_reconstruction_with_opt_in_modes(cuda, scan, mode):
     assert log["accumulator"] == ("f32" if "accumulator" in xpr else 64)
     jit = np.array(log["jitter"], np.float32)
     ref = O.reconstruct_naive_f64nn(imgs, pd32, grid, IT_REC, SIT_REC, 1.0, 0.9, jit,
-                                    threads=16, q22="accumulator" not in xpr,
+                                    threads=16, q22=True,
                                     interp=xpr.get("interp", "nearest"))
     rel = rel_l2(vol, ref)
     rmse = float(np.sqrt(np.mean((vol.astype(np.float64) - truth) ** 2)))

@@ -79,7 +79,8 @@ def _check_sizes(volume, det, projdata, a: ProjArgs):
 
 def forward(volume, sums, projdata, args: ProjArgs, stream=None) -> None:
     """Forward projection: sums[view,row,col] += trunc(v*2^22) (KernelForward.cl:10-55);
-    float32 sums (the opt-in float accumulator) add v itself."""
+    float32 sums (the opt-in float accumulator) add the same Q.22 values
+    times 2^-22 with float additions."""
     _need_cuda(volume, sums, projdata)
     _check_sizes(volume, sums, projdata, args)
     _lib.call("xpr_forward", _stream(stream), _ptr(volume), dtype_code(volume), _ptr(sums),

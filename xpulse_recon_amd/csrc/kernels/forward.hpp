@@ -17,7 +17,7 @@ namespace {
 // Forward with LDS binning.  A workgroup owns a brick of 64 (x) x 4 (layers)
 // x 16 (depths).  Per view the brick's 8 corners bound its detector footprint
 // (a rounding bound, or one pixel of margin, covers float rounding, see
-// bin_view); voxel-views accumulate there with LDS u64 atomics and the
+// bin_view); voxel-views accumulate there with LDS integer atomics and the
 // non-zero footprint pixels go to the detector sums with one global atomic
 // each.  Two tiles alternate: the flush of view v also clears its tile, so
 // one barrier per view separates splat and flush.  Footprints above the tile
@@ -30,11 +30,14 @@ namespace {
 // view whose W does not change across the layers computes W and its
 // reciprocal once per column (see the coherence modes below).
 //
-// Float sums (AccT = float, the opt-in XPR_ACC_F32): the same bricks, tiles
-// and paths with float tile entries (LDS float atomics) and global float
-// atomics; a voxel's 32-bit word then carries its float value instead of its
-// Q.22 integer.  The additions happen in whatever order the atomics land, so
-// the sums are within float rounding of the exact sum, not bit-reproducible.
+// Float sums (AccT = float, the opt-in XPR_ACC_F32): the same bricks, Q.22
+// values, integer tiles and paths; only the detector sums are float.  A
+// flushed tile entry, or a direct voxel-view, adds its real value (the Q.22
+// integer times 2^-22, rounded once to float) with a global float atomic.
+// The additions of a pixel happen in whatever order the atomics land, so the
+// sums are within float rounding of the exact Q.22 sum, not bit-reproducible.
+// (The tiles stay integer: ds_add_f32 issues about 40 times slower than
+// ds_add_u32 on gfx950.)
 //
 // The footprint, bounds thresholds, matrix rows, order and fast-path flags
 // of a chunk of 32 views are computed once per workgroup (8 corner lanes per
@@ -61,11 +64,20 @@ __device__ __forceinline__ void global_add(AccT *p, AccT v)
     __hip_atomic_fetch_add((Glob<AccT> *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// A voxel's 32-bit word added to a float tile entry or float sum: the float
-// whose bits it holds.  (The integer tiles and sums add the word itself; the
-// float forms sit in `if constexpr` branches so the integer kernels keep
-// exactly their own code.)
-__device__ __forceinline__ float word_float(uint32_t q) { return __builtin_bit_cast(float, q); }
+// A Q.22 integer (a tile entry or a voxel-view's value) in the sums' type:
+// the integer itself, or for float sums its real value q x 2^-22 rounded once
+// to float (a 64-bit q through double, exact below 2^53: a tile entry sums at
+// most 4096 values below 2^32).
+template <typename AccT, typename Q>
+__device__ __forceinline__ AccT sum_term(Q q)
+{
+    if constexpr (std::is_same<AccT, float>::value) {
+        if constexpr (sizeof(Q) == 8) return (float)((double)q * (double)kQ22Eps);
+        else return (float)q * kQ22Eps;
+    } else {
+        return (AccT)q;
+    }
+}
 
 enum : uint32_t {
     kBinSkip = 1u,     // no voxel of the brick lands inside the view's bounds
@@ -248,8 +260,9 @@ __device__ __forceinline__ void lds_barrier()
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-// one brick with some Q.22 value of 2^32 or more: one global atomic per
-// voxel-view with the exact 64-bit value (x-lane order)
+// one brick with some Q.22 value of 2^32 or more (64-bit or float sums): one
+// global atomic per voxel-view with the exact 64-bit value, or for float sums
+// its real value (x-lane order)
 template <typename VS, typename AccT, bool CROP>
 __device__ __noinline__ void forward_direct_u64(const typename VS::T *__restrict__ vol,
                                                 AccT *__restrict__ sums,
@@ -269,7 +282,9 @@ __device__ __noinline__ void forward_direct_u64(const typename VS::T *__restrict
         const int32_t k = k0 + i;
         if (x >= s.nx || k >= s.nk) continue;
         const float v = VS::load(vol, ((uint64_t)l * s.nk + k) * s.nx + x);
-        const AccT q = sizeof(AccT) == 8 ? (AccT)q22_u64(v) : (AccT)q22_u32(v);
+        AccT q;
+        if constexpr (sizeof(AccT) == 8 || std::is_same<AccT, float>::value) q = sum_term<AccT>(q22_u64(v));
+        else q = (AccT)q22_u32(v);
         if (q == 0) continue;
         const float pz = __builtin_fmaf((float)(k + a.vmin[2]), a.voxel[2], a.origin[2]);
         for (uint32_t view = 0, m = 0; view < a.n_views; ++view) {
@@ -328,12 +343,8 @@ template <typename TileT>
 __device__ __forceinline__ void lds_add(uint32_t addr, uint32_t q)
 {
     using L = __attribute__((address_space(3))) TileT;
-    if constexpr (std::is_same<TileT, float>::value)
-        __hip_atomic_fetch_add((L *)(uintptr_t)addr, word_float(q), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
-    else
-        __hip_atomic_fetch_add((L *)(uintptr_t)addr, (TileT)q, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add((L *)(uintptr_t)addr, (TileT)q, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // one voxel-view: its Q.22 value goes to its pixel's tile entry (inside: the
@@ -355,10 +366,7 @@ __device__ __forceinline__ void deposit(float u, float v, uint32_t q, const View
         const uint32_t tv = (uint32_t)pixel_of(v) - (uint32_t)t.v0;
         const bool hit = (tu < (uint32_t)t.tw) & (tv < (uint32_t)t.th);
         const uint32_t e = __umul24(tv, (uint32_t)t.tw) + tu;
-        if constexpr (std::is_same<TileT, float>::value)
-            atomicAdd(&t.tl[hit ? e : t.sk], word_float(hit ? q : 0u));
-        else
-            atomicAdd(&t.tl[hit ? e : t.sk], (TileT)(hit ? q : 0u));
+        atomicAdd(&t.tl[hit ? e : t.sk], (TileT)(hit ? q : 0u));
     } else {
         const bool inb = (u >= c.ulo) & (u < c.uhi) & (v >= c.vlo) & (v < c.vhi);
         const uint32_t iu = (uint32_t)pixel_of(u), iv = (uint32_t)pixel_of(v);
@@ -366,15 +374,9 @@ __device__ __forceinline__ void deposit(float u, float v, uint32_t q, const View
         const uint32_t tv = iv - (uint32_t)t.v0;
         const bool hit = inb & t.use_tile & (tu < (uint32_t)t.tw) & (tv < (uint32_t)t.th);
         const uint32_t e = __umul24(tv, (uint32_t)t.tw) + tu;
-        if constexpr (std::is_same<TileT, float>::value) {
-            atomicAdd(&t.tl[hit ? e : t.sk], word_float(hit ? q : 0u));
-            if (inb & !hit & (q != 0u))  // off the tile: direct global add
-                global_add(t.vsum + (__umul24(iv, t.det_width) + iu), word_float(q));
-        } else {
-            atomicAdd(&t.tl[hit ? e : t.sk], (TileT)(hit ? q : 0u));
-            if (inb & !hit & (q != 0u))  // off the tile: direct global add
-                global_add(t.vsum + (__umul24(iv, t.det_width) + iu), (AccT)q);
-        }
+        atomicAdd(&t.tl[hit ? e : t.sk], (TileT)(hit ? q : 0u));
+        if (inb & !hit & (q != 0u))      // off the tile: direct global add
+            global_add(t.vsum + (__umul24(iv, t.det_width) + iu), sum_term<AccT>(q));
     }
 }
 
@@ -590,7 +592,7 @@ __device__ __forceinline__ void forward_views(AccT *__restrict__ sums, const flo
                     if (v != 0) {
                         const uint32_t row = magic ? __umulhi(e, magic) : e;
                         const uint32_t col = e - row * (uint32_t)t.tw;
-                        global_add(base + (row * t.det_width + col), (AccT)v);
+                        global_add(base + (row * t.det_width + col), sum_term<AccT>(v));
                         t.tl[e] = 0;
                     }
                 }
@@ -630,10 +632,13 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
     const int32_t bl = (int32_t)(b / s.nbk);
     const int32_t x0 = bx * 64, l0 = bl * kWaves, k0 = bk * kBK;
     const int32_t dk = lane & 15, xg = lane >> 4;     // depth-lane order
+    // 64-bit and float sums take Q.22 values of 2^32 and more (the direct
+    // path) and tile sums of 2^32 and more (64-bit tiles); 32-bit sums wrap
+    // like the reference's
+    constexpr bool kWideSums = !std::is_same<AccT, uint32_t>::value;
 
     // Q.22 voxel values in both lane orders (32 bits suffice unless some
-    // value is >= 1024), qx[d][l] and qd[xs][l]; for float sums the float
-    // values' bits.  Raw buffer loads from each
+    // value is >= 1024), qx[d][l] and qd[xs][l].  Raw buffer loads from each
     // layer's brick corner (x0, l0 + l, k0): out-of-slab voxels read 0 without
     // a branch, so all 32 loads are in flight together.
     uint32_t qx[kD][kWaves], qd[kD][kWaves];
@@ -643,7 +648,7 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
         auto q22 = [&](float v, bool track) {
             const float sc = v * kQ22One;               // FLOAT_TO_FIXED (constants.cl:22)
             const bool pos = sc > 0.0f, over = sc >= 4294967296.0f;
-            if (sizeof(AccT) == 8 && track) big |= pos & over;
+            if (kWideSums && track) big |= pos & over;
             return pos ? (over ? 0xffffffffu : (uint32_t)sc) : 0u;
         };
         float vx_[kD][kWaves], vd_[kD][kWaves];
@@ -670,25 +675,15 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
                 vd_[xs][l] = load_or_zero<VS>(rsrc, in ? off : kOutOfRange);
             }
         }
-        if constexpr (std::is_same<AccT, float>::value) {
 #pragma unroll
-            for (int d = 0; d < kD; ++d)
+        for (int d = 0; d < kD; ++d)
 #pragma unroll
-                for (int l = 0; l < kWaves; ++l) {
-                    qx[d][l] = __builtin_bit_cast(uint32_t, vx_[d][l]);
-                    qd[d][l] = __builtin_bit_cast(uint32_t, vd_[d][l]);
-                }
-        } else {
-#pragma unroll
-            for (int d = 0; d < kD; ++d)
-#pragma unroll
-                for (int l = 0; l < kWaves; ++l) {
-                    qx[d][l] = q22(vx_[d][l], true);
-                    qd[d][l] = q22(vd_[d][l], false);
-                }
-        }
+            for (int l = 0; l < kWaves; ++l) {
+                qx[d][l] = q22(vx_[d][l], true);
+                qd[d][l] = q22(vd_[d][l], false);
+            }
     }
-    if (__syncthreads_or(big)) {
+    if (kWideSums && __syncthreads_or(big)) {
         forward_direct_u64<VS, AccT, CROP>(vol, sums, pd, a, s, x0, l0, k0);
         return;
     }
@@ -697,22 +692,26 @@ __device__ __forceinline__ void forward_brick(const typename VS::T *__restrict__
     // 2^20, and always when the detector sums themselves are 32-bit (the same
     // sums modulo 2^32): half the LDS atomic traffic.
     bool wide = false;
-    if (sizeof(AccT) == 8) {
+    if (kWideSums) {
 #pragma unroll
         for (int d = 0; d < kD; ++d)
 #pragma unroll
             for (int l = 0; l < kWaves; ++l) wide |= qx[d][l] >= (1u << 20);
     }
-    if constexpr (std::is_same<AccT, float>::value)
-        forward_views<VS, AccT, CROP, float>(sums, pd, a, s, x0, l0, k0, qx, qd, lds, bins);
-    else if (sizeof(AccT) == 8 && __syncthreads_or(wide))
+    if (kWideSums && __syncthreads_or(wide))
         forward_views<VS, AccT, CROP, unsigned long long>(sums, pd, a, s, x0, l0, k0, qx, qd, lds, bins);
     else
         forward_views<VS, AccT, CROP, uint32_t>(sums, pd, a, s, x0, l0, k0, qx, qd, lds, bins);
 }
 
+// Workgroups per CU the forward is built for: 4 (128 VGPRs) in general; 5
+// (96 VGPRs) with 32-bit sums, whose kernel holds no 64-bit tile path
+// (0.504 against 0.544 ms per 64-layer slab of the 768^3 workload).
+template <typename AccT>
+constexpr int forward_wg_per_cu() { return std::is_same<AccT, uint32_t>::value ? 5 : 4; }
+
 template <typename VS, typename AccT, bool CROP>
-__global__ __launch_bounds__(kThreads, 4) void k_forward_bin(
+__global__ __launch_bounds__(kThreads, forward_wg_per_cu<AccT>()) void k_forward_bin(
     const typename VS::T *__restrict__ vol, AccT *__restrict__ sums,
     const float *__restrict__ pd, xpr_proj_args a, BrickShape s)
 {
