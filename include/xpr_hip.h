@@ -43,10 +43,11 @@ extern "C" {
  * integers: the sums do not depend on launch order. */
 #define XPR_ACC_U32 32
 #define XPR_ACC_U64 64
-/* Opt-in, not the reference's arithmetic: float32 sums, to which every
- * voxel-view adds its float value (no Q.22 truncation) through LDS and
- * global float atomics.  The order of those additions changes from launch to
- * launch, so the sums do, within float rounding of the exact sum. */
+/* Opt-in, not the reference's arithmetic: float32 sums.  The forward bins the
+ * same Q.22 values exactly in integer LDS tiles and adds each binned sum
+ * (times 2^-22, rounded once to float) to the detector pixel with a global
+ * float atomic.  The order of those additions changes from launch to launch,
+ * so the sums do, within float rounding of the exact Q.22 sum. */
 #define XPR_ACC_F32 (-32)
 
 /* Detector sampling of the backprojector (xpr_proj_args.interp). */
