@@ -14,8 +14,9 @@
 //     the call returns, as the HIP runtime stages them;
 //   * a "kernel" is a host loop on the stream's thread that reads and writes
 //     exactly the buffers, with the extents, that the real kernel reads and
-//     writes (values are placeholders), so a host access that races with it,
-//     or an extent past an allocation, is reported.
+//     writes (values are placeholders; buffers the kernel only adds to with
+//     atomics get atomic adds), so a host access that races with it, or an
+//     extent past an allocation, is reported.
 //
 // Device memory is host memory (malloc), so ASan checks the extents too.
 #include <chrono>
@@ -141,6 +142,16 @@ void touch(const void *p, size_t bytes, bool w)
         acc += b[i];
         if (w) b[i] = (unsigned char)(acc & 0x7f);
     }
+}
+
+// atomic adds to every 32-bit word of [p, p + bytes): the accesses of a kernel
+// that only adds to a buffer with atomics (the forward's detector sums, the
+// backward's empty-voxel counter), which concurrent launches may share
+void touch_atomic(void *p, size_t bytes)
+{
+    if (!p || !bytes) return;
+    uint32_t *w = (uint32_t *)p;
+    for (size_t i = 0; i < bytes / 4; ++i) __atomic_fetch_add(&w[i], 0u, __ATOMIC_RELAXED);
 }
 
 uint64_t slab_voxels(const xpr_proj_args *a)
@@ -357,7 +368,7 @@ int xpr_forward(xpr_stream s, const void *vol, int vdt, void *sums, int acc, con
     S(s).push([=] {
         touch(pd, size_t(c.n_views) * c.modules * 20 * 4, false);
         touch(vol, slab_voxels(&c) * dtype_bytes(vdt), false);
-        touch(sums, det_samples(&c) * acc_bytes(acc), true);
+        touch_atomic(sums, det_samples(&c) * acc_bytes(acc));
     });
     return XPR_OK;
 }
@@ -380,7 +391,7 @@ int xpr_backward(xpr_stream s, void *vol, int vdt, const void *err, int edt, con
         touch(pd, size_t(c.n_views) * c.modules * 20 * 4, false);
         touch(err, det_samples(&c) * dtype_bytes(edt), false);
         touch(vol, slab_voxels(&c) * dtype_bytes(vdt), true);
-        if (empty) touch(empty, 8, true);
+        if (empty) touch_atomic(empty, 8);
     });
     return XPR_OK;
 }
@@ -508,7 +519,7 @@ int engine_launch(Engine *e, int slot)
                     break;
                 case kForward:
                     touch(it.dst, slab_voxels(&it.a) * dtype_bytes(it.dt_dst), false);
-                    touch(it.sums, det_samples(&it.a) * 8, true);
+                    touch_atomic(it.sums, det_samples(&it.a) * 8);
                     break;
                 case kError:
                     touch(it.src, it.n * dtype_bytes(it.dt_src), false);
