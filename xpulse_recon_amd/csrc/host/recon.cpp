@@ -806,10 +806,6 @@ public:
         resolve();
         check_memory(requiredGPUMemory(), method_name(p.recon.method));
     }
-    ~ReconstructionSlab() override
-    {
-        if (side_) xpr_stream_destroy(side_);
-    }
     uint64_t requiredGPUMemory() override { return need(store_, hs_, resident_); }
 
     void exec() override
@@ -997,9 +993,6 @@ private:
     // (slab updated, ready to go back), freed_ (buffer i free again: its
     // slab's kernel done, or its download done after a backward).
     Event up_[2], kd_[2], freed_[2];
-    // the device store's second compute stream and the pass's fork and join
-    xpr_stream side_ = nullptr;
-    Event fork_, join_;
 
     struct SlabViews {
         std::vector<float> pd;
@@ -1136,24 +1129,12 @@ private:
     void pass(bool backward, bool first, bool last, int64_t s, Operators &ops)
     {
         if (store_ == kDevice) {
-            // The layer store is the resident volume: slabs are launched in
-            // place, alternating between the compute stream and a second one.
-            // The launches of a pass are independent (the forward adds
-            // integers with atomics, each backward owns its layers), so each
-            // runs while the previous one drains; the pass ends when both
-            // streams are done.
-            if (!side_) check(xpr_stream_create(&side_), "stream");
-            check(xpr_event_record(fork_.e, stream_), "event");
-            check(xpr_stream_wait_event(side_, fork_.e), "wait");
+            // the layer store is the resident volume: slabs are launched in place
             for (int64_t b = 0; b < nslab_; ++b) {
-                ops.setStream(b & 1 ? side_ : stream_);
                 set_slab(ops, s, b, static_cast<char *>(cvol_) + size_t(b * hs_) * layer_ * csb_);
                 if (backward) ops.backward();
                 else ops.forward();
             }
-            ops.setStream(stream_);
-            check(xpr_event_record(join_.e, side_), "event");
-            check(xpr_stream_wait_event(stream_, join_.e), "wait");
             return;
         }
         host_pass(backward, first, last, s, ops);

@@ -189,7 +189,6 @@ public:
     void error();
     void backward();
     xpr_stream stream() const { return stream_; }
-    void setStream(xpr_stream s) { stream_ = s; }     // where the next launches go
     int acc_bits() const { return acc_; }
     const xpr_proj_args &args() const { return a_; }
 
