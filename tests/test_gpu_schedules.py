@@ -102,11 +102,11 @@ def test_naive_matches_oracle(scan):
     ("method_1b", {"slab_height": 6, "layer_store": "zstd", "threads": 4, "storage": "bf16"}),
     ("method_1c", {"slab_height": 5, "layer_store": "zstd"}),
     ("method_1c", {"slab_height": 5, "layer_store": "device"}),
-    # slab height from the budget: room for two 13-layer slab buffers beside
-    # the sums, error and images of one sub-iteration, ProjData and the two
-    # preprocessing upload slots
+    # slab height from the budget: room for three 13-layer slab buffers (the
+    # pipeline's ring) beside the sums, error and images of one sub-iteration,
+    # ProjData and the two preprocessing upload slots
     ("method_1a", {"layer_store": "ram",
-                   "device_budget_gb": (2 * 13 * W * W * 4 + 6 * W * H * 16 + 24 * 80 +
+                   "device_budget_gb": (3 * 13 * W * W * 4 + 6 * W * H * 16 + 24 * 80 +
                                         2 * 6 * W * H * 4 + 100) / 1e9}),
 ])
 def test_streaming_schedules_equal_naive(scan, method, xpr):

@@ -862,7 +862,7 @@ public:
             vol_.alloc(uint64_t(H) * layer_ * sb_);
             check(xpr_fill(stream_, vol_.get(), st_, 1.0f, uint64_t(H) * layer_), "fill");
         } else {
-            for (int k = 0; k < 2; ++k) {
+            for (int k = 0; k < kRing; ++k) {
                 dslab_[k].alloc(slab_n * sb_);
                 if (st_ != XPR_F32) dup_[k].alloc(slab_n * 4);
                 if (store_ == kZstd) {
@@ -895,8 +895,8 @@ public:
         }
         fill_sums(stream_, sums, p_.xpr.accumulator, true, uint64_t(As * P));
         // the slab buffers are free once the tuning launches above are done
-        check(xpr_event_record(freed_[0].e, stream_), "event");
-        check(xpr_event_record(freed_[1].e, stream_), "event");
+        for (int k = 0; k < kRing; ++k) check(xpr_event_record(kdone_[k].e, stream_), "event");
+        stored_ = std::vector<Event>(static_cast<size_t>(nslab_));
         // images: every sub-iteration's resident in HBM, preprocessed on the copy
         // stream ahead of the sub-iteration that first reads them, or one
         // sub-iteration's at a time on the compute stream
@@ -964,6 +964,8 @@ public:
             }
         }
         if (store_ == kDevice) write_final(ds, stream_, cvol_, cst_, 0, H, stage_);
+        note("slab_h2d_bytes", Json::number(double(h2d_bytes_)));
+        note("slab_d2h_bytes", Json::number(double(d2h_bytes_)));
         check(xpr_stream_sync(copy_back_), "sync");
         check(xpr_stream_sync(copy_), "sync");
         check(xpr_stream_sync(stream_), "sync");
@@ -986,13 +988,22 @@ private:
     // dup_[i]: float32 staging of slab buffer i for 16-bit storage, used in
     // both directions (the upload's conversion out of it is done before the
     // slab's kernel, the download's conversion into it after)
-    DevBuf vol_, stage_, dslab_[2], dup_[2];
-    HostBuf hin_[2], hout_[2];
+    // Slab buffers of the host-store pipeline: with three, a buffer's next
+    // upload waits for the download of the slab three places before instead
+    // of the one just before, so the uploads and downloads of the backward
+    // pass keep both copy directions busy.
+    static constexpr int kRing = 3;
+    DevBuf vol_, stage_, dslab_[kRing], dup_[kRing];
+    HostBuf hin_[kRing], hout_[kRing];
     std::vector<DevBuf> pds_;
-    // Events of the slab pipelines: up_ (slab uploaded into buffer i), kd_
-    // (slab updated, ready to go back), freed_ (buffer i free again: its
-    // slab's kernel done, or its download done after a backward).
-    Event up_[2], kd_[2], freed_[2];
+    // Events of the slab pipelines, per buffer i: up_ (its upload done),
+    // kdone_ (its last kernel, and the conversion of 16-bit values, done),
+    // dl_ (its last download done).
+    Event up_[kRing], kdone_[kRing], dl_[kRing];
+    int64_t held_[kRing] = {-1, -1, -1};   // the slab whose current values buffer i holds
+    bool reverse_ = false;                 // the next pass runs its slabs from the last
+    std::vector<Event> stored_;            // slab b's last download into the store done
+    uint64_t h2d_bytes_ = 0, d2h_bytes_ = 0;  // slab bytes the passes moved
 
     struct SlabViews {
         std::vector<float> pd;
@@ -1017,7 +1028,7 @@ private:
         uint64_t m = As * P * (acc + sv) + (resident ? S : 1) * As * P * sv +
                      ds_->projdata().size() * 4 + staging;
         if (st == kDevice) m += H * layer * sv;
-        else m += 2 * uint64_t(Hs) * layer * (sv + (sv != 4 ? 4 : 0));
+        else m += uint64_t(kRing) * uint64_t(Hs) * layer * (sv + (sv != 4 ? 4 : 0));
         if (st == kDevice && sv != 4 && p_.xpr.final_fp32) m += H * layer * 4 + As * P * 8;
         return m;
     }
@@ -1140,69 +1151,107 @@ private:
         host_pass(backward, first, last, s, ops);
     }
 
-    // Slabs through two device buffers, three streams: uploads on copy_,
+    // Slabs through kRing device buffers, three streams: uploads on copy_,
     // kernels on the compute stream, downloads on copy_back_; the host only
     // enqueues ("ram": DMA straight from and to the pinned store) or also
-    // decompresses slab b and compresses slab b-1 while the device works
-    // ("zstd").
+    // decompresses and compresses slabs while the device works ("zstd").
+    //
+    // Each pass runs its slabs in the reverse order of the previous one, so
+    // the last kRing slabs of a pass, still in their buffers with their
+    // current values, open the next pass without an upload (a forward
+    // leaves the volume as it was; a backward's buffers hold the values
+    // that are on their way back to the store).  The forward adds integers
+    // and each backward owns its voxels: the order changes no result.
     void host_pass(bool backward, bool first, bool last, int64_t s, Operators &ops)
     {
         auto &ds = *ds_;
-        const int64_t H = ds.vheight();
+        const int64_t H = ds.vheight(), n = nslab_;
         const bool zstd = store_ == kZstd;
-        auto save = [&](int64_t b) {                     // zstd: slab b back to the spill files
-            const int i = int(b & 1);
+        auto save = [&](int64_t b, int i) {              // zstd: slab b (buffer i) to the spill files
             const int64_t l0 = b * hs_, l1 = std::min(H, l0 + hs_);
-            check(xpr_event_sync(freed_[i].e), "sync");
+            check(xpr_event_sync(dl_[i].e), "sync");
             const float *src = hout_[i].as<float>();
             parallel_for(l1 - l0, threads_, [&](int64_t k) {
                 ds.save_layer(src + size_t(k) * layer_, l0 + k, false);
             });
         };
-        for (int64_t b = 0; b < nslab_; ++b) {
-            const int i = int(b & 1);
-            const int64_t l0 = b * hs_, l1 = std::min(H, l0 + hs_);
-            const size_t n = size_t(l1 - l0) * layer_;
-            const bool fill = backward && first;
-            const float *src = nullptr;
-            if (!fill) {
-                if (zstd) {
-                    // hin_[i] is free once the upload of slab b-2 is done
-                    check(xpr_event_sync(up_[i].e), "sync");
-                    float *dst = hin_[i].as<float>();
-                    parallel_for(l1 - l0, threads_, [&](int64_t k) {
-                        ds.load_layer(l0 + k, dst + size_t(k) * layer_);
-                    });
-                    src = dst;
-                } else {
-                    src = ds.ram_layer(l0);
+        // this pass's order and buffers: the slabs a buffer holds come first
+        // and keep their buffer; the others take the free buffers, then the
+        // buffer of the slab kRing places before them
+        std::vector<int64_t> ord(static_cast<size_t>(n));
+        std::vector<int> slot(static_cast<size_t>(n));
+        std::vector<char> resident(static_cast<size_t>(n), 0);
+        for (int64_t k = 0; k < n; ++k) ord[size_t(k)] = reverse_ ? n - 1 - k : k;
+        reverse_ = !reverse_;
+        bool taken[kRing] = {};
+        for (int64_t k = 0; k < std::min<int64_t>(n, kRing); ++k)
+            for (int j = 0; j < kRing; ++j)
+                if (!first && held_[j] == ord[size_t(k)]) {
+                    slot[size_t(k)] = j;
+                    resident[size_t(k)] = 1;
+                    taken[j] = true;
                 }
+        for (int64_t k = 0, j = 0; k < std::min<int64_t>(n, kRing); ++k) {
+            if (resident[size_t(k)]) continue;
+            while (taken[j]) ++j;
+            slot[size_t(k)] = j;
+            taken[j] = true;
+        }
+        for (int64_t k = kRing; k < n; ++k) slot[size_t(k)] = slot[size_t(k - kRing)];
+        for (int j = 0; j < kRing; ++j) held_[j] = -1;
+        int64_t prev_b = -1;
+        int prev_i = 0;
+        for (int64_t k = 0; k < n; ++k) {
+            const int64_t b = ord[size_t(k)];
+            const int i = slot[size_t(k)];
+            const int64_t l0 = b * hs_, l1 = std::min(H, l0 + hs_);
+            const size_t nv = size_t(l1 - l0) * layer_;
+            if (!resident[size_t(k)]) {
+                const bool fill = backward && first;
+                const float *src = nullptr;
+                if (!fill) {
+                    if (zstd) {
+                        // hin_[i] is free once its last upload is done
+                        check(xpr_event_sync(up_[i].e), "sync");
+                        float *dst = hin_[i].as<float>();
+                        parallel_for(l1 - l0, threads_, [&](int64_t q) {
+                            ds.load_layer(l0 + q, dst + size_t(q) * layer_);
+                        });
+                        src = dst;
+                    } else {
+                        src = ds.ram_layer(l0);
+                        // the store holds the slab once its last download is done
+                        check(xpr_stream_wait_event(copy_, stored_[size_t(b)].e), "wait");
+                    }
+                }
+                // buffer i is free once its last kernel and its last download
+                // (a slab that opened this pass without an upload may still be
+                // on its way back) are done
+                check(xpr_stream_wait_event(copy_, kdone_[i].e), "wait");
+                check(xpr_stream_wait_event(copy_, dl_[i].e), "wait");
+                if (!fill) h2d_bytes_ += nv * 4;
+                if (fill) {
+                    check(xpr_fill(copy_, dslab_[i].get(), st_, 1.0f, nv), "fill");
+                } else if (st_ == XPR_F32) {
+                    check(xpr_upload(copy_, dslab_[i].get(), src, nv * 4), "upload");
+                } else {
+                    check(xpr_upload(copy_, dup_[i].get(), src, nv * 4), "upload");
+                    check(xpr_convert(copy_, dslab_[i].get(), st_, dup_[i].get(), XPR_F32, nv), "convert");
+                }
+                check(xpr_event_record(up_[i].e, copy_), "event");
+                check(xpr_stream_wait_event(stream_, up_[i].e), "wait");
             }
-            // device buffer i is free once its last user (slab b-2, or a slab
-            // of the previous pass) is done with it.  Downloads run in order
-            // on copy_back_, so at b = 0, 1 this also waits for every write
-            // back of the previous pass into the store.
-            check(xpr_stream_wait_event(copy_, freed_[i].e), "wait");
-            if (fill) {
-                check(xpr_fill(copy_, dslab_[i].get(), st_, 1.0f, n), "fill");
-            } else if (st_ == XPR_F32) {
-                check(xpr_upload(copy_, dslab_[i].get(), src, n * 4), "upload");
-            } else {
-                check(xpr_upload(copy_, dup_[i].get(), src, n * 4), "upload");
-                check(xpr_convert(copy_, dslab_[i].get(), st_, dup_[i].get(), XPR_F32, n), "convert");
-            }
-            check(xpr_event_record(up_[i].e, copy_), "event");
-            check(xpr_stream_wait_event(stream_, up_[i].e), "wait");
+            held_[i] = b;
             set_slab(ops, s, b, dslab_[i].get());
             if (!backward) {
                 ops.forward();
-                check(xpr_event_record(freed_[i].e, stream_), "event");
+                check(xpr_event_record(kdone_[i].e, stream_), "event");
                 continue;
             }
             ops.backward();
             if (last) {
                 write_final(ds, stream_, dslab_[i].get(), st_, l0, l1, stage_);
-                check(xpr_event_record(freed_[i].e, stream_), "event");
+                check(xpr_event_record(kdone_[i].e, stream_), "event");
                 if (zstd)
                     for (int64_t l = l0; l < l1; ++l) {
                         std::error_code ec;
@@ -1212,16 +1261,22 @@ private:
             }
             const void *back = dslab_[i].get();
             if (st_ != XPR_F32) {
-                check(xpr_convert(stream_, dup_[i].get(), XPR_F32, dslab_[i].get(), st_, n), "convert");
+                // dup_[i] may still feed the buffer's last upload
+                check(xpr_stream_wait_event(stream_, up_[i].e), "wait");
+                check(xpr_convert(stream_, dup_[i].get(), XPR_F32, dslab_[i].get(), st_, nv), "convert");
                 back = dup_[i].get();
             }
-            check(xpr_event_record(kd_[i].e, stream_), "event");
-            check(xpr_stream_wait_event(copy_back_, kd_[i].e), "wait");
-            check(xpr_download(copy_back_, zstd ? hout_[i].get() : ds.ram_layer(l0), back, n * 4), "download");
-            check(xpr_event_record(freed_[i].e, copy_back_), "event");
-            if (zstd && b >= 1) save(b - 1);
+            check(xpr_event_record(kdone_[i].e, stream_), "event");
+            check(xpr_stream_wait_event(copy_back_, kdone_[i].e), "wait");
+            check(xpr_download(copy_back_, zstd ? hout_[i].get() : ds.ram_layer(l0), back, nv * 4), "download");
+            d2h_bytes_ += nv * 4;
+            check(xpr_event_record(dl_[i].e, copy_back_), "event");
+            check(xpr_event_record(stored_[size_t(b)].e, copy_back_), "event");
+            if (zstd && prev_b >= 0) save(prev_b, prev_i);
+            prev_b = b;
+            prev_i = i;
         }
-        if (zstd && backward && !last && nslab_ >= 1) save(nslab_ - 1);
+        if (zstd && prev_b >= 0) save(prev_b, prev_i);
     }
 };
 
