@@ -1381,16 +1381,17 @@ public:
     explicit ReconstructionChunk(Parameters &p) : Reconstruction(p)
     {
         chunks_ = plan_chunks(p, *ds_, budget());
-        std::cout << "Chunks : ";
+        const bool talk = !p.xpr.quiet;
+        if (talk) std::cout << "Chunks : ";
         Json cj = Json::array();
         for (const auto &c : chunks_) {
-            std::cout << " [" << c.vOffset << "-" << c.vOffset + c.vSize << "],";
+            if (talk) std::cout << " [" << c.vOffset << "-" << c.vOffset + c.vSize << "],";
             Json e = Json::array();
             for (int64_t v : {c.vOffset, c.vSize, c.iOffset, c.iSize, c.vPadTop, c.vPadBot})
                 e.arr.push_back(Json::number(double(v)));
             cj.arr.push_back(e);
         }
-        std::cout << std::endl;
+        if (talk) std::cout << std::endl;
         note("chunks", cj);
         check_memory(requiredGPUMemory(), "method_2");
     }
@@ -1419,7 +1420,7 @@ public:
         Event e0, e1, fb, fe, bb, be;
         note("launches_per_pass", Json::number(1));
         for (const Chunk &c : chunks_) {
-            std::cout << "[" << c.vOffset << "-" << c.vOffset + c.vSize << "]... " << std::flush;
+            if (!p_.xpr.quiet) std::cout << "[" << c.vOffset << "-" << c.vOffset + c.vSize << "]... " << std::flush;
             const int64_t l0 = c.vOffset - c.vPadTop, l1 = c.vOffset + c.vSize + c.vPadBot;
             const uint64_t nv = uint64_t(l1 - l0) * layer, npx = uint64_t(c.iSize * dw * As);
             DevBuf vol(nv * dtype_bytes(st)), img(npx * dtype_bytes(st)),
@@ -1466,7 +1467,7 @@ public:
             write_final(ds, stream_,
                         static_cast<const char *>(vol.get()) + size_t(c.vPadTop) * layer * dtype_bytes(st),
                         st, c.vOffset, c.vOffset + c.vSize, stage);
-            std::cout << "Done." << std::endl;
+            if (!p_.xpr.quiet) std::cout << "Done." << std::endl;
         }
     }
 
