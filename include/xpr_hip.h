@@ -231,6 +231,13 @@ int xpr_minmax_read(const int32_t *minmax_host, float *mn, float *mx);
  * if mlog (data_loader.hpp:80-86), from float input to dtype out_dtype. */
 int xpr_preprocess(xpr_stream s, const float *in, void *out, int out_dtype, uint64_t n,
                    int normalize, int mlog, float mn, float mx);
+/* xpr_preprocess of n_views views of n elements in one launch: view j is read
+ * from in + j*in_view_stride (elements; >= n) and written to out + j*n.  The
+ * views of one sub-iteration lie at a fixed stride in the projection stack
+ * (Dataset::getSitImages, dataset.hpp:133-140: views s, s+S, ...). */
+int xpr_preprocess_views(xpr_stream s, const float *in, uint64_t in_view_stride, void *out,
+                         int out_dtype, uint64_t n, uint32_t n_views, int normalize, int mlog,
+                         float mn, float mx);
 
 /* ---- tile-shape tuning (no counterpart in the reference: its kernels have a
  *      fixed work-item shape, gpu/kernel.hpp:73-81) ------------------------------

@@ -460,6 +460,19 @@ int xpr_preprocess(xpr_stream s, const float *in, void *out, int odt, uint64_t n
     return XPR_OK;
 }
 
+int xpr_preprocess_views(xpr_stream s, const float *in, uint64_t stride, void *out, int odt,
+                         uint64_t n, uint32_t nv, int, int, float, float)
+{
+    if (stride < n) return fail(XPR_E_INVALID, "views overlap");
+    S(s).push([=] {
+        for (uint32_t j = 0; j < nv; ++j) {
+            touch(in + uint64_t(j) * stride, n * 4, false);
+            touch(static_cast<char *>(out) + uint64_t(j) * n * dtype_bytes(odt), n * dtype_bytes(odt), true);
+        }
+    });
+    return XPR_OK;
+}
+
 } // extern "C"
 
 // The batch engine: one stream, two host tables; a launch copies its slot's

@@ -14,7 +14,7 @@ from . import _lib
 from ._lib import ProjArgs, XprError, proj_args  # noqa: F401
 
 __all__ = ["forward", "error", "backward", "subiteration", "fill", "fill_u32", "fill_u64", "convert",
-           "preprocess", "minmax_normal", "proj_args", "ProjArgs", "XprError",
+           "preprocess", "preprocess_views", "minmax_normal", "proj_args", "ProjArgs", "XprError",
            "dtype_code", "acc_bits", "tune_forward", "tune_backward", "tune_reset",
            "check_fast_paths"]
 
@@ -175,6 +175,17 @@ def preprocess(data, out, normalize: bool, mlog: bool, mn: float = 0.0, mx: floa
         raise XprError("preprocess: output smaller than input")
     _lib.call("xpr_preprocess", _stream(stream), _ptr(data), _ptr(out), dtype_code(out),
               data.numel(), int(normalize), int(mlog), float(mn), float(mx))
+
+
+def preprocess_views(data, out, step: int, normalize: bool, mlog: bool, mn: float = 0.0,
+                     mx: float = 1.0, stream=None) -> None:
+    """preprocess of the views 0, step, 2 step, ... of data ([N][rows][cols]),
+    as many as out ([n][rows][cols]) holds, in one launch."""
+    n_views, per = out.shape[0], out[0].numel()
+    if data[0].numel() != per or (n_views - 1) * step >= data.shape[0]:
+        raise XprError("preprocess_views: shapes do not match")
+    _lib.call("xpr_preprocess_views", _stream(stream), _ptr(data), step * per, _ptr(out),
+              dtype_code(out), per, n_views, int(normalize), int(mlog), float(mn), float(mx))
 
 
 def tune_forward(volume, sums_scratch, projdata, args: ProjArgs, stream=None) -> int:

@@ -156,6 +156,8 @@ _SIGS = {
     "xpr_minmax_read": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "xpr_preprocess": (C.c_int, [_P, _P, _P, C.c_int, C.c_uint64, C.c_int, C.c_int,
                                  C.c_float, C.c_float]),
+    "xpr_preprocess_views": (C.c_int, [_P, _P, C.c_uint64, _P, C.c_int, C.c_uint64, C.c_uint32,
+                                       C.c_int, C.c_int, C.c_float, C.c_float]),
     "xpr_geometry": (C.c_int, [C.POINTER(GeometryParams), _P, C.POINTER(Grid)]),
     "xpr_geometry_frames": (C.c_int, [C.POINTER(GeometryParams), _P]),
     "xpr_reconstruct": (C.c_int, [C.c_char_p, C.c_char_p, C.c_size_t]),

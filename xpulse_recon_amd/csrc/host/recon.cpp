@@ -325,12 +325,15 @@ void Dataset::upload_sit_images(xpr_stream s, int64_t sit, int64_t row0, int64_t
     if (device_io()) {                              // preprocess straight from HBM
         if (row0 < 0 || rows <= 0 || row0 + rows > dheight_)
             throw std::runtime_error("detector row window outside the projections");
+        // the sub-iteration's views s, s+S, ... in one launch
         const auto views = sit_views(sit);
-        for (size_t j = 0; j < views.size(); ++j)
-            check(xpr_preprocess(s, p_.xpr.mem_projections + size_t(views[j] * dheight_ + row0) *
-                                        size_t(dwidth_),
-                                 out + j * n * dtype_bytes(dt), dt, uint64_t(n),
-                                 p_.recon.normalize, p_.recon.mlog, mn_, mx_), "preprocess");
+        if (views.empty()) return;
+        const uint64_t step = views.size() > 1 ? uint64_t(views[1] - views[0]) : 1;
+        check(xpr_preprocess_views(s, p_.xpr.mem_projections + size_t(views[0] * dheight_ + row0) *
+                                          size_t(dwidth_),
+                                   step * uint64_t(dheight_ * dwidth_), out, dt, uint64_t(n),
+                                   uint32_t(views.size()), p_.recon.normalize, p_.recon.mlog,
+                                   mn_, mx_), "preprocess");
         return;
     }
     stream_views(s, sit_views(sit), row0, rows, [&](const float *d, size_t j0, size_t c) {

@@ -115,6 +115,36 @@ __global__ __launch_bounds__(256) void k_preprocess(const float *__restrict__ in
     }
 }
 
+// k_preprocess over n_views views of n elements each, view j read from
+// in + j * in_stride and written to out + j * n (one launch for the views of
+// a sub-iteration, which lie at a fixed stride in the projection stack)
+template <typename OS>
+__global__ __launch_bounds__(256) void k_preprocess_views(const float *__restrict__ in,
+                                                          uint64_t in_stride,
+                                                          typename OS::T *__restrict__ out,
+                                                          uint64_t n, uint32_t n_views,
+                                                          int normalize, int mlog, float mn,
+                                                          float mx)
+{
+    const float hi_clamp = 1.0f - kQ22Eps;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint32_t j = blockIdx.y; j < n_views; j += gridDim.y) {
+        const float *__restrict__ src = in + (uint64_t)j * in_stride;
+        const uint64_t o = (uint64_t)j * n;
+        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+            float v = src[i];
+            if (normalize)
+                v = __builtin_isnormal(v) ? (v - mn) / (mx - mn) : (mx + mn) / 2.0f;
+            if (mlog) {
+                float t = (hi_clamp < v) ? hi_clamp : v;   // std::min
+                t = (t < kQ22Eps) ? kQ22Eps : t;           // std::max
+                v = -logf(t);
+            }
+            OS::store(out, o + i, v);
+        }
+    }
+}
+
 // Self-check of the exact fast forms (recip's fast branch and pixel_of)
 // over every float32 bit pattern; bad[0..2] as xpr_check_fast_paths counts.
 __global__ __launch_bounds__(256) void k_check_fast_paths(unsigned long long *__restrict__ bad)

@@ -1142,4 +1142,35 @@ int xpr_preprocess(xpr_stream s, const float *in, void *out, int out_dtype, uint
     return XPR_OK;
 }
 
+int xpr_preprocess_views(xpr_stream s, const float *in, uint64_t in_view_stride, void *out,
+                         int out_dtype, uint64_t n, uint32_t n_views, int normalize, int mlog,
+                         float mn, float mx)
+{
+    if (int rc = check_dtype(out_dtype)) return rc;
+    if (n == 0 || n_views == 0) return XPR_OK;
+    if (!in || !out) return fail(XPR_E_INVALID, "null buffer");
+    if (in_view_stride < n) return fail(XPR_E_INVALID, "views overlap: in_view_stride < n");
+    hipStream_t st = (hipStream_t)s;
+    // x covers a view, y the views (at most 64 rows of the grid; each row
+    // strides over the views)
+    const unsigned gx = grid_for(n);
+    const dim3 g(gx, std::min<uint32_t>(n_views, std::max<uint32_t>(1u, 65536u / gx)));
+    switch (out_dtype) {
+    case XPR_F32:
+        hipLaunchKernelGGL(k_preprocess_views<F32>, g, dim3(256), 0, st, in, in_view_stride,
+                           (float *)out, n, n_views, normalize, mlog, mn, mx);
+        break;
+    case XPR_F16:
+        hipLaunchKernelGGL(k_preprocess_views<F16>, g, dim3(256), 0, st, in, in_view_stride,
+                           (uint16_t *)out, n, n_views, normalize, mlog, mn, mx);
+        break;
+    default:
+        hipLaunchKernelGGL(k_preprocess_views<BF16>, g, dim3(256), 0, st, in, in_view_stride,
+                           (uint16_t *)out, n, n_views, normalize, mlog, mn, mx);
+        break;
+    }
+    XPR_LAUNCH_CHECK();
+    return XPR_OK;
+}
+
 } // extern "C"
