@@ -531,6 +531,8 @@ int engine_launch(Engine *e, int slot)
                     touch(it.dst, it.n * dtype_bytes(it.dt_dst), true);
                     break;
                 case kForward:
+                case kForwardF16:
+                case kForwardBF16:
                     touch(it.dst, slab_voxels(&it.a) * dtype_bytes(it.dt_dst), false);
                     touch_atomic(it.sums, det_samples(&it.a) * 8);
                     break;
@@ -540,6 +542,8 @@ int engine_launch(Engine *e, int slot)
                     touch(it.dst, it.n * dtype_bytes(it.dt_dst), true);
                     break;
                 case kBackward:
+                case kBackwardF16:
+                case kBackwardBF16:
                     touch(it.err, det_samples(&it.a) * dtype_bytes(it.dt_aux), false);
                     touch(it.dst, slab_voxels(&it.a) * dtype_bytes(it.dt_dst), true);
                     break;

@@ -4,10 +4,11 @@
 // One step is a table of work items grouped by phase.  Per phase, one
 // persistent kernel walks every item's work units (forward bricks, backward
 // row blocks, elementwise chunks) through an atomic counter, so a step runs
-// the sub-iteration pieces of many jobs of different sizes in five launches.
-// The launch sequence of a step (table copy, counter reset, convert, forward,
-// error, backward, fill) is captured once into a HIP graph per table slot and
-// replayed; two slots let the host write step k+1 while step k runs.
+// the sub-iteration pieces of many jobs of different sizes in nine launches.
+// The launch sequence of a step (table copy, counter reset, convert, the
+// three forwards, error, the three backwards, fill) is captured once into a
+// HIP graph per table slot and replayed; two slots let the host write step
+// k+1 while step k runs.
 #pragma once
 
 #include <cstdint>
@@ -16,8 +17,22 @@
 
 namespace xpr_batch {
 
-enum Phase : uint32_t { kConvert = 0, kForward = 1, kError = 2, kBackward = 3, kFill = 4 };
-constexpr int kPhases = 5;
+// The forward and the backward have one phase per volume storage type (F32,
+// F16, BF16), each walked by its own kernel: one kernel holding the three
+// paths would need the registers of all three and spill.
+enum Phase : uint32_t {
+    kConvert = 0, kForward = 1, kError = 2, kBackward = 3, kFill = 4,
+    kForwardF16 = 5, kForwardBF16 = 6, kBackwardF16 = 7, kBackwardBF16 = 8
+};
+constexpr int kPhases = 9;
+inline Phase forward_phase(int32_t dt)
+{
+    return dt == XPR_F16 ? kForwardF16 : dt == XPR_BF16 ? kForwardBF16 : kForward;
+}
+inline Phase backward_phase(int32_t dt)
+{
+    return dt == XPR_F16 ? kBackwardF16 : dt == XPR_BF16 ? kBackwardBF16 : kBackward;
+}
 constexpr int kMaxItems = 32;              // items per phase and step
 constexpr uint32_t kElemUnit = 1u << 16;   // elements per elementwise work unit
 

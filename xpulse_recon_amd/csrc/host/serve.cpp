@@ -305,7 +305,7 @@ struct BatchJob {
         char *vbase = static_cast<char *>(f32 ? vol32.get() : vol.get());
         if (phase == kFwd) {
             const int64_t n = layers_for(quota - used);
-            Item &f = add(kForward);
+            Item &f = add(xpr_batch::forward_phase(vdt));
             f.dst = vbase + size_t(layer) * size_t(W * W) * vb;
             f.sums = sums.get();
             f.pd = pds[size_t(s)].as<float>();
@@ -332,7 +332,7 @@ struct BatchJob {
             layer = 0;
         }
         const int64_t n = layers_for(std::max(quota - used, quantum_cost()));
-        Item &b = add(kBackward);
+        Item &b = add(xpr_batch::backward_phase(vdt));
         b.dst = vbase + size_t(layer) * size_t(W * W) * vb;
         b.err = f32 ? err32.get() : err.get();
         b.pd = pds[size_t(s)].as<float>();

@@ -107,13 +107,14 @@ __global__ __launch_bounds__(kThreads) void k_batch_reset(uint32_t *counters)
     if (threadIdx.x < (unsigned)xpr_batch::kPhases * kXcds) counters[threadIdx.x] = 0u;
 }
 
+// One kernel per volume storage type VS, walking the forward phase of that type.
+template <typename VS>
 __global__ __launch_bounds__(kThreads, 4) void k_batch_forward(const Table *__restrict__ t,
-                                                               uint32_t *counters)
+                                                               uint32_t *counters, uint32_t ph)
 {
     __shared__ unsigned long long lds[2 * kTileCap + kThreads];
     __shared__ ViewBin bins[kViewChunk];
     __shared__ uint32_t slot;
-    const uint32_t ph = xpr_batch::kForward;
     const uint32_t total = uniform_u32(&t->total[ph]);
     if (total == 0) return;
     for (int i = threadIdx.x; i < 2 * kTileCap; i += kThreads) lds[i] = 0ull;
@@ -123,34 +124,24 @@ __global__ __launch_bounds__(kThreads, 4) void k_batch_forward(const Table *__re
         const Item it = uniform_item(&t->items[ph][item_of(t, ph, u)]);
         const BrickShape s = batch_brick_shape(it.a);
         const int64_t b = (int64_t)(u - it.first);
-        uint64_t *sums = (uint64_t *)it.sums;
-        const void *vol = it.dst;
-        switch (it.dt_dst) {
-        case XPR_F32:
-            forward_brick<F32, uint64_t, false>((const float *)vol, sums, it.pd, it.a, s, b, lds, bins);
-            break;
-        case XPR_F16:
-            forward_brick<F16, uint64_t, false>((const uint16_t *)vol, sums, it.pd, it.a, s, b, lds, bins);
-            break;
-        default:
-            forward_brick<BF16, uint64_t, false>((const uint16_t *)vol, sums, it.pd, it.a, s, b, lds, bins);
-            break;
-        }
+        forward_brick<VS, uint64_t, false>((const typename VS::T *)it.dst, (uint64_t *)it.sums, it.pd, it.a, s,
+                                           b, lds, bins);
     }
 }
 
 // Backward units are the single-job launch's workgroups: segment unit % nseg
 // of the kWaves adjacent wave rows (layer group, depth) of row block
 // unit / nseg, one row per wave, run together so the four rows (which gather
-// nearly the same detector pixels) share the CU's L1.
+// nearly the same detector pixels) share the CU's L1.  One kernel per volume
+// storage type VS (the error samples share it).
+template <typename VS>
 __global__ __launch_bounds__(kThreads, 7) void k_batch_backward(const Table *__restrict__ t,
-                                                                uint32_t *counters)
+                                                                uint32_t *counters, uint32_t ph)
 {
     __shared__ uint32_t slot;
     constexpr int kRec = 64 * BwdRecord<kBwdL>::kQuads;
     __shared__ float4 rec[kWaves * kRec];             // per-wave view-record tables
     float4 *wrec = rec + (threadIdx.x >> 6) * kRec;
-    const uint32_t ph = xpr_batch::kBackward;
     const uint32_t total = uniform_u32(&t->total[ph]);
     if (total == 0) return;
     for (;;) {
@@ -161,20 +152,8 @@ __global__ __launch_bounds__(kThreads, 7) void k_batch_backward(const Table *__r
         const int64_t unit = (int64_t)(u - it.first);
         const int64_t row = (unit / s.nseg) * kWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
         const int seg = (int)(unit % s.nseg);
-        switch (it.dt_dst) {                                // volume and error share the dtype
-        case XPR_F32:
-            backward_row<F32, F32, kBwdJ, kBwdL, false>((float *)it.dst, (const float *)it.err, it.pd, it.a, s,
-                                                 row, seg, nullptr, wrec);
-            break;
-        case XPR_F16:
-            backward_row<F16, F16, kBwdJ, kBwdL, false>((uint16_t *)it.dst, (const uint16_t *)it.err, it.pd,
-                                                 it.a, s, row, seg, nullptr, wrec);
-            break;
-        default:
-            backward_row<BF16, BF16, kBwdJ, kBwdL, false>((uint16_t *)it.dst, (const uint16_t *)it.err, it.pd,
-                                                   it.a, s, row, seg, nullptr, wrec);
-            break;
-        }
+        backward_row<VS, VS, kBwdJ, kBwdL, false>((typename VS::T *)it.dst, (const typename VS::T *)it.err, it.pd,
+                                                  it.a, s, row, seg, nullptr, wrec);
     }
 }
 

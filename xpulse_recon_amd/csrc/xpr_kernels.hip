@@ -499,10 +499,20 @@ int capture(Engine &e, int k, int cus)
     hipLaunchKernelGGL(k_batch_reset, dim3(1), blk, 0, e.stream, e.counters[k]);
     hipLaunchKernelGGL(k_batch_elementwise, grid(per_cu[2]), blk, 0, e.stream, e.dev[k], e.counters[k],
                        (uint32_t)kConvert);
-    hipLaunchKernelGGL(k_batch_forward, grid(per_cu[0]), blk, 0, e.stream, e.dev[k], e.counters[k]);
+    hipLaunchKernelGGL(k_batch_forward<F32>, grid(per_cu[0]), blk, 0, e.stream, e.dev[k], e.counters[k],
+                       (uint32_t)kForward);
+    hipLaunchKernelGGL(k_batch_forward<F16>, grid(per_cu[0]), blk, 0, e.stream, e.dev[k], e.counters[k],
+                       (uint32_t)kForwardF16);
+    hipLaunchKernelGGL(k_batch_forward<BF16>, grid(per_cu[0]), blk, 0, e.stream, e.dev[k], e.counters[k],
+                       (uint32_t)kForwardBF16);
     hipLaunchKernelGGL(k_batch_elementwise, grid(per_cu[2]), blk, 0, e.stream, e.dev[k], e.counters[k],
                        (uint32_t)kError);
-    hipLaunchKernelGGL(k_batch_backward, grid(per_cu[1]), blk, 0, e.stream, e.dev[k], e.counters[k]);
+    hipLaunchKernelGGL(k_batch_backward<F32>, grid(per_cu[1]), blk, 0, e.stream, e.dev[k], e.counters[k],
+                       (uint32_t)kBackward);
+    hipLaunchKernelGGL(k_batch_backward<F16>, grid(per_cu[1]), blk, 0, e.stream, e.dev[k], e.counters[k],
+                       (uint32_t)kBackwardF16);
+    hipLaunchKernelGGL(k_batch_backward<BF16>, grid(per_cu[1]), blk, 0, e.stream, e.dev[k], e.counters[k],
+                       (uint32_t)kBackwardBF16);
     hipLaunchKernelGGL(k_batch_elementwise, grid(per_cu[2]), blk, 0, e.stream, e.dev[k], e.counters[k],
                        (uint32_t)kFill);
     const hipError_t le = hipGetLastError();
