@@ -157,16 +157,19 @@ def kernel_sha16():
     return h.hexdigest()[:16]
 
 
-def profile_counters():
+def profile_counters(vv_launch):
     """Per-launch hardware counters of the projectors on the headline
     workload, from the committed rocprofv3 summary: lines
-    "counters <kernel> <SQ_INSTS_VALU> <HBM bytes read> <HBM bytes written> <sha16>",
-    used only when sha16 is that of this build's kernel sources."""
-    out, sha = {}, kernel_sha16()
+    "counters <kernel> <SQ_INSTS_VALU> <HBM bytes read> <HBM bytes written> <sha16>"
+    and "launch_voxel_views <n>", used only when sha16 is that of this build's
+    kernel sources and n the voxel-views of this run's launches."""
+    out, sha, n = {}, kernel_sha16(), None
     try:
         with open(os.path.join(HERE, PROFILE)) as f:
             for line in f:
                 p = line.split()
+                if len(p) == 2 and p[0] == "launch_voxel_views":
+                    n = int(p[1])
                 if len(p) == 6 and p[0] == "counters":
                     if p[5] != sha:
                         return {}, f"{PROFILE} was made from other kernel sources ({p[5]} != {sha})"
@@ -174,6 +177,8 @@ def profile_counters():
                                  "write_bytes": float(p[4])}
     except OSError:
         return {}, f"{PROFILE} missing"
+    if out and n != vv_launch:
+        return {}, f"{PROFILE} was made with launches of {n} voxel-views, not {vv_launch}"
     return out, PROFILE if out else f"{PROFILE} holds no counters"
 
 
@@ -580,6 +585,9 @@ def headline(args):
     ms = f32["total_s"] / args.steps * 1e3
     rec0 = f32["recs"][-1]
     L = int(rec0["launches_per_pass"])
+    if H % L:
+        raise SystemExit(f"{L} launches per pass do not split {H} layers evenly")
+    LAY = H // L                 # layers per projector launch
 
     # ---- roofline of the dominant projector: its launches timed by the
     # engine with HIP events on the stream they run on, the VALU ceiling
@@ -590,8 +598,8 @@ def headline(args):
     name = max(kernels, key=lambda k: float(kernels[k].sum()))
     launch_ms = float(kernels[name].mean())
     ceiling = valu_ceiling()
-    counters, source = profile_counters()
-    vv_launch = W * W * SLAB * A_s
+    vv_launch = W * W * LAY * A_s
+    counters, source = profile_counters(vv_launch)
     c = counters.get(name)
     roof = {"bound": "valu", "unit": "lane-instr/s", "kernel": name,
             "launch_ms": round(launch_ms, 4), "peak": ceiling,
@@ -664,11 +672,11 @@ def headline(args):
         "launch_ms": {k: stats(xs) for k, xs in kernels.items()},
         "rmse_vs_phantom": f32["rmse"], "maxabs_vs_phantom": f32["maxabs"],
         "compulsory_hbm": {
-            "k_forward_bin": {"bytes_per_launch": W * W * SLAB * 4 + 2 * A_s * D * D * 8,
-                              "frac": round((W * W * SLAB * 4 + 2 * A_s * D * D * 8) /
+            "k_forward_bin": {"bytes_per_launch": W * W * LAY * 4 + 2 * A_s * D * D * 8,
+                              "frac": round((W * W * LAY * 4 + 2 * A_s * D * D * 8) /
                                             (float(fwd.mean()) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-            "k_backward": {"bytes_per_launch": 2 * W * W * SLAB * 4 + A_s * D * D * 4,
-                           "frac": round((2 * W * W * SLAB * 4 + A_s * D * D * 4) /
+            "k_backward": {"bytes_per_launch": 2 * W * W * LAY * 4 + A_s * D * D * 4,
+                           "frac": round((2 * W * W * LAY * 4 + A_s * D * D * 4) /
                                          (float(bwd.mean()) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}},
         "flops_per_voxel_view": {"algorithmic": 27,
                                  "note": "3 dot4 as 12 FMA + rcp + 2 mul + sub; the kernels "
@@ -718,11 +726,15 @@ def headline(args):
         "config": {"workload": f"{W}^3 grid from a {D}x{D} detector, {NV} views over 360 deg, "
                                f"sit={SIT} (A_s={A_s}), it={IT}, rx 2.5 px, roll 0.3 deg, "
                                f"method_1a slab {SLAB} with the layer store in "
-                               f"{rec0['layer_store']}, FP32 storage; one step = one complete "
+                               f"{rec0['layer_store']} ("
+                               + ("one launch per projector pass over the resident volume"
+                                  if L == 1 else f"{L} slab launches per projector pass")
+                               + "), FP32 storage; one step = one complete "
                                "reconstruction (preprocessing, 60 sub-iterations, volume out) "
                                "through xpr_reconstruct_dev",
                    "grid": [W, H, W], "views": NV, "sit": SIT, "it": IT,
                    "views_per_subiteration": A_s, "method": "method_1a", "slab": SLAB,
+                   "launches_per_pass": L,
                    "layer_store": rec0["layer_store"], "images_resident": rec0["images_resident"],
                    "voxel_views_per_reconstruction": vv_recon, "tile_shapes": tuned or "defaults"},
         "roofline": roof,

@@ -124,7 +124,10 @@ def test_streaming_schedules_equal_naive(scan, method, xpr):
         assert log["spill_compression"] == "zstd"          # libzstd.so.1 present
     if "slab_height" not in xpr:
         assert log["slab_height"] == 13
-    assert log["launches_per_pass"] == -(-H // log["slab_height"])
+    # an HBM store is one launch per pass over the resident volume, except
+    # method_1c's per-slab view lists
+    whole = xpr["layer_store"] == "device" and method != "method_1c"
+    assert log["launches_per_pass"] == (1 if whole else -(-H // log["slab_height"]))
     assert len(log["forward_pass_ms"]) == 7 and len(log["backward_pass_ms"]) == 8
     assert not list((t / name).glob("*.zstd"))          # spill removed at the end
 

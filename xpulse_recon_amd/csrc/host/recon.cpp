@@ -885,8 +885,12 @@ public:
         ops.setImageOffset(0, int(ds.height()));
         ops.setAngleNumber(As, mods);
         {
-            // tile shapes timed on a central slab holding 1.0 (error 0 keeps it)
-            const int64_t b = nslab_ / 2, l0 = b * hs_, l1 = std::min(H, l0 + hs_);
+            // tile shapes timed on a central slab holding 1.0 (error 0 keeps
+            // it); whole-volume passes on 128 central layers, like the naive
+            // schedule's
+            const int64_t b = nslab_ / 2;
+            const int64_t l0 = whole_ ? std::max<int64_t>(0, H / 2 - 64) : b * hs_;
+            const int64_t l1 = std::min(H, l0 + (whole_ ? 128 : hs_));
             void *v = store_ == kDevice ? static_cast<void *>(static_cast<char *>(vol_.get()) + size_t(l0) * layer_ * sb_)
                                         : dslab_[0].get();
             xpr_proj_args ta = ops.args();
@@ -916,7 +920,7 @@ public:
             }
         };
         if (resident_) upload(0);
-        note("launches_per_pass", Json::number(double(nslab_)));
+        note("launches_per_pass", Json::number(whole_ ? 1.0 : double(nslab_)));
         Event e0, e1, fb, fe, bb, be;
         for (int64_t mit = 0; mit < p_.recon.it; ++mit) {
             ops.setWeight(next_weight(mit));
@@ -984,6 +988,10 @@ private:
     int st_ = XPR_F32;
     size_t sb_ = 4, layer_ = 0;
     bool final32_ = false;           // last sub-iteration with float32 storage
+    // HBM layer store without per-slab view lists: a projector pass is one
+    // launch over the resident volume whatever the slab height (which then
+    // only sets the slab streaming granularity of the host stores)
+    bool whole_ = false;
     void *cvol_ = nullptr;           // the HBM volume the kernels update, its dtype and size
     int cst_ = XPR_F32;
     size_t csb_ = 4;
@@ -1058,6 +1066,7 @@ private:
         }
         resident_ = need(store_, hs_, true) <= budget();
         final32_ = store_ == kDevice && storage() != XPR_F32 && p_.xpr.final_fp32;
+        whole_ = store_ == kDevice && !cull_;
         note("final_fp32", Json::boolean(final32_));
         p_.xpr.layer_store = store_ == kDevice ? "device" : store_ == kRam ? "ram" : "zstd";
         if (store_ == kZstd) {
@@ -1142,8 +1151,23 @@ private:
 
     void pass(bool backward, bool first, bool last, int64_t s, Operators &ops)
     {
+        if (whole_) {
+            // the resident volume in one launch: the sums are integers and each
+            // voxel's update is its own, so they equal the slab launches' bit for
+            // bit, without a launch tail per slab
+            const int64_t W = ds_->vwidth(), H = ds_->vheight();
+            ops.setVolumeBuffer(cvol_, cst_);
+            ops.setVolumeParameters({0, 0, 0}, {int(W), int(H), int(W)}, ds_->vx());
+            ops.setProjDataBuffer(pds_[size_t(s)].as<float>());
+            ops.setViewIndex(nullptr);
+            ops.setAngleNumber(ds_->concurrent(), p_.detector.module_number);
+            if (backward) ops.backward();
+            else ops.forward();
+            return;
+        }
         if (store_ == kDevice) {
             // the layer store is the resident volume: slabs are launched in place
+            // (method_1c: each with its own view list)
             for (int64_t b = 0; b < nslab_; ++b) {
                 set_slab(ops, s, b, static_cast<char *>(cvol_) + size_t(b * hs_) * layer_ * csb_);
                 if (backward) ops.backward();
