@@ -42,10 +42,12 @@ sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
 # MI355X dense FP32 vector peak 157.3 TFLOP/s counts packed FP32 FMAs (two
-# per lane, two flops each); this build issues unpacked FP32, so its VALU
-# lane-instruction peak is 157.3e12 / 4.
+# per lane, two flops each): 157.3e12 / 4 unpacked FMA lane-instructions per
+# second.  That is no bound on other instruction kinds: gfx950 issues simple
+# integer instructions faster (xpr_valu_rates).
 FP32_PEAK_TFLOPS = 157.3
-VALU_PEAK_PUBLISHED = FP32_PEAK_TFLOPS * 1e12 / 4.0
+FMA_PEAK_PUBLISHED = FP32_PEAK_TFLOPS * 1e12 / 4.0
+VALU_KINDS = ["fma", "mul", "add", "8fma_8xor", "4fma_12xor", "xor"]
 METRIC = "voxel-proj updates/s vs plain FP32 at fixed RMSE; p50/p99 recon latency"
 D, NV, SIT, IT, SLAB = 768, 720, 20, 3, 64
 PROFILE = os.path.join("profiles", "pmc_768cube_f32.md")
@@ -182,11 +184,12 @@ def profile_counters(vv_launch):
     return out, PROFILE if out else f"{PROFILE} holds no counters"
 
 
-def valu_ceiling():
+def valu_rates():
+    """Measured VALU lane-instructions per second of each calibration kind."""
     from xpulse_recon_amd import _lib
-    v = C.c_double()
-    _lib.call("xpr_valu_ceiling", None, C.byref(v))
-    return v.value
+    v = (C.c_double * len(VALU_KINDS))()
+    _lib.call("xpr_valu_rates", None, v, len(VALU_KINDS))
+    return dict(zip(VALU_KINDS, list(v)))
 
 
 def stats(xs):
@@ -592,27 +595,37 @@ def headline(args):
     # ---- roofline of the dominant projector: its launches timed by the
     # engine with HIP events on the stream they run on, the VALU ceiling
     # measured now, instruction and byte counters from the committed profile
+    # (forwards with 32-bit detector sums, where the engine's hit bound allows
+    # them, run the 32-bit-sum instantiation of the kernel)
     fwd = np.concatenate([np.asarray(r["forward_pass_ms"]) for r in f32["recs"]]) / L
+    bits = np.concatenate([np.asarray(r.get("forward_sum_bits") or [64] * len(r["forward_pass_ms"]))
+                           for r in f32["recs"]])
     bwd = np.concatenate([np.asarray(r["backward_pass_ms"]) for r in f32["recs"]]) / L
-    kernels = {"k_forward_bin": fwd, "k_backward": bwd}
+    kernels = {"k_forward_bin": fwd[bits == 64], "k_forward_bin_u32": fwd[bits == 32],
+               "k_backward": bwd}
+    kernels = {k: v for k, v in kernels.items() if v.size}
+    sum_bytes = {"k_forward_bin": 8, "k_forward_bin_u32": 4}
     name = max(kernels, key=lambda k: float(kernels[k].sum()))
     launch_ms = float(kernels[name].mean())
-    ceiling = valu_ceiling()
+    rates = valu_rates()
+    ceiling = max(rates.values())
     vv_launch = W * W * LAY * A_s
     counters, source = profile_counters(vv_launch)
     c = counters.get(name)
     roof = {"bound": "valu", "unit": "lane-instr/s", "kernel": name,
             "launch_ms": round(launch_ms, 4), "peak": ceiling,
-            "peak_note": "VALU lane-instructions per second of xpr_valu_ceiling (16 chains of "
-                         "v_fma_f32, v_mul_f32 or v_add_f32 per lane, 8 waves per SIMD, best "
-                         "kind) measured in this run; achieved = SQ_INSTS_VALU per launch "
-                         "(committed profile of these kernel sources) x 64 / launch time",
-            "peak_published_derived": VALU_PEAK_PUBLISHED,
+            "peak_note": "VALU lane-instructions per second of the best kind of "
+                         "xpr_valu_rates (16 independent chains per lane, 8 waves per SIMD: "
+                         "v_fma_f32, v_mul_f32, v_add_f32, or FMAs mixed with two-VGPR v_xor_b32, "
+                         "which gfx950 issues faster) measured in this run; achieved = "
+                         "SQ_INSTS_VALU per launch (committed profile of these kernel sources) x 64 "
+                         "/ launch time",
+            "peak_kinds": {k: round(v, -9) for k, v in rates.items()},
+            "fma_peak_published_derived": FMA_PEAK_PUBLISHED,
             "counters": source}
     if c:
         ach = c["valu_instr"] * 64.0 / (launch_ms * 1e-3)
         roof.update({"achieved": ach, "frac": round(ach / ceiling, 4),
-                     "frac_of_published_derived": round(ach / VALU_PEAK_PUBLISHED, 4),
                      "valu_instr_per_launch": c["valu_instr"],
                      "valu_instr_per_voxel_view": round(c["valu_instr"] * 64.0 / vv_launch, 3),
                      "traffic": int(c["fetch_bytes"] + c["write_bytes"])})
@@ -671,13 +684,12 @@ def headline(args):
         "sub_iteration_ms": stats(subit),
         "launch_ms": {k: stats(xs) for k, xs in kernels.items()},
         "rmse_vs_phantom": f32["rmse"], "maxabs_vs_phantom": f32["maxabs"],
+        "forward_sum_bits": {"32": int((bits == 32).sum()), "64": int((bits == 64).sum())},
         "compulsory_hbm": {
-            "k_forward_bin": {"bytes_per_launch": W * W * LAY * 4 + 2 * A_s * D * D * 8,
-                              "frac": round((W * W * LAY * 4 + 2 * A_s * D * D * 8) /
-                                            (float(fwd.mean()) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-            "k_backward": {"bytes_per_launch": 2 * W * W * LAY * 4 + A_s * D * D * 4,
-                           "frac": round((2 * W * W * LAY * 4 + A_s * D * D * 4) /
-                                         (float(bwd.mean()) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}},
+            k: {"bytes_per_launch": b,
+                "frac": round(b / (float(kernels[k].mean()) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            for k, b in [(k, W * W * LAY * 4 + 2 * A_s * D * D * sum_bytes[k]) for k in sum_bytes
+                         if k in kernels] + [("k_backward", 2 * W * W * LAY * 4 + A_s * D * D * 4)]},
         "flops_per_voxel_view": {"algorithmic": 27,
                                  "note": "3 dot4 as 12 FMA + rcp + 2 mul + sub; the kernels "
                                          "hoist the (y,z,w) part of each dot per row and share W "

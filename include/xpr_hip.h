@@ -172,6 +172,20 @@ int xpr_error(xpr_stream s, const void *images, int img_dtype, const void *sums,
 int xpr_backward(xpr_stream s, void *volume, int vol_dtype, const void *err, int err_dtype,
                  const float *projdata, const xpr_proj_args *a, uint64_t *empty_count);
 
+/* No counterpart in the reference.  xpr_backward that also reports the
+ * largest float32 voxel value of the slab after the update, updated or kept
+ * (computed before a 16-bit store rounds it): max_bits is a device array of
+ * XPR_MAX_SLOTS uint32, each raised to the bits of the largest value of the
+ * rows it serves (spread so that the waves' atomics do not queue on one
+ * address); the largest slot is the slab's.  Non-negative floats order as
+ * their bits; a negative or NaN value reads as a huge one.  Clear the slots
+ * before the call.  For callers that bound the next forward's detector sums
+ * (xpr_hit_bound). */
+#define XPR_MAX_SLOTS 1024
+int xpr_backward_max(xpr_stream s, void *volume, int vol_dtype, const void *err, int err_dtype,
+                     const float *projdata, const xpr_proj_args *a, uint64_t *empty_count,
+                     uint32_t *max_bits);
+
 /* One sub-iteration of the slab (the body of ReconstructionN::exec's loop,
  * reconstruction_n.hpp:66-76, with the values setOrigin / setWeight would
  * set, src/reconstruction.hpp:255-272, in *a and weight):
@@ -270,10 +284,15 @@ int xpr_check_fast_paths(xpr_stream s, uint64_t mismatches[3]);
 /* No counterpart in the reference: the device's VALU issue ceiling, the
  * roofline of the projectors (VALU-bound: a dozen vector instructions per
  * voxel-view against well under a byte of HBM traffic).  Kernels of 16
- * independent v_fma_f32, v_mul_f32 or v_add_f32 chains per lane, one VGPR
- * read per instruction, at 8 waves per SIMD on every CU; returns vector
- * lane-instructions per second, the best of five launches of each kind
- * after a warm-up.  Synchronous; allocates a small output buffer. */
+ * independent instruction chains per lane at 8 waves per SIMD on every CU,
+ * one kind each: 0 v_fma_f32, 1 v_mul_f32, 2 v_add_f32 (one VGPR read each),
+ * 3, 4, 5 with 8, 12, 16 of the chains v_xor_b32 of two VGPRs and the rest
+ * v_fma_f32 (gfx950 issues such integer instructions faster than FMAs).
+ * rates[k] receives kind k's vector lane-instructions per second (the best of
+ * five launches after a warm-up; n_rates up to 6, 0 beyond);
+ * xpr_valu_ceiling the best kind's.  Synchronous; allocate a small output
+ * buffer. */
+int xpr_valu_rates(xpr_stream s, double *rates, int n_rates);
 int xpr_valu_ceiling(xpr_stream s, double *lane_instr_per_s);
 
 /* ---- geometry (Geometry, dataset/geometry.hpp:35-76, 182-328) ----------- */
@@ -312,6 +331,18 @@ int xpr_geometry(const xpr_geometry_params *p, float *projdata_out, xpr_grid *gr
  * position, upper-left detector corner, column step, row step (mm per pixel),
  * vpl, vpb (geometry.hpp:244-292).  Used by the synthetic-data generator. */
 int xpr_geometry_frames(const xpr_geometry_params *p, double *frames_out);
+
+/* No counterpart in the reference.  *hits receives a bound on the voxels of
+ * an xpr_forward launch of args *a whose nearest pixel is one pixel of one
+ * view (any pixel, any view, summed over a view's modules), valid for every
+ * origin in [a->origin, a->origin + a->voxel] per axis: pass the grid origin
+ * without jitter.  projdata: host copy of the launch's n_views * modules
+ * records.  +inf when some voxel of the box is not in front of the source.
+ * Every 64-bit detector sum of the launch is then at most *hits times the
+ * largest Q.22 voxel value, and when that is below 2^32 the launch's 32-bit
+ * sums (XPR_ACC_U32) equal its 64-bit sums.  a->view_index must be NULL.
+ * Host function, no device work. */
+int xpr_hit_bound(const float *projdata, const xpr_proj_args *a, double *hits);
 
 /* ---- synthetic data (no counterpart in the reference: test and benchmark
  *      input generation) ------------------------------------------------------ */

@@ -382,8 +382,8 @@ int xpr_error(xpr_stream s, const void *img, int idt, const void *sums, int acc,
     });
     return XPR_OK;
 }
-int xpr_backward(xpr_stream s, void *vol, int vdt, const void *err, int edt, const float *pd,
-                 const xpr_proj_args *a, uint64_t *empty)
+int xpr_backward_max(xpr_stream s, void *vol, int vdt, const void *err, int edt, const float *pd,
+                     const xpr_proj_args *a, uint64_t *empty, uint32_t *max_bits)
 {
     if (!a || !vol || !err || !pd) return fail(XPR_E_INVALID, "null");
     const xpr_proj_args c = *a;
@@ -392,8 +392,15 @@ int xpr_backward(xpr_stream s, void *vol, int vdt, const void *err, int edt, con
         touch(err, det_samples(&c) * dtype_bytes(edt), false);
         touch(vol, slab_voxels(&c) * dtype_bytes(vdt), true);
         if (empty) touch_atomic(empty, 8);
+        if (max_bits) touch_atomic(max_bits, 4 * XPR_MAX_SLOTS);
     });
     return XPR_OK;
+}
+
+int xpr_backward(xpr_stream s, void *vol, int vdt, const void *err, int edt, const float *pd,
+                 const xpr_proj_args *a, uint64_t *empty)
+{
+    return xpr_backward_max(s, vol, vdt, err, edt, pd, a, empty, nullptr);
 }
 int xpr_tune_forward(xpr_stream s, const void *vol, int vdt, void *sums, int acc, const float *pd,
                      const xpr_proj_args *a, int *choice)

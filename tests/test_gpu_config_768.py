@@ -218,14 +218,14 @@ def raw_projections(scan, cuda):
     return proj
 
 
-def _reconstruct(proj, method, storage, store="ram"):
+def _reconstruct(proj, method, storage, store="ram", **xpr):
     from xpulse_recon_amd import _lib
     params = {"recon": {"method": method, "it": 1, "sit": SIT, "mlog": True, "weight": 1.0,
                         "weight_factor": 0.9},
               "geometry": {"so": 100.0, "sd": 1000.0},
               "detector": {"px": 0.1, "rx": 2.5, "roll": 0.3},
               "xpr": {"jitter_seed": 7, "layer_store": store, "slab_height": SLAB,
-                      "storage": storage, "final_fp32": False, "quiet": True}}
+                      "storage": storage, "final_fp32": False, "quiet": True, **xpr}}
     vol = np.empty((D, D, D), np.float32)
     rec = C.create_string_buffer(1 << 16)
     print(f"xpr_reconstruct_mem {method} {storage} {store} 768^3", flush=True)
@@ -248,3 +248,16 @@ def test_engine_method_1a_equals_naive(scan, raw_projections, storage, store):
     assert rs["launches_per_pass"] == (1 if store == "device" else D // SLAB)
     assert np.isfinite(naive).all()
     np.testing.assert_array_equal(slab, naive)
+
+
+def test_narrow_sums_equal_wide_sums(scan, raw_projections):
+    """At full size: method_1a over the resident volume with 32-bit detector
+    sums wherever the hit bound allows them (the default) equals the naive
+    schedule with 64-bit sums throughout, bit for bit."""
+    wide, rw = _reconstruct(raw_projections, "naive", "f32", narrow_sums=False)
+    narrow, rn = _reconstruct(raw_projections, "method_1a", "f32", "device")
+    bits = rn["forward_sum_bits"]
+    print(f"hit bound {rn['hit_bound']:.0f}; 32-bit forwards {bits.count(32)} of {len(bits)}", flush=True)
+    assert set(rw["forward_sum_bits"]) == {64}
+    assert bits.count(32) >= len(bits) // 2
+    np.testing.assert_array_equal(narrow, wide)

@@ -139,6 +139,70 @@ def test_backward_parity(cuda, case, tilt):
     assert rel_l2(got, ref) < 1e-6
 
 
+def _band_case():
+    """All 36 layers against 8 held detector rows (14-21): the outer layers
+    have no in-bounds view."""
+    spec = small_spec(40, 36, 10)
+    pd, grid = projdata_for(spec, [1, 4, 7])
+    d = geom_args(spec, grid, 3, line_offset=14, line_rows=8, jitter=(0.002, 0.003, 0.001))
+    return spec, pd, d
+
+
+@pytest.mark.parametrize("case", list(range(len(CASES))) + ["band"])
+@pytest.mark.parametrize("dtype", ["f32", "f16"])
+def test_backward_max_reports_the_largest_value(cuda, case, dtype):
+    """xpr_backward_max updates the volume exactly as xpr_backward and raises
+    max_bits to the largest value after the update, kept voxels (no in-bounds
+    view) included: in the band case one of them holds the maximum."""
+    import torch
+    import xpulse_recon_amd as X
+    spec, pd, d = _band_case() if case == "band" else _case(case)
+    nx, nl, nk = (d["vmax"][i] - d["vmin"][i] for i in range(3))
+    rng = np.random.default_rng(400 + (9 if case == "band" else case))
+    vol = rng.uniform(0.5, 2.0, (nl, nk, nx)).astype(np.float32)
+    err = rng.normal(0.0, 0.4, (d["n_views"], d["line_rows"], d["det_width"])).astype(np.float32)
+    # voxels with no in-bounds view keep their value even under a NaN error
+    probe = vol.copy()
+    O.backward_f32(probe, np.full_like(err, np.nan), pd, oracle_geom(d))
+    kept = np.argwhere(np.isfinite(probe))
+    assert (len(kept) > 0) == (case == "band")
+    if len(kept):
+        vol[tuple(kept[len(kept) // 2])] = 50.0
+    tdt = {"f32": torch.float32, "f16": torch.float16}[dtype]
+    te, tp = torch.from_numpy(err).to(cuda).to(tdt), torch.from_numpy(pd).to(cuda)
+    plain = torch.from_numpy(vol).to(cuda).to(tdt)
+    X.backward(plain, te, tp, _args(d))
+    tv = torch.from_numpy(vol).to(cuda).to(tdt)
+    top = torch.zeros(X.MAX_SLOTS, dtype=torch.int32, device=cuda)
+    X.backward(tv, te, tp, _args(d), max_bits=top)
+    assert torch.equal(tv, plain)
+    got = float(np.array([int(top.max())], np.int32).view(np.float32)[0])
+    stored = float(tv.float().max())
+    if dtype == "f32":
+        assert got == stored
+    else:                                     # the value before the 16-bit store rounds it
+        assert abs(got - stored) <= stored * 2.0 ** -11
+    if len(kept):
+        assert got == 50.0 == stored
+
+
+def test_valu_rates_and_ceiling(cuda):
+    """xpr_valu_rates measures every calibration kind; xpr_valu_ceiling is the
+    best of them (separate launches: within 10 %).  The integer mixes issue at
+    least as fast as FMAs alone."""
+    import ctypes as C
+    from xpulse_recon_amd import _lib
+    v = (C.c_double * 6)()
+    _lib.call("xpr_valu_rates", None, v, 6)
+    best = C.c_double()
+    _lib.call("xpr_valu_ceiling", None, C.byref(best))
+    rates = list(v)
+    print("VALU lane-instructions/s per kind", [f"{r:.3e}" for r in rates], flush=True)
+    assert all(r > 1e12 for r in rates)
+    assert 0.9 * max(rates) <= best.value <= 1.1 * max(rates)
+    assert max(rates[3:]) >= 0.9 * rates[0]
+
+
 def test_error_parity(cuda):
     import torch
     import xpulse_recon_amd as X

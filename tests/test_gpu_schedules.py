@@ -132,6 +132,26 @@ def test_streaming_schedules_equal_naive(scan, method, xpr):
     assert not list((t / name).glob("*.zstd"))          # spill removed at the end
 
 
+@pytest.mark.parametrize("method,xpr", [
+    ("naive", {}),
+    ("naive", {"storage": "f16"}),
+    ("method_1a", {"layer_store": "device", "slab_height": 7}),
+])
+def test_narrow_sums_equal_wide_sums(scan, method, xpr):
+    """32-bit detector sums wherever the hit bound times the volume's largest
+    Q.22 value stays below 2^32 (xpr.narrow_sums, the default) give the volume
+    of 64-bit sums bit for bit; the run record says which forwards took them."""
+    t, spec, d = scan
+    tag = f"{method}_{xpr.get('storage', 'f32')}"
+    wide, lw = _run(t, f"wide_{tag}", d, method=method, xpr=dict(xpr, narrow_sums=False))
+    narrow, ln = _run(t, f"narrow_{tag}", d, method=method, xpr=dict(xpr))
+    np.testing.assert_array_equal(narrow, wide)
+    assert set(lw["forward_sum_bits"]) == {64} and "hit_bound" not in lw
+    assert 32 in ln["forward_sum_bits"]
+    assert len(ln["forward_sum_bits"]) == len(ln["forward_pass_ms"])
+    assert 0 < ln["hit_bound"] < W * W * H
+
+
 @pytest.mark.parametrize("storage", ["f16", "bf16"])
 def test_slab_final_fp32_equals_naive(scan, storage):
     """16-bit storage with the slab schedule's store in HBM: its final

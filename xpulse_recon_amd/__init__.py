@@ -16,7 +16,7 @@ from ._lib import ProjArgs, XprError, proj_args  # noqa: F401
 __all__ = ["forward", "error", "backward", "subiteration", "fill", "fill_u32", "fill_u64", "convert",
            "preprocess", "preprocess_views", "minmax_normal", "proj_args", "ProjArgs", "XprError",
            "dtype_code", "acc_bits", "tune_forward", "tune_backward", "tune_reset",
-           "check_fast_paths"]
+           "check_fast_paths", "hit_bound"]
 
 
 def _stream(stream):
@@ -97,13 +97,41 @@ def error(images, sums, err, weight: float, n: int | None = None, stream=None) -
               acc_bits(sums), _ptr(err), dtype_code(err), n, float(weight))
 
 
-def backward(volume, err, projdata, args: ProjArgs, empty=None, stream=None) -> None:
-    """Backward projection + multiplicative update (KernelBackward.cl:8-41)."""
+MAX_SLOTS = 1024          # XPR_MAX_SLOTS
+
+
+def backward(volume, err, projdata, args: ProjArgs, empty=None, stream=None, max_bits=None) -> None:
+    """Backward projection + multiplicative update (KernelBackward.cl:8-41).
+    max_bits (optional, a device tensor of MAX_SLOTS 32-bit elements, cleared
+    by the caller): its slots are raised to the bits of the largest voxel
+    values after the update; the largest slot is the volume's
+    (xpr_backward_max)."""
     _need_cuda(volume, err, projdata)
     _check_sizes(volume, err, projdata, args)
+    if max_bits is not None:
+        if not max_bits.is_cuda or max_bits.numel() < MAX_SLOTS or max_bits.element_size() != 4:
+            raise XprError("max_bits must be a device tensor of MAX_SLOTS 32-bit elements")
+        _lib.call("xpr_backward_max", _stream(stream), _ptr(volume), dtype_code(volume), _ptr(err),
+                  dtype_code(err), _ptr(projdata), C.byref(args),
+                  None if empty is None else _ptr(empty), _ptr(max_bits))
+        return
     _lib.call("xpr_backward", _stream(stream), _ptr(volume), dtype_code(volume), _ptr(err),
               dtype_code(err), _ptr(projdata), C.byref(args),
               None if empty is None else _ptr(empty))
+
+
+def hit_bound(projdata, args: ProjArgs) -> float:
+    """Bound on the voxels of a forward launch of args whose nearest pixel is
+    one pixel of one view, for any jitter of args.origin up to one voxel
+    (xpr_hit_bound; projdata: host float32 records).  Below 2^32 / the largest
+    Q.22 voxel value, 32-bit sums equal 64-bit sums."""
+    import numpy as np
+    pd = np.ascontiguousarray(projdata, dtype=np.float32)
+    if pd.size < 20 * args.n_views * max(args.modules, 1):
+        raise XprError("projdata holds fewer records than args.n_views * args.modules")
+    out = C.c_double()
+    _lib.call("xpr_hit_bound", pd.ctypes.data_as(C.c_void_p), C.byref(args), C.byref(out))
+    return out.value
 
 
 def subiteration(volume, images, projdata, args: ProjArgs, weight: float, first_pass: bool,

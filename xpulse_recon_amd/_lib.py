@@ -137,6 +137,7 @@ _SIGS = {
     "xpr_forward": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, _P, C.POINTER(ProjArgs)]),
     "xpr_error": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, _P, C.c_int, C.c_uint64, C.c_float]),
     "xpr_backward": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, _P, C.POINTER(ProjArgs), _P]),
+    "xpr_backward_max": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, _P, C.POINTER(ProjArgs), _P, _P]),
     "xpr_subiteration": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, _P, C.POINTER(ProjArgs),
                                    C.c_float, C.c_int, _P, C.c_int, _P, C.c_int, _P]),
     "xpr_subiteration_static": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, _P,
@@ -150,6 +151,7 @@ _SIGS = {
     "xpr_check_fast_paths": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "xpr_q22_to_float": (C.c_int, [_P, _P, C.c_int, _P, C.c_uint64]),
     "xpr_valu_ceiling": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "xpr_valu_rates": (C.c_int, [_P, _P, C.c_int]),
     "xpr_scale": (C.c_int, [_P, _P, C.c_uint64, C.c_float]),
     "xpr_minmax_init": (C.c_int, [_P, _P]),
     "xpr_minmax_normal": (C.c_int, [_P, _P, C.c_uint64, _P]),
@@ -160,6 +162,7 @@ _SIGS = {
                                        C.c_int, C.c_int, C.c_float, C.c_float]),
     "xpr_geometry": (C.c_int, [C.POINTER(GeometryParams), _P, C.POINTER(Grid)]),
     "xpr_geometry_frames": (C.c_int, [C.POINTER(GeometryParams), _P]),
+    "xpr_hit_bound": (C.c_int, [_P, C.POINTER(ProjArgs), C.POINTER(C.c_double)]),
     "xpr_reconstruct": (C.c_int, [C.c_char_p, C.c_char_p, C.c_size_t]),
     "xpr_template_json": (C.c_int, [C.c_char_p, C.c_size_t]),
     "xpr_reconstruct_dev": (C.c_int, [C.c_char_p, _P, C.c_int64, C.c_int64, C.c_int64, _P,

@@ -216,8 +216,147 @@ void build(const xpr_geometry_params &p, float *out, xpr_grid &grid, double *fra
 
 } // namespace xpr_geom
 
+// ---------------------------------------------------------------------------
+// Voxel hits per detector pixel of one forward launch (no counterpart in the
+// reference: it lets the engine use 32-bit detector sums exactly).
+//
+// H bounds the number of voxels of the launch whose computed nearest pixel is
+// one given pixel of one view, for every origin in [origin, origin + voxel]
+// per axis (any jitter).  Every detector sum is then at most H times the
+// largest Q.22 voxel value.
+//
+// Voxel i = (x, layer, depth) sits at P = origin + voxel * i, inside the box
+// from origin + voxel * vmin to origin + voxel * vmax.  The projection
+// u = X/W, v = Y/W (ProjData rows 0, 1 and 3) is smooth where W > 0, which
+// must hold on the box.  The projector's float evaluation (the rounded
+// position, three FMAs per row, the reciprocal, the product) stays within
+//   du = 24 eps ((Tx + U Tw) / Wmin + U),  eps = 2^-24,
+// of the exact u of the unrounded position (three times the constant of the
+// forward's tile bound; Tx, Tw: sums of |coefficient x coordinate|, U the
+// largest |u|), likewise dv, so two voxels on one pixel have exact u within
+// 1 + 2 du and v within 1 + 2 dv of each other.  In one lattice plane (x,
+// layer or depth fixed) their difference d satisfies, by the mean value
+// theorem, du_diff = grad u(xi) . d and dv_diff = grad v(eta) . d with every
+// gradient entry in an interval over the box: d u / d i_a =
+// voxel_a (m_a W - m_(12+a) X) / W^2, whose affine numerator takes its range
+// at the corners and W^2 between its corner extremes.  With those rows
+// scaled by 1 / (1 + 2 du), 1 / (1 + 2 dv), diagonal dominance of either
+// pairing of rows and in-plane axes gives |J d|_inf >= mu |d|_inf for every
+// J of the intervals, hence |d|_inf <= 1 / mu: the voxels of one plane on one
+// pixel span at most floor(1 / mu) + 1 lattice values per in-plane axis.  A
+// record's bound is the least over the three plane families of
+// (planes) x (floor(1 / mu) + 1)^2; a view's is the sum over its modules.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Interval {
+    double lo, hi;
+};
+double abs_lo(Interval a) { return a.lo > 0.0 ? a.lo : a.hi < 0.0 ? -a.hi : 0.0; }
+double abs_hi(Interval a) { return std::fmax(std::fabs(a.lo), std::fabs(a.hi)); }
+
+double record_hits(const float *m, const double (&lo)[3], const double (&hi)[3],
+                   const double (&vox)[3], const int64_t (&n)[3], double line_off)
+{
+    const double inf = HUGE_VAL;
+    double X[8], Y[8], W[8], tx = 0.0, ty = 0.0, tw = 0.0, xmax = 0.0, ymax = 0.0;
+    double wlo = inf, whi = -inf;
+    for (int c = 0; c < 8; ++c) {
+        const double p[3] = {(c & 1) ? hi[0] : lo[0], (c & 2) ? hi[1] : lo[1], (c & 4) ? hi[2] : lo[2]};
+        X[c] = double(m[0]) * p[0] + double(m[1]) * p[1] + double(m[2]) * p[2] + double(m[3]);
+        Y[c] = double(m[4]) * p[0] + double(m[5]) * p[1] + double(m[6]) * p[2] + double(m[7]);
+        W[c] = double(m[12]) * p[0] + double(m[13]) * p[1] + double(m[14]) * p[2] + double(m[15]);
+        double ax = std::fabs(double(m[3])), ay = std::fabs(double(m[7])), aw = std::fabs(double(m[15]));
+        for (int a = 0; a < 3; ++a) {
+            ax += std::fabs(double(m[a]) * p[a]);
+            ay += std::fabs(double(m[4 + a]) * p[a]);
+            aw += std::fabs(double(m[12 + a]) * p[a]);
+        }
+        tx = std::fmax(tx, ax);
+        ty = std::fmax(ty, ay);
+        tw = std::fmax(tw, aw);
+        xmax = std::fmax(xmax, std::fabs(X[c]));
+        ymax = std::fmax(ymax, std::fabs(Y[c]));
+        wlo = std::fmin(wlo, W[c]);
+        whi = std::fmax(whi, W[c]);
+    }
+    if (!(wlo > 0.0) || !std::isfinite(whi) || !std::isfinite(tx + ty + tw)) return inf;
+    constexpr double eps = 1.0 / 16777216.0;
+    const double U = xmax / wlo + 1.0, V = ymax / wlo + std::fabs(line_off) + 1.0;
+    const double su = 1.0 + 2.0 * (24.0 * eps * ((tx + U * tw) / wlo + U) + 1e-4);
+    const double sv = 1.0 + 2.0 * (24.0 * eps * ((ty + V * tw) / wlo + V) + 1e-4);
+    // g[r][a]: interval of d(u, v)[r] / d i_a over the box, scaled by 1 / s
+    Interval g[2][3];
+    for (int r = 0; r < 2; ++r) {
+        const double *XY = r ? Y : X;
+        for (int a = 0; a < 3; ++a) {
+            double nlo = inf, nhi = -inf, mag = 0.0;
+            for (int c = 0; c < 8; ++c) {
+                const double t0 = double(m[4 * r + a]) * W[c], t1 = double(m[12 + a]) * XY[c];
+                nlo = std::fmin(nlo, t0 - t1);
+                nhi = std::fmax(nhi, t0 - t1);
+                mag = std::fmax(mag, std::fabs(t0) + std::fabs(t1));
+            }
+            nlo -= 1e-12 * mag;                          // the double evaluation above
+            nhi += 1e-12 * mag;
+            const double w2lo = wlo * wlo, w2hi = whi * whi;
+            Interval q;
+            if (nlo >= 0.0) q = {nlo / w2hi, nhi / w2lo};
+            else if (nhi <= 0.0) q = {nlo / w2lo, nhi / w2hi};
+            else q = {nlo / w2lo, nhi / w2lo};
+            const double k = vox[a] / (r ? sv : su);
+            g[r][a] = {q.lo * k, q.hi * k};
+        }
+    }
+    double best = inf;
+    for (int c = 0; c < 3; ++c) {
+        const int a = (c + 1) % 3, b = (c + 2) % 3;
+        const double mu1 = std::fmin(abs_lo(g[0][a]) - abs_hi(g[0][b]), abs_lo(g[1][b]) - abs_hi(g[1][a]));
+        const double mu2 = std::fmin(abs_lo(g[0][b]) - abs_hi(g[0][a]), abs_lo(g[1][a]) - abs_hi(g[1][b]));
+        const double mu = std::fmax(mu1, mu2) * (1.0 - 1e-9);
+        if (!(mu > 0.0)) continue;
+        const double k = std::floor(1.0 / mu) + 1.0;
+        best = std::fmin(best, double(n[c]) * k * k);
+    }
+    return best;
+}
+
+} // namespace
+
 // error reporting shared with the kernels' translation unit
 int xprint_fail(int code, const char *msg);
+
+extern "C" int xpr_hit_bound(const float *projdata, const xpr_proj_args *a, double *hits)
+{
+    if (!projdata || !a || !hits) return xprint_fail(XPR_E_INVALID, "null pointer");
+    if (a->view_index) return xprint_fail(XPR_E_INVALID, "xpr_hit_bound: indexed views");
+    double lo[3], hi[3], vox[3];
+    int64_t n[3];
+    for (int i = 0; i < 3; ++i) {
+        if (a->vmax[i] < a->vmin[i]) return xprint_fail(XPR_E_INVALID, "xpr_hit_bound: empty box");
+        n[i] = int64_t(a->vmax[i]) - a->vmin[i];
+        vox[i] = double(a->voxel[i]);
+        if (!(vox[i] > 0.0)) return xprint_fail(XPR_E_INVALID, "xpr_hit_bound: voxel size");
+        const double o = double(a->origin[i]);
+        lo[i] = o + vox[i] * a->vmin[i];
+        hi[i] = o + vox[i] * a->vmax[i];
+        // the origin may carry any jitter in [0, voxel] after its float rounding
+        lo[i] -= 1e-6 * std::fabs(lo[i]);
+        hi[i] += 1e-6 * std::fabs(hi[i]);
+    }
+    double worst = 0.0;
+    if (n[0] > 0 && n[1] > 0 && n[2] > 0) {
+        const uint32_t mods = a->modules ? a->modules : 1;
+        for (uint32_t v = 0; v < a->n_views; ++v) {
+            double h = 0.0;
+            for (uint32_t k = 0; k < mods; ++k)
+                h += record_hits(projdata + 20u * (v * mods + k), lo, hi, vox, n, double(a->line_offset));
+            worst = std::fmax(worst, h);
+        }
+    }
+    *hits = worst;
+    return XPR_OK;
+}
 
 extern "C" int xpr_geometry(const xpr_geometry_params *p, float *projdata_out,
                             xpr_grid *grid_out)

@@ -197,6 +197,7 @@ Parameters parameters_from_json(const Json &root)
         rdn(x, "threads", q.threads, "xpr");
         rd(x, "run_log", q.run_log, "xpr");
         rd(x, "autotune", q.autotune, "xpr");
+        rd(x, "narrow_sums", q.narrow_sums, "xpr");
         rd(x, "quiet", q.quiet, "xpr");
         rdn(x, "device", q.device, "xpr");
     }
@@ -278,6 +279,7 @@ Json parameters_to_json(const Parameters &p)
     x.set("threads", Json::number(p.xpr.threads));
     x.set("run_log", Json::string(p.xpr.run_log));
     x.set("autotune", Json::boolean(p.xpr.autotune));
+    x.set("narrow_sums", Json::boolean(p.xpr.narrow_sums));
     x.set("quiet", Json::boolean(p.xpr.quiet));
     x.set("device", Json::number(p.xpr.device));
     root.set("xpr", x);

@@ -462,17 +462,18 @@ void Operators::setVolumeParameters(std::array<int, 3> mn, std::array<int, 3> mx
 
 void Operators::forward()
 {
-    check(xpr_forward(stream_, vol_, vol_dt_, sums_, acc_, pd_, &a_), "forward");
+    check(xpr_forward(stream_, vol_, vol_dt_, sums_, narrow_ ? XPR_ACC_U32 : acc_, pd_, &a_), "forward");
 }
 
 void Operators::error()
 {
-    check(xpr_error(stream_, img_, img_dt_, sums_, acc_, err_, err_dt_, elements_, w_), "error");
+    check(xpr_error(stream_, img_, img_dt_, sums_, narrow_ ? XPR_ACC_U32 : acc_, err_, err_dt_, elements_, w_),
+          "error");
 }
 
 void Operators::backward()
 {
-    check(xpr_backward(stream_, vol_, vol_dt_, err_, err_dt_, pd_, &a_, nullptr), "backward");
+    check(xpr_backward_max(stream_, vol_, vol_dt_, err_, err_dt_, pd_, &a_, nullptr, max_bits_), "backward");
 }
 
 // ---------------------------------------------------------------------------
@@ -655,6 +656,86 @@ void fill_sums(xpr_stream s, DevBuf &sums, int acc, bool one, uint64_t n)
     else check(xpr_fill_u32(s, sums.as<uint32_t>(), one ? 1u << 22 : 0u, n), "fill");
 }
 
+// 32-bit detector sums in the forwards over a resident volume whenever every
+// sum provably stays below 2^32 (xpr.narrow_sums): the launch's hit bound
+// (xpr_hit_bound over the sub-iteration's views, for any jitter) times the
+// largest Q.22 voxel value, which the previous backward reports
+// (xpr_backward_max).  Such sums are the same integers as 64-bit ones, so no
+// result changes; the 32-bit forward is the lighter kernel.
+class NarrowSums {
+public:
+    // hits: the bound of each sub-iteration's whole-volume forward
+    void init(bool enable, int acc, int storage, std::vector<double> hits)
+    {
+        on_ = enable && acc == XPR_ACC_U64;
+        if (!on_) return;
+        hits_ = std::move(hits);
+        // a 16-bit store rounds the reported value up by less than this factor
+        margin_ = storage == XPR_F32 ? 1.0 : 1.0 + 1.0 / 128.0;
+        top_.alloc(4 * XPR_MAX_SLOTS);
+        top_host_.alloc(4 * XPR_MAX_SLOTS);
+    }
+    // before the forward of sub-iteration s: whether its sums may be 32-bit
+    bool choose(int64_t s)
+    {
+        bool narrow = false;
+        if (on_ && known_) {
+            const uint32_t *slot = top_host_.as<uint32_t>();
+            const uint32_t bits = *std::max_element(slot, slot + XPR_MAX_SLOTS);
+            if (bits <= 0x7f7fffffu) {                 // finite and not negative
+                float v;
+                std::memcpy(&v, &bits, 4);
+                const double q = std::floor(double(v) * margin_ * 4194304.0);
+                narrow = hits_[size_t(s)] * q <= 4294967295.0;
+            }
+        }
+        bits_.arr.push_back(Json::number(narrow ? 32 : 64));
+        return narrow;
+    }
+    // around each backward (enqueued on its stream): clear the device maximum's
+    // slots, then copy them to pinned host memory, read after the caller's
+    // next sync
+    uint32_t *before_backward(xpr_stream st)
+    {
+        if (!on_) return nullptr;
+        check(xpr_fill_u32(st, top_.as<uint32_t>(), 0u, XPR_MAX_SLOTS), "fill");
+        return top_.as<uint32_t>();
+    }
+    void after_backward(xpr_stream st)
+    {
+        if (on_) check(xpr_download(st, top_host_.get(), top_.get(), 4 * XPR_MAX_SLOTS), "download");
+    }
+    void synced() { known_ = on_; }
+    bool on() const { return on_; }
+    const Json &bits() const { return bits_; }
+    double max_hits() const
+    {
+        double m = 0.0;
+        for (double h : hits_) m = std::max(m, h);
+        return m;
+    }
+
+private:
+    bool on_ = false, known_ = false;
+    std::vector<double> hits_;
+    double margin_ = 1.0;
+    DevBuf top_;
+    HostBuf top_host_;
+    Json bits_ = Json::array();
+};
+
+// the bound of a whole-volume forward of each sub-iteration's views (grid
+// origin: the bound holds for any jitter)
+std::vector<double> whole_volume_hits(const std::vector<std::vector<float>> &pd_host, xpr_proj_args a,
+                                      const std::array<float, 3> &orig)
+{
+    std::vector<double> h(pd_host.size(), HUGE_VAL);
+    for (int i = 0; i < 3; ++i) a.origin[i] = orig[size_t(i)];
+    a.view_index = nullptr;
+    for (size_t s = 0; s < pd_host.size(); ++s) check(xpr_hit_bound(pd_host[s].data(), &a, &h[s]), "hit bound");
+    return h;
+}
+
 } // namespace
 
 float Reconstruction::next_weight(int64_t mit) const
@@ -747,6 +828,9 @@ public:
         }
         fill_sums(stream_, sums, p_.xpr.accumulator, true, uint64_t(As * P));
         upload(0);
+        NarrowSums narrow;
+        narrow.init(p_.xpr.narrow_sums, p_.xpr.accumulator, st,
+                    p_.xpr.narrow_sums ? whole_volume_hits(pd_host, ops.args(), ds.orig()) : std::vector<double>());
         int vdt = st;
         Event e0, e1, fb, fe, bb, be;
         note("launches_per_pass", Json::number(1));
@@ -769,19 +853,23 @@ public:
                     vdt = XPR_F32;
                 }
                 const bool fwd = s > 0 || mit > 0;
+                ops.setNarrowSums(fwd && narrow.choose(s));
                 if (fwd) {
                     check(xpr_event_record(fb.e, stream_), "event");
                     ops.forward();
                     check(xpr_event_record(fe.e, stream_), "event");
                 }
                 ops.error();
+                ops.setMaxBuffer(narrow.before_backward(stream_));
                 check(xpr_event_record(bb.e, stream_), "event");
                 ops.backward();
                 check(xpr_event_record(be.e, stream_), "event");
+                narrow.after_backward(stream_);
                 fill_sums(stream_, sums, p_.xpr.accumulator, false, uint64_t(As * P));
                 check(xpr_event_record(e1.e, stream_), "event");
                 if (mit == 0 && s + 1 < S) upload(s + 1);
                 check(xpr_event_sync(e1.e), "sync");
+                narrow.synced();
                 float ms = 0, fms = -1.0f, bms = 0;
                 check(xpr_event_elapsed_ms(e0.e, e1.e, &ms), "elapsed");
                 if (fwd) check(xpr_event_elapsed_ms(fb.e, fe.e, &fms), "elapsed");
@@ -789,6 +877,8 @@ public:
                 log_sit(mit, s, ms, fms, bms);
             }
         }
+        note("forward_sum_bits", narrow.bits());
+        if (narrow.on()) note("hit_bound", Json::number(narrow.max_hits()));
         const DevBuf &result = vdt == XPR_F32 && final32 ? vol32 : vol;
         write_final(ds, stream_, result.get(), vdt, 0, H, stage);
     }
@@ -833,8 +923,10 @@ public:
         pds_.resize(size_t(S));
         cviews_.clear();
         cviews_.resize(size_t(S));
+        std::vector<std::vector<float>> pd_whole(whole_ ? size_t(S) : 0);
         for (int64_t s = 0; s < S; ++s) {
             const auto pd = ds.sit_projdata(s);
+            if (whole_) pd_whole[size_t(s)] = pd;
             pds_[size_t(s)].alloc(pd.size() * 4);
             check(xpr_upload(stream_, pds_[size_t(s)].get(), pd.data(), pd.size() * 4), "upload");
             if (!cull_) continue;
@@ -921,6 +1013,18 @@ public:
         };
         if (resident_) upload(0);
         note("launches_per_pass", Json::number(whole_ ? 1.0 : double(nslab_)));
+        // 32-bit sums where provably exact: whole-volume passes only
+        NarrowSums narrow;
+        {
+            xpr_proj_args wa = ops.args();
+            wa.vmin[0] = wa.vmin[1] = wa.vmin[2] = 0;
+            wa.vmax[0] = int32_t(W);
+            wa.vmax[1] = int32_t(H);
+            wa.vmax[2] = int32_t(W);
+            for (int i = 0; i < 3; ++i) wa.voxel[i] = ds.vx();
+            const bool on = whole_ && p_.xpr.narrow_sums;
+            narrow.init(on, p_.xpr.accumulator, st_, on ? whole_volume_hits(pd_whole, wa, ds.orig()) : std::vector<double>());
+        }
         Event e0, e1, fb, fe, bb, be;
         for (int64_t mit = 0; mit < p_.recon.it; ++mit) {
             ops.setWeight(next_weight(mit));
@@ -947,6 +1051,7 @@ public:
                     ops.setImagesBuffer(img32.get(), XPR_F32);
                     ops.setErrorBuffer(err32.get(), XPR_F32);
                 }
+                ops.setNarrowSums(!first && narrow.choose(s));
                 if (!first) {
                     check(xpr_event_record(fb.e, stream_), "event");
                     pass(false, false, false, s, ops);
@@ -956,13 +1061,16 @@ public:
                 ops.setViewIndex(nullptr);
                 ops.setVolumeParameters({0, 0, 0}, {int(W), int(H), int(W)}, ds.vx());
                 ops.error();
+                ops.setMaxBuffer(narrow.before_backward(stream_));
                 check(xpr_event_record(bb.e, stream_), "event");
                 pass(true, first, last, s, ops);
                 check(xpr_event_record(be.e, stream_), "event");
+                narrow.after_backward(stream_);
                 fill_sums(stream_, sums, p_.xpr.accumulator, false, uint64_t(As * P));
                 check(xpr_event_record(e1.e, stream_), "event");
                 if (resident_ && mit == 0 && s + 1 < S) upload(s + 1);
                 check(xpr_event_sync(e1.e), "sync");
+                narrow.synced();
                 float ms = 0, fms = -1.0f, bms = 0;
                 check(xpr_event_elapsed_ms(e0.e, e1.e, &ms), "elapsed");
                 if (!first) check(xpr_event_elapsed_ms(fb.e, fe.e, &fms), "elapsed");
@@ -970,6 +1078,8 @@ public:
                 log_sit(mit, s, ms, fms, bms);
             }
         }
+        note("forward_sum_bits", narrow.bits());
+        if (narrow.on()) note("hit_bound", Json::number(narrow.max_hits()));
         if (store_ == kDevice) write_final(ds, stream_, cvol_, cst_, 0, H, stage_);
         note("slab_h2d_bytes", Json::number(double(h2d_bytes_)));
         note("slab_d2h_bytes", Json::number(double(d2h_bytes_)));

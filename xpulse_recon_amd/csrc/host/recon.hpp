@@ -185,6 +185,12 @@ public:
     void setImageOffset(int offset, int rows) { a_.line_offset = offset; a_.line_rows = rows; }
     void setImageElements(int64_t n) { elements_ = uint64_t(n); }
     void setViewIndex(const uint32_t *idx) { a_.view_index = idx; }
+    // 32-bit detector sums for the next forward and error (only in place of
+    // 64-bit ones, and only when a bound proves every sum below 2^32: the
+    // sums are then the same integers)
+    void setNarrowSums(bool narrow) { narrow_ = narrow && acc_ == XPR_ACC_U64; }
+    // device uint32 the backward raises to its largest voxel value's bits (or NULL)
+    void setMaxBuffer(uint32_t *max_bits) { max_bits_ = max_bits; }
     void forward();
     void error();
     void backward();
@@ -195,6 +201,8 @@ public:
 private:
     xpr_stream stream_;
     int acc_;
+    bool narrow_ = false;
+    uint32_t *max_bits_ = nullptr;
     const void *img_ = nullptr;
     void *sums_ = nullptr, *err_ = nullptr, *vol_ = nullptr;
     int img_dt_ = XPR_F32, err_dt_ = XPR_F32, vol_dt_ = XPR_F32;

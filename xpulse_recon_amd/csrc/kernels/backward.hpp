@@ -179,7 +179,9 @@ __device__ __forceinline__ float bilinear_sample(__amdgpu_buffer_rsrc_t rsrc, fl
 // table as broadcast loads straight into VGPRs; LDS operations of a wave run
 // in order, so the table needs no barrier.  Every branch of a view leaves its
 // samples in smp[] and one accumulation follows, in view order, so the sums
-// equal the canonical ones.
+// equal the canonical ones.  max_bits (optional, XPR_MAX_SLOTS entries): the
+// slot of this wave is raised to the bits of the row's largest voxel value
+// after the update (updated or kept; before a 16-bit store rounds it).
 template <typename VS, typename ES, int J, int L, bool CROP, bool BILIN = false>
 __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
                                              const typename ES::T *__restrict__ err,
@@ -187,7 +189,8 @@ __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
                                              const xpr_proj_args &a, const SlabShape &s,
                                              int64_t row, int seg,
                                              unsigned long long *__restrict__ empty_count,
-                                             float4 *__restrict__ rec)
+                                             float4 *__restrict__ rec,
+                                             uint32_t *__restrict__ max_bits = nullptr)
 {
     using R = BwdRecord<L>;
     const int lane = threadIdx.x & 63;
@@ -405,7 +408,8 @@ __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
             }
         }
     }
-    uint32_t empty = 0;
+    uint32_t empty = 0, top = 0;           // top: the largest value's bits (max_bits)
+    const bool track = max_bits != nullptr;
 #pragma unroll
     for (int l = 0; l < L; ++l) {
         if (l >= nlv) break;                          // uniform
@@ -417,9 +421,12 @@ __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
             if (x < s.nx) {
                 if (n == 0) {
                     ++empty;              // the reference computes exp(0/0) here
+                    if (track) top = max(top, __float_as_uint(VS::load(vol, rowoff + x)));
                 } else {
                     const float v = VS::load(vol, rowoff + x);
-                    VS::store(vol, rowoff + x, v * expf(sum[j][l] / (float)n));
+                    const float nv = v * expf(sum[j][l] / (float)n);
+                    VS::store(vol, rowoff + x, nv);
+                    top = max(top, __float_as_uint(nv));
                 }
             }
         }
@@ -428,6 +435,12 @@ __device__ __forceinline__ void backward_row(typename VS::T *__restrict__ vol,
         uint32_t e = empty;
         for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off);
         if (lane == 0 && e) atomicAdd(empty_count, (unsigned long long)e);
+    }
+    if (track) {
+        for (int off = 32; off > 0; off >>= 1) top = max(top, (uint32_t)__shfl_xor((int)top, off));
+        static_assert((XPR_MAX_SLOTS & (XPR_MAX_SLOTS - 1)) == 0, "slots: a power of two");
+        const uint32_t slot = (uint32_t)row & (XPR_MAX_SLOTS - 1u);
+        if (lane == 0) atomicMax(max_bits + slot, top);
     }
 }
 
@@ -441,7 +454,7 @@ template <typename VS, typename ES, int J, int L, bool CROP, bool BILIN = false>
 __global__ __launch_bounds__(kThreads) void k_backward(
     typename VS::T *__restrict__ vol, const typename ES::T *__restrict__ err,
     const float *__restrict__ pd, xpr_proj_args a, SlabShape s,
-    unsigned long long *__restrict__ empty_count)
+    unsigned long long *__restrict__ empty_count, uint32_t *__restrict__ max_bits)
 {
     __shared__ float4 rec[kWaves * 64 * BwdRecord<L>::kQuads];   // per-wave view-record tables
     // segment blockIdx % nseg of the kWaves rows of row block blockIdx / nseg
@@ -449,7 +462,8 @@ __global__ __launch_bounds__(kThreads) void k_backward(
     const int64_t row = s.row_base + (int64_t)(blockIdx.x / s.nseg) * kWaves + wave;
     backward_row<VS, ES, J, L, CROP, BILIN>(vol, err, pd, a, s, row, (int)(blockIdx.x % s.nseg),
                                             empty_count,
-                                            rec + (threadIdx.x >> 6) * 64 * BwdRecord<L>::kQuads);
+                                            rec + (threadIdx.x >> 6) * 64 * BwdRecord<L>::kQuads,
+                                            max_bits);
 }
 
 } // namespace

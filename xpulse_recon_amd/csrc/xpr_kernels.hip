@@ -18,6 +18,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cstddef>
 #include <cstdint>
 #include <cstdio>
@@ -242,25 +244,28 @@ int backward_default_j(int32_t nx) { return nx <= 128 ? kBwdL : kBwdJ * kBwdL; }
 
 template <typename VS, typename ES, int J, int L, bool BILIN = false>
 int backward_launch(hipStream_t st, void *vol, const void *err, const float *pd,
-                    const xpr_proj_args &a, SlabShape s, uint64_t *empty)
+                    const xpr_proj_args &a, SlabShape s, uint64_t *empty, uint32_t *max_bits)
 {
     const bool crop = a.line_offset != 0;
     return launch_rows(s, J, L, [&](unsigned blocks, const SlabShape &t) {
         if (crop)
             hipLaunchKernelGGL((k_backward<VS, ES, J, L, true, BILIN>), dim3(blocks),
                                dim3(kThreads), 0, st, (typename VS::T *)vol,
-                               (const typename ES::T *)err, pd, a, t, (unsigned long long *)empty);
+                               (const typename ES::T *)err, pd, a, t, (unsigned long long *)empty,
+                               max_bits);
         else
             hipLaunchKernelGGL((k_backward<VS, ES, J, L, false, BILIN>), dim3(blocks),
                                dim3(kThreads), 0, st, (typename VS::T *)vol,
-                               (const typename ES::T *)err, pd, a, t, (unsigned long long *)empty);
+                               (const typename ES::T *)err, pd, a, t, (unsigned long long *)empty,
+                               max_bits);
     });
 }
 
 // J < 0: the table's choice for this shape, else the default
 template <typename VS, typename ES>
 int backward_impl(hipStream_t st, void *vol, const void *err, const float *pd,
-                  const xpr_proj_args &a, SlabShape s, uint64_t *empty, int J = -1)
+                  const xpr_proj_args &a, SlabShape s, uint64_t *empty, int J = -1,
+                  uint32_t *max_bits = nullptr)
 {
     if ((uint64_t)a.det_width * (uint64_t)a.line_rows * sizeof(typename ES::T) >= 0x80000000ull)
         return fail(XPR_E_INVALID, "a detector view above 2 GiB");
@@ -268,7 +273,7 @@ int backward_impl(hipStream_t st, void *vol, const void *err, const float *pd,
     if (const char *e = std::getenv("XPR_BACKWARD_INSIDE")) s.inside = std::atoi(e) != 0;
     // bilinear sampling (opt-in): the default shape only
     if (a.interp == XPR_INTERP_BILINEAR)
-        return backward_launch<VS, ES, kBwdJ, kBwdL, true>(st, vol, err, pd, a, s, empty);
+        return backward_launch<VS, ES, kBwdJ, kBwdL, true>(st, vol, err, pd, a, s, empty, max_bits);
     if constexpr (std::is_same<VS, ES>::value) {       // the schedules' dtype pairs: all J compiled
         if (J < 0) {
             // tuning knob: force the voxels per lane
@@ -277,14 +282,14 @@ int backward_impl(hipStream_t st, void *vol, const void *err, const float *pd,
         }
         if (J < 0) J = backward_default_j(s.nx);
         switch (J) {
-        case 4: return backward_launch<VS, ES, 1, 4>(st, vol, err, pd, a, s, empty);
-        case 8: return backward_launch<VS, ES, 2, 4>(st, vol, err, pd, a, s, empty);
-        case 16: return backward_launch<VS, ES, 4, 4>(st, vol, err, pd, a, s, empty);
-        default: return backward_launch<VS, ES, kBwdJ, kBwdL>(st, vol, err, pd, a, s, empty);
+        case 4: return backward_launch<VS, ES, 1, 4>(st, vol, err, pd, a, s, empty, max_bits);
+        case 8: return backward_launch<VS, ES, 2, 4>(st, vol, err, pd, a, s, empty, max_bits);
+        case 16: return backward_launch<VS, ES, 4, 4>(st, vol, err, pd, a, s, empty, max_bits);
+        default: return backward_launch<VS, ES, kBwdJ, kBwdL>(st, vol, err, pd, a, s, empty, max_bits);
         }
     } else {
         (void)J;
-        return backward_launch<VS, ES, kBwdJ, kBwdL>(st, vol, err, pd, a, s, empty);
+        return backward_launch<VS, ES, kBwdJ, kBwdL>(st, vol, err, pd, a, s, empty, max_bits);
     }
 }
 
@@ -843,6 +848,13 @@ int xpr_error(xpr_stream s, const void *images, int img_dtype, const void *sums,
 int xpr_backward(xpr_stream s, void *volume, int vol_dtype, const void *err, int err_dtype,
                  const float *projdata, const xpr_proj_args *a, uint64_t *empty_count)
 {
+    return xpr_backward_max(s, volume, vol_dtype, err, err_dtype, projdata, a, empty_count, nullptr);
+}
+
+int xpr_backward_max(xpr_stream s, void *volume, int vol_dtype, const void *err, int err_dtype,
+                     const float *projdata, const xpr_proj_args *a, uint64_t *empty_count,
+                     uint32_t *max_bits)
+{
     SlabShape sh;
     if (int rc = check_args(a, sh)) return rc;
     if (int rc = check_dtype(vol_dtype)) return rc;
@@ -850,7 +862,7 @@ int xpr_backward(xpr_stream s, void *volume, int vol_dtype, const void *err, int
     if (sh.nx == 0 || sh.nrows == 0) return XPR_OK;
     if (!volume || !err || !projdata) return fail(XPR_E_INVALID, "null buffer");
     hipStream_t st = (hipStream_t)s;
-#define XPR_BW(VS, ES) backward_impl<VS, ES>(st, volume, err, projdata, *a, sh, empty_count)
+#define XPR_BW(VS, ES) backward_impl<VS, ES>(st, volume, err, projdata, *a, sh, empty_count, -1, max_bits)
     switch (vol_dtype * 3 + err_dtype) {
     case XPR_F32 * 3 + XPR_F32: return XPR_BW(F32, F32);
     case XPR_F32 * 3 + XPR_F16: return XPR_BW(F32, F16);
@@ -1013,9 +1025,9 @@ int xpr_check_fast_paths(xpr_stream s, uint64_t mismatches[3])
     return rc;
 }
 
-int xpr_valu_ceiling(xpr_stream s, double *lane_instr_per_s)
+int xpr_valu_rates(xpr_stream s, double *rates, int n_rates)
 {
-    if (!lane_instr_per_s) return fail(XPR_E_INVALID, "null output");
+    if (!rates || n_rates < 1) return fail(XPR_E_INVALID, "null output");
     hipStream_t st = (hipStream_t)s;
     int dev = 0;
     hipDeviceProp_t prop;
@@ -1028,38 +1040,47 @@ int xpr_valu_ceiling(xpr_stream s, double *lane_instr_per_s)
     XPR_HIP_CHECK(hipMallocAsync((void **)&out, (size_t)blocks * 256 * sizeof(float), st));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = XPR_OK;
-    float best = 3.4e38f;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    const int kinds = std::min(n_rates, kValuKinds);
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         rc = fail(XPR_E_HIP, "event");
-    } else {
-        // per kind: one warm-up, then the best of five
-        for (int r = 0; r < 18 && rc == XPR_OK; ++r) {
+    const double lane_instr = (double)blocks * 256.0 * (double)iters * (double)kValuChains;
+    for (int k = 0; k < kinds && rc == XPR_OK; ++k) {
+        float best = 3.4e38f;
+        // one warm-up, then the best of five
+        for (int r = 0; r < 6 && rc == XPR_OK; ++r) {
             (void)hipEventRecord(e0, st);
-            if (r < 6)
-                hipLaunchKernelGGL(k_valu_ceiling<0>, dim3(blocks), dim3(256), 0, st, out, iters,
-                                   0.999f);
-            else if (r < 12)
-                hipLaunchKernelGGL(k_valu_ceiling<1>, dim3(blocks), dim3(256), 0, st, out, iters,
-                                   0.999f);
-            else
-                hipLaunchKernelGGL(k_valu_ceiling<2>, dim3(blocks), dim3(256), 0, st, out, iters,
-                                   0.999f);
+            switch (k) {
+            case 0: hipLaunchKernelGGL(k_valu_ceiling<0>, dim3(blocks), dim3(256), 0, st, out, iters, 0.999f); break;
+            case 1: hipLaunchKernelGGL(k_valu_ceiling<1>, dim3(blocks), dim3(256), 0, st, out, iters, 0.999f); break;
+            case 2: hipLaunchKernelGGL(k_valu_ceiling<2>, dim3(blocks), dim3(256), 0, st, out, iters, 0.999f); break;
+            case 3: hipLaunchKernelGGL(k_valu_ceiling<3>, dim3(blocks), dim3(256), 0, st, out, iters, 0.999f); break;
+            case 4: hipLaunchKernelGGL(k_valu_ceiling<4>, dim3(blocks), dim3(256), 0, st, out, iters, 0.999f); break;
+            default: hipLaunchKernelGGL(k_valu_ceiling<5>, dim3(blocks), dim3(256), 0, st, out, iters, 0.999f); break;
+            }
             (void)hipEventRecord(e1, st);
             float ms = 0.0f;
             if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                 hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
                 rc = fail(XPR_E_HIP, "VALU ceiling launch");
-            else if (r % 6 != 0 && ms < best)
+            else if (r != 0 && ms < best)
                 best = ms;
         }
+        if (rc == XPR_OK) rates[k] = lane_instr / ((double)best * 1e-3);
     }
+    for (int k = kinds; k < n_rates; ++k) rates[k] = 0.0;
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     (void)hipFreeAsync(out, st);
     (void)hipStreamSynchronize(st);
-    if (rc != XPR_OK) return rc;
-    const double lane_instr = (double)blocks * 256.0 * (double)iters * (double)kValuChains;
-    *lane_instr_per_s = lane_instr / ((double)best * 1e-3);
+    return rc;
+}
+
+int xpr_valu_ceiling(xpr_stream s, double *lane_instr_per_s)
+{
+    if (!lane_instr_per_s) return fail(XPR_E_INVALID, "null output");
+    double rates[kValuKinds];
+    if (int rc = xpr_valu_rates(s, rates, kValuKinds)) return rc;
+    *lane_instr_per_s = *std::max_element(rates, rates + kValuKinds);
     return XPR_OK;
 }
 
