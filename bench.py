@@ -491,6 +491,8 @@ def ooc_main(args):
            "device_s": round(dev_ms / 1e3, 1), "voxel_views": vv,
            "chunks": record.get("chunks"), "rmse_vs_phantom": (sq / n) ** 0.5, "pcie": pcie,
            "maxabs_vs_phantom": mx, "finite": finite,
+           "forward_sum_bits": {str(b): (record.get("forward_sum_bits") or []).count(b) for b in (32, 64)},
+           "hit_bound": record.get("hit_bound"),
            "data": "synthetic: 40 random ellipsoids + 200 spheres r 2-6 voxels mu 0.05/voxel, "
                    "FP64 line integrals, Poisson I0=5e4",
            "config": {"workload": f"{W}^2x{H} grid, {N} views of {D}x{D}, it={IT} sit={S}, "

@@ -136,11 +136,13 @@ def test_streaming_schedules_equal_naive(scan, method, xpr):
     ("naive", {}),
     ("naive", {"storage": "f16"}),
     ("method_1a", {"layer_store": "device", "slab_height": 7}),
+    ("method_2", {"device_budget_gb": 0.30 * (W * W * H * 4 + 6 * W * H * 16) / 1e9}),
 ])
 def test_narrow_sums_equal_wide_sums(scan, method, xpr):
     """32-bit detector sums wherever the hit bound times the volume's largest
     Q.22 value stays below 2^32 (xpr.narrow_sums, the default) give the volume
-    of 64-bit sums bit for bit; the run record says which forwards took them."""
+    of 64-bit sums bit for bit; the run record says which forwards took them
+    (method_2: several chunks, each its own bound on its layers and rows)."""
     t, spec, d = scan
     tag = f"{method}_{xpr.get('storage', 'f32')}"
     wide, lw = _run(t, f"wide_{tag}", d, method=method, xpr=dict(xpr, narrow_sums=False))

@@ -664,16 +664,21 @@ void fill_sums(xpr_stream s, DevBuf &sums, int acc, bool one, uint64_t n)
 // result changes; the 32-bit forward is the lighter kernel.
 class NarrowSums {
 public:
-    // hits: the bound of each sub-iteration's whole-volume forward
+    // hits: the bound of each sub-iteration's forward over the resident
+    // volume; called again for each new volume (method_2's chunks: the
+    // maximum is then unknown until the chunk's first backward)
     void init(bool enable, int acc, int storage, std::vector<double> hits)
     {
         on_ = enable && acc == XPR_ACC_U64;
+        known_ = false;
         if (!on_) return;
         hits_ = std::move(hits);
         // a 16-bit store rounds the reported value up by less than this factor
         margin_ = storage == XPR_F32 ? 1.0 : 1.0 + 1.0 / 128.0;
-        top_.alloc(4 * XPR_MAX_SLOTS);
-        top_host_.alloc(4 * XPR_MAX_SLOTS);
+        if (!top_.get()) {
+            top_.alloc(4 * XPR_MAX_SLOTS);
+            top_host_.alloc(4 * XPR_MAX_SLOTS);
+        }
     }
     // before the forward of sub-iteration s: whether its sums may be 32-bit
     bool choose(int64_t s)
@@ -724,10 +729,10 @@ private:
     Json bits_ = Json::array();
 };
 
-// the bound of a whole-volume forward of each sub-iteration's views (grid
-// origin: the bound holds for any jitter)
-std::vector<double> whole_volume_hits(const std::vector<std::vector<float>> &pd_host, xpr_proj_args a,
-                                      const std::array<float, 3> &orig)
+// the bound of a forward of args a (volume range, detector rows) over each
+// sub-iteration's views (grid origin: the bound holds for any jitter)
+std::vector<double> launch_hits(const std::vector<std::vector<float>> &pd_host, xpr_proj_args a,
+                                const std::array<float, 3> &orig)
 {
     std::vector<double> h(pd_host.size(), HUGE_VAL);
     for (int i = 0; i < 3; ++i) a.origin[i] = orig[size_t(i)];
@@ -830,7 +835,7 @@ public:
         upload(0);
         NarrowSums narrow;
         narrow.init(p_.xpr.narrow_sums, p_.xpr.accumulator, st,
-                    p_.xpr.narrow_sums ? whole_volume_hits(pd_host, ops.args(), ds.orig()) : std::vector<double>());
+                    p_.xpr.narrow_sums ? launch_hits(pd_host, ops.args(), ds.orig()) : std::vector<double>());
         int vdt = st;
         Event e0, e1, fb, fe, bb, be;
         note("launches_per_pass", Json::number(1));
@@ -1023,7 +1028,7 @@ public:
             wa.vmax[2] = int32_t(W);
             for (int i = 0; i < 3; ++i) wa.voxel[i] = ds.vx();
             const bool on = whole_ && p_.xpr.narrow_sums;
-            narrow.init(on, p_.xpr.accumulator, st_, on ? whole_volume_hits(pd_whole, wa, ds.orig()) : std::vector<double>());
+            narrow.init(on, p_.xpr.accumulator, st_, on ? launch_hits(pd_whole, wa, ds.orig()) : std::vector<double>());
         }
         Event e0, e1, fb, fe, bb, be;
         for (int64_t mit = 0; mit < p_.recon.it; ++mit) {
@@ -1549,13 +1554,17 @@ public:
         const size_t layer = size_t(W * W);
         DevBuf stage;
         std::vector<DevBuf> pds(static_cast<size_t>(S));
+        std::vector<std::vector<float>> pd_host(static_cast<size_t>(S));
         for (int64_t s = 0; s < S; ++s) {
-            auto pd = ds.sit_projdata(s);
+            pd_host[size_t(s)] = ds.sit_projdata(s);
+            const auto &pd = pd_host[size_t(s)];
             pds[size_t(s)].alloc(pd.size() * 4);
             check(xpr_upload(stream_, pds[size_t(s)].get(), pd.data(), pd.size() * 4), "upload");
         }
         Event e0, e1, fb, fe, bb, be;
         note("launches_per_pass", Json::number(1));
+        NarrowSums narrow;             // 32-bit sums where provably exact, per chunk
+        double max_hits = 0.0;
         for (const Chunk &c : chunks_) {
             if (!p_.xpr.quiet) std::cout << "[" << c.vOffset << "-" << c.vOffset + c.vSize << "]... " << std::flush;
             const int64_t l0 = c.vOffset - c.vPadTop, l1 = c.vOffset + c.vSize + c.vPadBot;
@@ -1573,6 +1582,9 @@ public:
             ops.setImageOffset(int(c.iOffset), int(c.iSize));
             ops.setAngleNumber(As, p_.detector.module_number);
             ops.setVolumeParameters({0, int(l0), 0}, {int(W), int(l1), int(W)}, ds.vx());
+            narrow.init(p_.xpr.narrow_sums, p_.xpr.accumulator, st,
+                        p_.xpr.narrow_sums ? launch_hits(pd_host, ops.args(), ds.orig()) : std::vector<double>());
+            if (narrow.on()) max_hits = std::max(max_hits, narrow.max_hits());
             for (int64_t mit = 0; mit < p_.recon.it; ++mit) {
                 ops.setWeight(next_weight(mit));
                 for (int64_t s = 0; s < S; ++s) {
@@ -1581,18 +1593,22 @@ public:
                     ops.setOrigin(next_origin());
                     ops.setProjDataBuffer(pds[size_t(s)].as<float>());
                     const bool fwd = s > 0 || mit > 0;
+                    ops.setNarrowSums(fwd && narrow.choose(s));
                     if (fwd) {
                         check(xpr_event_record(fb.e, stream_), "event");
                         ops.forward();
                         check(xpr_event_record(fe.e, stream_), "event");
                     }
                     ops.error();
+                    ops.setMaxBuffer(narrow.before_backward(stream_));
                     check(xpr_event_record(bb.e, stream_), "event");
                     ops.backward();
                     check(xpr_event_record(be.e, stream_), "event");
+                    narrow.after_backward(stream_);
                     fill_sums(stream_, sums, p_.xpr.accumulator, false, npx);
                     check(xpr_event_record(e1.e, stream_), "event");
                     check(xpr_event_sync(e1.e), "sync");
+                    narrow.synced();
                     float ms = 0, fms = -1.0f, bms = 0;
                     check(xpr_event_elapsed_ms(e0.e, e1.e, &ms), "elapsed");
                     if (fwd) check(xpr_event_elapsed_ms(fb.e, fe.e, &fms), "elapsed");
@@ -1606,6 +1622,8 @@ public:
                         st, c.vOffset, c.vOffset + c.vSize, stage);
             if (!p_.xpr.quiet) std::cout << "Done." << std::endl;
         }
+        note("forward_sum_bits", narrow.bits());
+        if (p_.xpr.narrow_sums && p_.xpr.accumulator == XPR_ACC_U64) note("hit_bound", Json::number(max_hits));
     }
 
 private:
