@@ -80,8 +80,8 @@ struct XprSection {
     int threads = 0;                 // host IO threads (0: hardware concurrency, max 16)
     std::string run_log = "";        // JSON run record (jitter, timings, memory)
     bool autotune = true;            // time the projectors' tile shapes once per launch shape
-    bool narrow_sums = true;         // 32-bit detector sums in a whole-volume forward when a bound
-                                     // proves every sum below 2^32 (the same integers)
+    bool narrow_sums = true;         // 32-bit detector sums in a forward over a resident volume when
+                                     // a bound proves every sum below 2^32 (the same integers)
     int device = 0;
     // in-process jobs (not JSON): projections from memory, volume to memory
     const float *mem_projections = nullptr;   // [n][h][w] raw images
