@@ -235,22 +235,25 @@ def test_preprocess_and_minmax_parity(cuda):
     mn, mx = O.minmax_normal(x)
     tx = torch.from_numpy(x).to(cuda)
     assert X.minmax_normal(tx) == (mn, mx)
-    for normalize in (False, True):
-        ref = O.preprocess(x.copy(), normalize, True, mn, mx)
-        out = torch.empty_like(tx)
-        X.preprocess(tx, out, normalize, True, mn, mx)
-        got = out.cpu().numpy()
-        fin = np.isfinite(ref)
-        np.testing.assert_array_equal(np.isnan(got), np.isnan(ref))
-        np.testing.assert_allclose(got[fin], ref[fin], rtol=4e-7, atol=1e-7)
+    # 70001 pixels take the one-pixel kernel, 70000 (aligned) four per access
+    for n in (70001, 70000):
+        for normalize in (False, True):
+            ref = O.preprocess(x[:n].copy(), normalize, True, mn, mx)
+            out = torch.empty(n, dtype=torch.float32, device=cuda)
+            X.preprocess(tx[:n], out, normalize, True, mn, mx)
+            got = out.cpu().numpy()
+            fin = np.isfinite(ref)
+            np.testing.assert_array_equal(np.isnan(got), np.isnan(ref))
+            np.testing.assert_allclose(got[fin], ref[fin], rtol=4e-7, atol=1e-7)
 
 
 @pytest.mark.parametrize("dtype", ["f32", "f16", "bf16"])
 def test_preprocess_views_equals_per_view(cuda, dtype):
     """The views s, s+S, ... of a projection stack preprocessed in one launch
     (xpr_preprocess_views) equal the same views preprocessed one call each,
-    bit for bit: 36 views of 768^2 (more views than the launch's grid rows,
-    so rows loop over views), step 20, normalised or not."""
+    bit for bit: 36 views of 768^2, step 20, normalised or not; and the
+    four-pixel kernel (aligned rows) equals the one-pixel kernel (a view read
+    4 bytes off alignment)."""
     import torch
     import xpulse_recon_amd as X
     tdt = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16}[dtype]
@@ -262,9 +265,15 @@ def test_preprocess_views_equals_per_view(cuda, dtype):
         ref = torch.empty_like(one)
         for j in range(36):
             X.preprocess(raw[7 + 20 * j], ref[j], normalize, True, -0.05, 1.2)
+        # one view through the one-pixel kernel: its input 4 bytes off alignment
+        shifted = torch.empty(768 * 768 + 1, device=cuda)
+        shifted[1:] = raw[7 + 20 * 5].reshape(-1)
+        scalar = torch.empty((768, 768), dtype=tdt, device=cuda)
+        X.preprocess(shifted[1:], scalar, normalize, True, -0.05, 1.2)
         torch.cuda.synchronize()
-        assert torch.equal(one.view(torch.int16 if dtype != "f32" else torch.int32),
-                           ref.view(torch.int16 if dtype != "f32" else torch.int32))
+        bits = torch.int16 if dtype != "f32" else torch.int32
+        assert torch.equal(one.view(bits), ref.view(bits))
+        assert torch.equal(scalar.view(bits), one[5].view(bits))
 
 
 def _bf16_bits(a):

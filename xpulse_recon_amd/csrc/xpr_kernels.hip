@@ -1147,6 +1147,36 @@ int xpr_minmax_read(const int32_t *minmax_host, float *mn, float *mx)
     return XPR_OK;
 }
 
+// n_views views of n pixels at in + j * in_stride, four pixels per access,
+// when the rows allow it (n and in_stride multiples of 4, in 16-byte
+// aligned); false otherwise (nothing launched)
+static bool preprocess_views4(hipStream_t st, const float *in, uint64_t in_stride, void *out,
+                              int out_dtype, uint64_t n, uint32_t n_views, int normalize, int mlog, float mn, float mx)
+{
+    if (n % 4 || in_stride % 4 || (uintptr_t)in % 16) return false;
+    const uint64_t n4 = n / 4;
+    const float4 *in4 = (const float4 *)in;
+    // x: half the float4s of a view (two per thread), y: the views (each row
+    // of the grid strides over them)
+    const unsigned gx = grid_for((n4 + 1) / 2);
+    const dim3 g(gx, std::min<uint32_t>(n_views, std::max<uint32_t>(1u, 65536u / gx)));
+    switch (out_dtype) {
+    case XPR_F32:
+        hipLaunchKernelGGL(k_preprocess_views4<F32>, g, dim3(256), 0, st, in4, in_stride / 4,
+                           (float *)out, n4, n_views, normalize, mlog, mn, mx);
+        break;
+    case XPR_F16:
+        hipLaunchKernelGGL(k_preprocess_views4<F16>, g, dim3(256), 0, st, in4, in_stride / 4,
+                           (uint16_t *)out, n4, n_views, normalize, mlog, mn, mx);
+        break;
+    default:
+        hipLaunchKernelGGL(k_preprocess_views4<BF16>, g, dim3(256), 0, st, in4, in_stride / 4,
+                           (uint16_t *)out, n4, n_views, normalize, mlog, mn, mx);
+        break;
+    }
+    return true;
+}
+
 int xpr_preprocess(xpr_stream s, const float *in, void *out, int out_dtype, uint64_t n,
                    int normalize, int mlog, float mn, float mx)
 {
@@ -1154,6 +1184,10 @@ int xpr_preprocess(xpr_stream s, const float *in, void *out, int out_dtype, uint
     if (n == 0) return XPR_OK;
     if (!in || !out) return fail(XPR_E_INVALID, "null buffer");
     hipStream_t st = (hipStream_t)s;
+    if (preprocess_views4(st, in, n, out, out_dtype, n, 1, normalize, mlog, mn, mx)) {
+        XPR_LAUNCH_CHECK();
+        return XPR_OK;
+    }
     const unsigned g = grid_for(n);
     switch (out_dtype) {
     case XPR_F32:
@@ -1182,6 +1216,10 @@ int xpr_preprocess_views(xpr_stream s, const float *in, uint64_t in_view_stride,
     if (!in || !out) return fail(XPR_E_INVALID, "null buffer");
     if (in_view_stride < n) return fail(XPR_E_INVALID, "views overlap: in_view_stride < n");
     hipStream_t st = (hipStream_t)s;
+    if (preprocess_views4(st, in, in_view_stride, out, out_dtype, n, n_views, normalize, mlog, mn, mx)) {
+        XPR_LAUNCH_CHECK();
+        return XPR_OK;
+    }
     // x covers a view, y the views (at most 64 rows of the grid; each row
     // strides over the views)
     const unsigned gx = grid_for(n);
