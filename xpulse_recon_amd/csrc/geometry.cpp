@@ -242,10 +242,12 @@ void build(const xpr_geometry_params &p, float *out, xpr_grid &grid, double *fra
 // at the corners and W^2 between its corner extremes.  With those rows
 // scaled by 1 / (1 + 2 du), 1 / (1 + 2 dv), diagonal dominance of either
 // pairing of rows and in-plane axes gives |J d|_inf >= mu |d|_inf for every
-// J of the intervals, hence |d|_inf <= 1 / mu: the voxels of one plane on one
-// pixel span at most floor(1 / mu) + 1 lattice values per in-plane axis.  A
-// record's bound is the least over the three plane families of
-// (planes) x (floor(1 / mu) + 1)^2; a view's is the sum over its modules.
+// J of the intervals, hence |d|_inf <= D = 1 / mu, and each row then bounds
+// its dominant axis's span: |J_ux| |d_x| <= 1 + |J_uy| D.  The voxels of one
+// plane on one pixel span at most floor(span) + 1 lattice values per
+// in-plane axis.  A record's bound is the least over the three plane
+// families and two pairings of (planes) x (product of the two counts); a
+// view's is the sum over its modules.
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -308,15 +310,25 @@ double record_hits(const float *m, const double (&lo)[3], const double (&hi)[3],
             g[r][a] = {q.lo * k, q.hi * k};
         }
     }
+    // voxels of one plane on one pixel when row p of the scaled gradient is
+    // dominated by in-plane axis x and row q by axis y: |d|inf <= D = 1 / mu,
+    // then each row alone, |J_px| |d_x| <= 1 + |J_py| |d_y| <= 1 + |J_py| D,
+    // bounds that axis's span (likewise y); floor(span) + 1 lattice values
+    auto plane_count = [&](const Interval &px, const Interval &py, const Interval &qx,
+                           const Interval &qy) {
+        const double mu = std::fmin(abs_lo(px) - abs_hi(py), abs_lo(qy) - abs_hi(qx)) * (1.0 - 1e-9);
+        if (!(mu > 0.0)) return inf;
+        const double D = 1.0 / mu;
+        const double sx = std::fmin(D, (1.0 + abs_hi(py) * D) / abs_lo(px) * (1.0 + 1e-9));
+        const double sy = std::fmin(D, (1.0 + abs_hi(qx) * D) / abs_lo(qy) * (1.0 + 1e-9));
+        return (std::floor(sx) + 1.0) * (std::floor(sy) + 1.0);
+    };
     double best = inf;
     for (int c = 0; c < 3; ++c) {
         const int a = (c + 1) % 3, b = (c + 2) % 3;
-        const double mu1 = std::fmin(abs_lo(g[0][a]) - abs_hi(g[0][b]), abs_lo(g[1][b]) - abs_hi(g[1][a]));
-        const double mu2 = std::fmin(abs_lo(g[0][b]) - abs_hi(g[0][a]), abs_lo(g[1][a]) - abs_hi(g[1][b]));
-        const double mu = std::fmax(mu1, mu2) * (1.0 - 1e-9);
-        if (!(mu > 0.0)) continue;
-        const double k = std::floor(1.0 / mu) + 1.0;
-        best = std::fmin(best, double(n[c]) * k * k);
+        const double k = std::fmin(plane_count(g[0][a], g[0][b], g[1][a], g[1][b]),   // u by a, v by b
+                                   plane_count(g[0][b], g[0][a], g[1][b], g[1][a]));  // u by b, v by a
+        best = std::fmin(best, double(n[c]) * k);
     }
     return best;
 }
